@@ -1,0 +1,55 @@
+// bssn_stage.h -- shared declarations between the host runtime and the
+// fused BSSN RK4 stage kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace cg {
+
+constexpr int NF = 18;  // evolved fields, reference order (bssn_macros.h:77-99)
+enum Field {
+  G11 = 0, G12, G13, G22, G23, G33, PHI, A11, A12, A13, A22, A23, A33, KK, GAM1, GAM2, GAM3, ALPHA
+};
+enum Matter { MATTER_NONE = 0, MATTER_STATIC = 1, MATTER_DUST = 2 };
+enum Lapse { LAPSE_STATIC = 0, LAPSE_HARMONIC = 1, LAPSE_CONFORMAL_FLRW = 2 };
+
+// One fused RK4 stage over the whole grid: stencils of the `in` register,
+// BSSN right-hand side, matter sources, and the RK4Register update
+// (utils/RK4Register.h:118-176) in the epilogue.
+struct StageArgs {
+  void* base;             // registers [field][reg][point], storage type TS
+  long fstride, rstride;  // element strides between fields / registers
+  int r_in, r_p, r_f, r_out;
+  const void* DIFFD;      // static dust: conserved density perturbation
+  int nx, ny, nz;
+  int ntx, nty, ntz;      // tile counts per axis
+  int per_xcd;            // tiles handled per XCD (XCD-aware block remap)
+  int stage;              // 0: RHS only into r_out; 1..4: fused K{n}Finalize
+  int normalize;          // stage 1: det(gamma)=1 rescale, trace-free A on the fly
+  int matter;
+  int lapse;
+  int use_ko;
+  int use_H;              // a_adj_amp or k_damping_amp non-zero
+  double inv_dx, inv_dx2, dt;
+  double phi_F, K_F, rho_F, S_F;  // FRW values seen by this stage
+  double K_avg;
+  double lambda;
+  double ko_pref;         // KO prefactor (sign included), per unit stencil sum
+  double a_adj, k_damp;
+  double H_frw;           // 2/3 K_F^2 - 16 pi rho_F (for the split-precision H)
+  unsigned int* nan_count;
+};
+
+struct KernelShape {
+  int ti, tj, tk;         // tile extents (i slowest, k fastest)
+};
+
+// Launch the stage kernel for (storage precision, stencil radius, halo).
+// prec: 0 = FP64, 1 = FP32 split.  Returns hipError_t.
+int launch_stage(int prec, int order, int halo, const KernelShape& shape, const StageArgs& a,
+                 hipStream_t stream);
+int stage_kernel_info(int prec, int order, int halo, const KernelShape& shape, int* vgprs,
+                      int* spills, int* lds_bytes);
+
+}  // namespace cg

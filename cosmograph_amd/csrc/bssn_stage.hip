@@ -1,0 +1,507 @@
+// bssn_stage.hip -- fused BSSN RK4 stage kernel for gfx950.
+//
+// One launch = one RK4 stage over the whole periodic grid:
+//   * a 3D tile of points per workgroup; the tile plus a halo of H points is
+//     staged into LDS one stencil field at a time (double-buffered: the
+//     global loads of the next field are in flight while the current one is
+//     differentiated),
+//   * the stencil results are folded into per-point tensors immediately
+//     (e.g. gamma^{kl} d_k d_l gamma_ij per metric field), which keeps the
+//     live register set near 60 values instead of ~110 raw derivatives,
+//   * the BSSN right-hand side (components/bssn/bssn.cc:361-831 and the
+//     macros in components/bssn/bssn_macros.h) is evaluated in registers,
+//   * the RK4Register update (utils/RK4Register.h:118-176) is the epilogue,
+//     so each stage streams every field register once.
+// Stage 1 also applies BSSN::set_DIFFgamma_Aij_norm (bssn.cc:61-119) to the
+// _p register on the fly: the unit-determinant factor is computed for every
+// halo point and applied while staging the metric into LDS.
+//
+// TS is the storage type, TC the stencil/algebra type.  FP64 drop-in: both
+// double.  FP32 split: both float, with the RK4 accumulation in double and
+// FRW scalars passed in double.
+#include <hip/hip_runtime.h>
+
+#include "bssn_stage.h"
+#include "stencil.h"
+
+namespace cg {
+
+__device__ __host__ constexpr int sy(int a, int b) {
+  return a == b ? (a == 0 ? 0 : a == 1 ? 3 : 5) : (a + b == 1 ? 1 : a + b == 2 ? 2 : 4);
+}
+__device__ __host__ constexpr bool diag6(int s) { return s == 0 || s == 3 || s == 5; }
+
+template <typename T>
+__device__ __forceinline__ T norm_factor(T g11, T g12, T g13, T g22, T g23, T g33) {
+  // bssn.cc:75-88:  1 - det(1 + DIFFgamma), then -(det^(1/3) - 1)
+  T omd = -(g11 + g22 + g33 - g12 * g12 - g13 * g13 - g23 * g23 + g11 * g22 + g11 * g33 + g22 * g33 -
+            g23 * g23 * g11 - g13 * g13 * g22 - g12 * g12 * g33 + T(2) * g12 * g13 * g23 + g11 * g22 * g33);
+  return -expm1(log1p(-omd) / T(3));
+}
+
+template <typename T>
+__device__ __forceinline__ bool nan_as_float(T v) {
+  // numNaNs tests the float-rounded value (utils/math.h:1524-1541)
+  float f = (float)v;
+  unsigned u = __float_as_uint(f);
+  return (u << 1) > 0xff000000u;
+}
+
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK>
+__global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
+  constexpr int NT = TI * TJ * TK;
+  constexpr int HJ = TJ + 2 * H, HK = TK + 2 * H, HI = TI + 2 * H;
+  constexpr int HV = HI * HJ * HK;
+  constexpr int SI = HJ * HK, SJ = HK, SK = 1;
+  constexpr int NPF = (HV + NT - 1) / NT;
+  __shared__ TC lds[2][HV];
+  __shared__ TC ldn[HV];
+
+  const int ntiles = a.ntx * a.nty * a.ntz;
+  const int b = blockIdx.x;
+  const int t = (b & 7) * a.per_xcd + (b >> 3);  // consecutive tiles stay on one XCD
+  if (t >= ntiles) return;
+  const int tz = t % a.ntz, ty = (t / a.ntz) % a.nty, tx = t / (a.ntz * a.nty);
+  const int i0 = tx * TI, j0 = ty * TJ, k0 = tz * TK;
+
+  const int tid = threadIdx.x;
+  const int lk = tid % TK, lj = (tid / TK) % TJ, li = tid / (TK * TJ);
+  const int px = i0 + li, py = j0 + lj, pz = k0 + lk;
+  const bool valid = (px < a.nx) && (py < a.ny) && (pz < a.nz);
+  const long idx = valid ? ((long)px * a.ny + py) * a.nz + pz : 0;
+  const int c = ((li + H) * HJ + (lj + H)) * HK + (lk + H);
+
+  // wrapped global offsets of this thread's share of the halo tile
+  unsigned goff[NPF];
+#pragma unroll
+  for (int q = 0; q < NPF; ++q) {
+    const int e = tid + q * NT;
+    const int ee = e < HV ? e : 0;
+    const int hk = ee % HK, hj = (ee / HK) % HJ, hi = ee / (HK * HJ);
+    int x = i0 + hi - H, y = j0 + hj - H, z = k0 + hk - H;
+    x += x < 0 ? a.nx : 0; x -= x >= a.nx ? a.nx : 0;
+    y += y < 0 ? a.ny : 0; y -= y >= a.ny ? a.ny : 0;
+    z += z < 0 ? a.nz : 0; z -= z >= a.nz ? a.nz : 0;
+    goff[q] = (unsigned)(((long)x * a.ny + y) * a.nz + z);
+  }
+
+  const TS* __restrict__ base = (const TS*)a.base;
+  auto reg = [&](int f, int r) -> const TS* { return base + (long)f * a.fstride + (long)r * a.rstride; };
+  const bool norm = a.normalize != 0;
+
+  // stage-1 normalisation factor at every halo point
+  if (norm) {
+#pragma unroll
+    for (int q = 0; q < NPF; ++q) {
+      const int e = tid + q * NT;
+      if (e < HV) {
+        const unsigned o = goff[q];
+        ldn[e] = norm_factor<TC>((TC)reg(G11, a.r_in)[o], (TC)reg(G12, a.r_in)[o], (TC)reg(G13, a.r_in)[o],
+                                 (TC)reg(G22, a.r_in)[o], (TC)reg(G23, a.r_in)[o], (TC)reg(G33, a.r_in)[o]);
+      }
+    }
+  }
+
+  TC pf[NPF];
+  auto fetch = [&](int f) {
+    const TS* __restrict__ src = reg(f, a.r_in);
+#pragma unroll
+    for (int q = 0; q < NPF; ++q) {
+      const int e = tid + q * NT;
+      pf[q] = (e < HV) ? (TC)src[goff[q]] : TC(0);
+    }
+  };
+  // metric slot (0..5) or -1: applies the unit-determinant rescale (bssn.cc:94-99)
+  auto store = [&](TC* __restrict__ dst, int gslot) {
+#pragma unroll
+    for (int q = 0; q < NPF; ++q) {
+      const int e = tid + q * NT;
+      if (e < HV) {
+        TC v = pf[q];
+        if (norm && gslot >= 0) {
+          const TC tn = ldn[e];
+          v = (v + (diag6(gslot) ? tn : TC(0))) / (TC(1) - tn);
+        }
+        dst[e] = v;
+      }
+    }
+  };
+
+  // ---------------------------------------------------------- point values
+  TC v[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) v[f] = valid ? (TC)reg(f, a.r_in)[idx] : TC(0);
+  if (norm) {
+    // bssn.cc:94-117 at this point
+    const TC tn = norm_factor<TC>(v[G11], v[G12], v[G13], v[G22], v[G23], v[G33]);
+    const TC den = TC(1) - tn;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) v[G11 + s] = (v[G11 + s] + (diag6(s) ? tn : TC(0))) / den;
+    const TC g11 = v[G11], g12 = v[G12], g13 = v[G13], g22 = v[G22], g23 = v[G23], g33 = v[G33];
+    const TC i11 = TC(1) + g22 + g33 - g23 * g23 + g22 * g33;
+    const TC i22 = TC(1) + g11 + g33 - g13 * g13 + g11 * g33;
+    const TC i33 = TC(1) + g11 + g22 - g12 * g12 + g11 * g22;
+    const TC i12 = g13 * g23 - g12 * (TC(1) + g33);
+    const TC i13 = g12 * g23 - g13 * (TC(1) + g22);
+    const TC i23 = g12 * g13 - g23 * (TC(1) + g11);
+    const TC trA = i11 * v[A11] + i22 * v[A22] + i33 * v[A33] +
+                   TC(2) * (i12 * v[A12] + i13 * v[A13] + i23 * v[A23]);
+    const TC third = TC(1.0 / 3.0);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) v[A11 + s] = (v[A11 + s] - third * ((diag6(s) ? TC(1) : TC(0)) + v[G11 + s]) * trA) / den;
+  }
+
+  // inverse conformal metric from DIFF variables (bssn.cc:447-455)
+  TC gi[6];
+  {
+    const TC g11 = v[G11], g12 = v[G12], g13 = v[G13], g22 = v[G22], g23 = v[G23], g33 = v[G33];
+    gi[0] = TC(1) + g22 + g33 - g23 * g23 + g22 * g33;
+    gi[3] = TC(1) + g11 + g33 - g13 * g13 + g11 * g33;
+    gi[5] = TC(1) + g11 + g22 - g12 * g12 + g11 * g22;
+    gi[1] = g13 * g23 - g12 * (TC(1) + g33);
+    gi[2] = g12 * g23 - g13 * (TC(1) + g22);
+    gi[4] = g12 * g13 - g23 * (TC(1) + g11);
+  }
+  auto GI = [&](int p, int q) -> TC { return gi[sy(p, q)]; };
+  auto GM = [&](int p, int q) -> TC { return v[G11 + sy(p, q)] + (p == q ? TC(1) : TC(0)); };
+  auto AM = [&](int p, int q) -> TC { return v[A11 + sy(p, q)]; };
+
+  // ------------------------------------------------ stencil phase (LDS)
+  const TC idx1 = (TC)a.inv_dx, idx2 = (TC)a.inv_dx2;
+  TC dg[3][6];      // d_k gamma_ij
+  TC T1[6];         // gamma^{kl} d_k d_l gamma_ij
+  TC dphi[3], ddphi[6], dal[3], ddal[6], dK[3], dG[3][3];
+  TC ko[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) ko[f] = TC(0);
+  const bool use_ko = a.use_ko != 0;
+  const TC kop = (TC)a.ko_pref;
+  constexpr int KO_ORDER = (H > R) ? 2 * R : 0;
+
+  // fields staged in order; A_ij only when KO is active
+  constexpr int SEQ[NF] = {G11, G12, G13, G22, G23, G33, PHI, ALPHA, KK, GAM1, GAM2, GAM3,
+                           A11, A12, A13, A22, A23, A33};
+  const int nseq = use_ko ? NF : 12;
+
+  fetch(SEQ[0]);
+  store(lds[0], 0);
+  __syncthreads();
+
+#pragma unroll
+  for (int s = 0; s < NF; ++s) {
+    if (s < nseq) {
+      const bool more = (s + 1) < nseq;
+      if (more) fetch(SEQ[(s + 1) < NF ? s + 1 : 0]);
+      const TC* __restrict__ L = lds[s & 1];
+      const int f = SEQ[s];
+      if (valid) {
+        if (KO_ORDER && use_ko) ko[f] = kop * ko_sum<KO_ORDER, TC>(L, c, SI, SJ, SK);
+        if (f <= G33 || f == PHI || f == ALPHA) {
+          const TC d0 = fd1<R>(L, c, SI) * idx1, d1 = fd1<R>(L, c, SJ) * idx1, d2 = fd1<R>(L, c, SK) * idx1;
+          const TC e00 = fd2<R>(L, c, SI) * idx2, e11 = fd2<R>(L, c, SJ) * idx2, e22 = fd2<R>(L, c, SK) * idx2;
+          const TC e01 = fdm<R>(L, c, SI, SJ) * idx2, e02 = fdm<R>(L, c, SI, SK) * idx2,
+                   e12 = fdm<R>(L, c, SJ, SK) * idx2;
+          if (f <= G33) {
+            dg[0][f] = d0; dg[1][f] = d1; dg[2][f] = d2;
+            T1[f] = gi[0] * e00 + gi[3] * e11 + gi[5] * e22 + TC(2) * (gi[1] * e01 + gi[2] * e02 + gi[4] * e12);
+          } else if (f == PHI) {
+            dphi[0] = d0; dphi[1] = d1; dphi[2] = d2;
+            ddphi[0] = e00; ddphi[1] = e01; ddphi[2] = e02; ddphi[3] = e11; ddphi[4] = e12; ddphi[5] = e22;
+          } else {
+            dal[0] = d0; dal[1] = d1; dal[2] = d2;
+            ddal[0] = e00; ddal[1] = e01; ddal[2] = e02; ddal[3] = e11; ddal[4] = e12; ddal[5] = e22;
+          }
+        } else if (f == KK) {
+          dK[0] = fd1<R>(L, c, SI) * idx1; dK[1] = fd1<R>(L, c, SJ) * idx1; dK[2] = fd1<R>(L, c, SK) * idx1;
+        } else if (f >= GAM1 && f <= GAM3) {
+          const int kk = f - GAM1;
+          dG[0][kk] = fd1<R>(L, c, SI) * idx1; dG[1][kk] = fd1<R>(L, c, SJ) * idx1; dG[2][kk] = fd1<R>(L, c, SK) * idx1;
+        }
+      }
+      if (more) store(lds[(s + 1) & 1], (s + 1) < 6 ? s + 1 : -1);
+      __syncthreads();
+    }
+  }
+  if (!valid) return;
+
+  // ---------------------------------------------------------- algebra
+  const TC phiF = (TC)a.phi_F, KF = (TC)a.K_F, rhoF = (TC)a.rho_F, SF = (TC)a.S_F;
+  const TC PI = TC(3.14159265358979323846);
+  const TC alpha = v[ALPHA] + TC(1);
+  const TC phi = v[PHI] + phiF;
+  const TC K = v[KK] + KF;
+
+  // sources (components/static/static.cc:28-35, components/Lambda/lambda.cc:32-41)
+  TC DIFFr = TC(0), DIFFS = TC(0);
+  if (a.matter == MATTER_STATIC) {
+    const TC D = (TC)((const TS*)a.DIFFD)[idx];
+    DIFFr = exp(TC(-6) * (v[PHI] + phiF)) * D + rhoF * expm1(TC(-6) * v[PHI]);
+  }
+  DIFFr += (TC)a.lambda;
+  DIFFS += TC(-3) * (TC)a.lambda;
+
+  // A^ij and A_ij A^ij (bssn.cc:463-472)
+  TC Ac[6];
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int J = I; J < 3; ++J) {
+      TC acc = TC(0);
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const TC w = GI(I, p);
+        acc += w * (GI(J, 0) * AM(p, 0) + GI(J, 1) * AM(p, 1) + GI(J, 2) * AM(p, 2));
+      }
+      Ac[sy(I, J)] = acc;
+    }
+  const TC AijAij = Ac[0] * v[A11] + Ac[3] * v[A22] + Ac[5] * v[A33] +
+                    TC(2) * (Ac[1] * v[A12] + Ac[2] * v[A13] + Ac[4] * v[A23]);
+
+  // Christoffels (bssn_macros.h:221-229):  GL_ijk = 1/2(d_j g_ki + d_k g_ji - d_i g_jk),
+  // G^i_jk = gamma^{im} GL_mjk
+  TC GL[3][6], G[3][6];
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int J = 0; J < 3; ++J)
+#pragma unroll
+      for (int Kx = J; Kx < 3; ++Kx)
+        GL[I][sy(J, Kx)] = TC(0.5) * (dg[J][sy(Kx, I)] + dg[Kx][sy(J, I)] - dg[I][sy(J, Kx)]);
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int s = 0; s < 6; ++s) G[I][s] = GI(I, 0) * GL[0][s] + GI(I, 1) * GL[1][s] + GI(I, 2) * GL[2][s];
+  TC Gd[3];
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+    Gd[I] = G[I][0] * gi[0] + G[I][3] * gi[3] + G[I][5] * gi[5] + TC(2) * (G[I][1] * gi[1] + G[I][2] * gi[2] + G[I][4] * gi[4]);
+
+  // D_i D_j phi (bssn.cc:592-602)
+  TC DDphi[6];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) DDphi[s] = ddphi[s] - (G[0][s] * dphi[0] + G[1][s] * dphi[1] + G[2][s] * dphi[2]);
+
+  // (D_i D_j alpha)^TF and its trace (bssn.cc:604-626)
+  TC DDa[6], DDaTR;
+  {
+    TC gild[3];
+#pragma unroll
+    for (int I = 0; I < 3; ++I) gild[I] = GI(I, 0) * dphi[0] + GI(I, 1) * dphi[1] + GI(I, 2) * dphi[2];
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+      for (int J = I; J < 3; ++J) {
+        const int s = sy(I, J);
+        TC acc = TC(0);
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+          acc += (G[m][s] + TC(2) * ((m == I ? dphi[J] : TC(0)) + (m == J ? dphi[I] : TC(0)) - GM(I, J) * gild[m])) * dal[m];
+        DDa[s] = ddal[s] - acc;
+      }
+    DDaTR = gi[0] * DDa[0] + gi[3] * DDa[3] + gi[5] * DDa[5] + TC(2) * (gi[1] * DDa[1] + gi[2] * DDa[2] + gi[4] * DDa[4]);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) DDa[s] -= TC(1.0 / 3.0) * (v[G11 + s] + (diag6(s) ? TC(1) : TC(0))) * DDaTR;
+  }
+  const TC em4phi = exp(TC(-4) * phi);
+  DDaTR *= em4phi;
+
+  // conformal Ricci (bssn_macros.h:248-277).  TERM4 is regrouped as
+  //   sum_{K,M} X[K][I][M] Y[J][K][M] + (I<->J),
+  //   X[K][I][M] = gamma^{ML} G^K_{LI},  Y[J][K][M] = GL_{J,KM} + 1/2 GL_{K,MJ}
+  TC Z[3][3];
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int J = 0; J < 3; ++J) Z[I][J] = TC(0);
+#pragma unroll
+  for (int Kx = 0; Kx < 3; ++Kx) {
+    TC X[3][3], Y[3][3];
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+      for (int M = 0; M < 3; ++M)
+        X[I][M] = GI(M, 0) * G[Kx][sy(0, I)] + GI(M, 1) * G[Kx][sy(1, I)] + GI(M, 2) * G[Kx][sy(2, I)];
+#pragma unroll
+    for (int J = 0; J < 3; ++J)
+#pragma unroll
+      for (int M = 0; M < 3; ++M) Y[J][M] = GL[J][sy(Kx, M)] + TC(0.5) * GL[Kx][sy(M, J)];
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+      for (int J = 0; J < 3; ++J) Z[I][J] += X[I][0] * Y[J][0] + X[I][1] * Y[J][1] + X[I][2] * Y[J][2];
+  }
+  TC ric[6];
+  {
+    // phi part (bssn.cc:639-655)
+    const TC expression =
+        gi[0] * (DDphi[0] + TC(2) * dphi[0] * dphi[0]) + gi[3] * (DDphi[3] + TC(2) * dphi[1] * dphi[1]) +
+        gi[5] * (DDphi[5] + TC(2) * dphi[2] * dphi[2]) +
+        TC(2) * (gi[1] * (DDphi[1] + TC(2) * dphi[0] * dphi[1]) + gi[2] * (DDphi[2] + TC(2) * dphi[0] * dphi[2]) +
+                 gi[4] * (DDphi[4] + TC(2) * dphi[1] * dphi[2]));
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+      for (int J = I; J < 3; ++J) {
+        const int s = sy(I, J);
+        TC t2 = TC(0), t3 = TC(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          t2 += GM(k, I) * dG[J][k] + GM(k, J) * dG[I][k];
+          t3 += Gd[k] * (GL[I][sy(J, k)] + GL[J][sy(I, k)]);
+        }
+        const TC U = TC(-0.5) * T1[s] + TC(0.5) * t2 + TC(0.5) * t3 + (Z[I][J] + Z[J][I]);
+        ric[s] = U - TC(2) * (DDphi[s] - TC(2) * dphi[I] * dphi[J] + GM(I, J) * expression);
+      }
+  }
+  const TC ricci = em4phi * (ric[0] * gi[0] + ric[3] * gi[3] + ric[5] * gi[5] +
+                             TC(2) * (ric[1] * gi[1] + ric[2] * gi[2] + ric[4] * gi[4]));
+  const TC e4phi = exp(TC(4) * phi);
+
+  // Hamiltonian constraint (bssn.cc:1073-1084), only when a damping term uses it
+  TC Hc = TC(0);
+  if (a.use_H) {
+    const TC rr = DIFFr + rhoF;
+    if constexpr (sizeof(TC) == 8) {
+      Hc = -exp(TC(5) * phi) / TC(8) * (ricci + TC(2.0 / 3.0) * K * K - AijAij - TC(16) * PI * rr);
+    } else {
+      // split precision: (2/3)K^2 - 16 pi rho in difference form around FRW
+      const TC dK2 = TC(2.0 / 3.0) * (TC(2) * KF + v[KK]) * v[KK];
+      Hc = -exp(TC(5) * phi) / TC(8) * (ricci + ((TC)a.H_frw + dK2) - AijAij - TC(16) * PI * DIFFr);
+    }
+  }
+
+  // ------------------------------------------------ right-hand sides
+  TC rhs[NF];
+  const TC dt = (TC)a.dt;
+  const TC adj = (TC)a.a_adj;
+#pragma unroll
+  for (int s = 0; s < 6; ++s)
+    rhs[G11 + s] = TC(-2) * alpha * v[A11 + s] + TC(0.5) * adj * dt * Hc * v[G11 + s] - ko[G11 + s];
+  // bssn_macros.h:309-338
+#pragma unroll
+  for (int I = 0; I < 3; ++I)
+#pragma unroll
+    for (int J = I; J < 3; ++J) {
+      const int s = sy(I, J);
+      TC AA = TC(0);
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        AA += AM(m, I) * (GI(m, 0) * AM(0, J) + GI(m, 1) * AM(1, J) + GI(m, 2) * AM(2, J));
+      const TC ricTF = ric[s] - TC(1.0 / 3.0) * e4phi * GM(I, J) * ricci;
+      const TC STF = TC(0);  // static dust and Lambda have no trace-free stress
+      rhs[A11 + s] = em4phi * (alpha * (ricTF - TC(8) * PI * STF) - DDa[s]) +
+                     alpha * (K * v[A11 + s] - TC(2) * AA) - adj * dt * v[A11 + s] * Hc - ko[A11 + s];
+    }
+  // bssn_macros.h:340-361 (S_i = 0 for static dust / Lambda)
+#pragma unroll
+  for (int I = 0; I < 3; ++I) {
+    const TC second = G[I][0] * Ac[0] + G[I][3] * Ac[3] + G[I][5] * Ac[5] +
+                      TC(2) * (G[I][1] * Ac[1] + G[I][2] * Ac[2] + G[I][4] * Ac[4]) +
+                      TC(6) * (Ac[sy(I, 0)] * dphi[0] + Ac[sy(I, 1)] * dphi[1] + Ac[sy(I, 2)] * dphi[2]);
+    rhs[GAM1 + I] = TC(-2) * (Ac[sy(I, 0)] * dal[0] + Ac[sy(I, 1)] * dal[1] + Ac[sy(I, 2)] * dal[2]) +
+                    TC(2) * alpha * (second - TC(2.0 / 3.0) * (GI(I, 0) * dK[0] + GI(I, 1) * dK[1] + GI(I, 2) * dK[2])) -
+                    ko[GAM1 + I];
+  }
+  // bssn.cc:766-795
+  rhs[KK] = -DDaTR + alpha * (TC(1.0 / 3.0) * v[KK] * (v[KK] + TC(2) * KF) + AijAij) +
+            TC(4) * PI * alpha * (DIFFr + DIFFS) + TC(4) * PI * v[ALPHA] * (rhoF + SF) -
+            (TC)a.k_damp * Hc * exp(TC(-5) * phi) - ko[KK];
+  // bssn.cc:797-820
+  rhs[PHI] = TC(0.1) * adj * dt * Hc - TC(1.0 / 6.0) * (alpha * v[KK] + v[ALPHA] * KF) - ko[PHI];
+  // BSSNGaugeHandler.cc:16-35
+  TC lapse = TC(0);
+  if (a.lapse == LAPSE_HARMONIC) lapse = -alpha * K;
+  else if (a.lapse == LAPSE_CONFORMAL_FLRW) lapse = TC(-1.0 / 3.0) * alpha * (TC)a.K_avg;
+  rhs[ALPHA] = lapse - ko[ALPHA];
+
+  // ------------------------------------------------ RK4Register epilogue
+  TS* __restrict__ wbase = (TS*)a.base;
+  auto wreg = [&](int f, int r) -> TS* { return wbase + (long)f * a.fstride + (long)r * a.rstride; };
+  const double hdt = a.dt;
+  const int st = a.stage;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const double cc = (double)rhs[f];
+    if (st == 0) {
+      wreg(f, a.r_out)[idx] = (TS)cc;
+    } else if (st == 1) {
+      const double p = (double)reg(f, a.r_p)[idx];
+      wreg(f, a.r_f)[idx] = (TS)(hdt * cc / 6.0);
+      wreg(f, a.r_out)[idx] = (TS)(p + hdt * cc / 2.0);
+    } else if (st == 2 || st == 3) {
+      const double p = (double)reg(f, a.r_p)[idx];
+      const double fo = (double)reg(f, a.r_f)[idx];
+      wreg(f, a.r_f)[idx] = (TS)(fo + hdt * cc / 3.0);
+      wreg(f, a.r_out)[idx] = (TS)(st == 2 ? p + hdt * cc / 2.0 : p + hdt * cc);
+    } else {
+      const double p = (double)reg(f, a.r_p)[idx];
+      const double fo = (double)reg(f, a.r_f)[idx];
+      const double pn = fo + (hdt * cc / 6.0 + p);
+      wreg(f, a.r_out)[idx] = (TS)pn;
+      if (f == PHI && nan_as_float(pn)) atomicAdd(a.nan_count, 1u);
+    }
+  }
+}
+
+// ------------------------------------------------------------ dispatch
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK>
+static int launch_t(const StageArgs& a, hipStream_t s) {
+  const int ntiles = a.ntx * a.nty * a.ntz;
+  const int grid = 8 * a.per_xcd;
+  (void)ntiles;
+  hipLaunchKernelGGL((stage_kernel<TS, TC, R, H, TI, TJ, TK>), dim3(grid), dim3(TI * TJ * TK), 0, s, a);
+  return (int)hipGetLastError();
+}
+
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK>
+static int info_t(int* vgprs, int* spills, int* lds) {
+  hipFuncAttributes attr;
+  hipError_t e = hipFuncGetAttributes(&attr, (const void*)stage_kernel<TS, TC, R, H, TI, TJ, TK>);
+  if (e != hipSuccess) return (int)e;
+  *vgprs = attr.numRegs;
+  *spills = (int)attr.localSizeBytes;
+  *lds = (int)attr.sharedSizeBytes;
+  return 0;
+}
+
+#define CG_SHAPES(X, TS, TC, R, H) \
+  X(TS, TC, R, H, 4, 8, 8)         \
+  X(TS, TC, R, H, 2, 8, 16)        \
+  X(TS, TC, R, H, 4, 4, 16)
+
+#define CG_ORDERS(X, TS, TC)  \
+  CG_SHAPES(X, TS, TC, 1, 1)  \
+  CG_SHAPES(X, TS, TC, 1, 2)  \
+  CG_SHAPES(X, TS, TC, 2, 2)  \
+  CG_SHAPES(X, TS, TC, 3, 3)  \
+  CG_SHAPES(X, TS, TC, 4, 4)  \
+  CG_SHAPES(X, TS, TC, 4, 5)
+
+#define CG_CASE_LAUNCH(TS, TC, R, H, A, B, C)                                    \
+  if (order == 2 * R && halo == H && shape.ti == A && shape.tj == B && shape.tk == C) \
+    return launch_t<TS, TC, R, H, A, B, C>(a, stream);
+#define CG_CASE_INFO(TS, TC, R, H, A, B, C)                                      \
+  if (order == 2 * R && halo == H && shape.ti == A && shape.tj == B && shape.tk == C) \
+    return info_t<TS, TC, R, H, A, B, C>(vgprs, spills, lds_bytes);
+
+int launch_stage(int prec, int order, int halo, const KernelShape& shape, const StageArgs& a,
+                 hipStream_t stream) {
+  if (prec == 0) {
+    CG_ORDERS(CG_CASE_LAUNCH, double, double)
+  } else {
+    CG_ORDERS(CG_CASE_LAUNCH, float, float)
+  }
+  return (int)hipErrorInvalidConfiguration;
+}
+
+int stage_kernel_info(int prec, int order, int halo, const KernelShape& shape, int* vgprs, int* spills,
+                      int* lds_bytes) {
+  if (prec == 0) {
+    CG_ORDERS(CG_CASE_INFO, double, double)
+  } else {
+    CG_ORDERS(CG_CASE_INFO, float, float)
+  }
+  return (int)hipErrorInvalidConfiguration;
+}
+
+}  // namespace cg

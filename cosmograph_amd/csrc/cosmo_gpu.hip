@@ -1,0 +1,510 @@
+// cosmo_gpu.hip -- host runtime and C ABI for the MI355X BSSN hot path.
+//
+// Owns the device registers of one periodic grid, the FP64 FRW reference
+// integrator (host side, scalar), and the launch sequence of the fused RK4
+// stage kernels.  See include/cosmo_gpu.h for the reference method each
+// entry point replaces.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cosmo_gpu.h"
+#include "bssn_stage.h"
+
+using namespace cg;
+
+namespace {
+
+// ---------------------------------------------------------------- FRW --
+// FP64 background integrator with the stage structure of utils/FRW.h:93-254
+// (RK4; the getters after stage n are what the RHS of stage n+1 sees).
+// The 4*pi factor is formed in long double like the reference's PI_L
+// literal (FRW.h:4), so the two agree bitwise.
+struct Frw {
+  static constexpr long double kPi = 3.141592653589793238462643383279502884L;
+  double phi = 0, K = 0;
+  std::vector<double> rho, w;
+  double kphi[4] = {0, 0, 0, 0}, kK[4] = {0, 0, 0, 0};
+  std::vector<double> krho[4];
+  double g_phi = 0, g_K = 0, g_rho = 0, g_S = 0;
+
+  void set(double p, double k) { phi = g_phi = p; K = g_K = k; }
+  void add_fluid(double r, double ww) {
+    rho.push_back(r);
+    w.push_back(ww);
+    for (auto& v : krho) v.resize(rho.size());
+    sum_get(rho);
+  }
+  void sum_get(const std::vector<double>& rr) {
+    g_rho = 0.0;
+    g_S = 0.0;
+    for (size_t n = 0; n < rr.size(); ++n) {
+      g_rho += rr[n];
+      g_S += 3.0 * rr[n] * w[n];
+    }
+  }
+  // stage n = 1..4 (P1_step, P2_step, P3_step, RK_total_step)
+  void stage(int n, double h) {
+    const size_t nf = rho.size();
+    double Kin = K, frac = 0.0;
+    const std::vector<double>* prev = nullptr;
+    if (n == 2) { Kin = K + (h / 2.0) * kK[0]; prev = &krho[0]; frac = h / 2.0; }
+    if (n == 3) { Kin = K + (h / 2.0) * kK[1]; prev = &krho[1]; frac = h / 2.0; }
+    if (n == 4) { Kin = K + h * kK[2]; prev = &krho[2]; frac = h; }
+    std::vector<double> rin(nf);
+    kphi[n - 1] = -1.0 * Kin / 6.0;
+    double src = 0.0;
+    for (size_t q = 0; q < nf; ++q) {
+      rin[q] = prev ? rho[q] + frac * (*prev)[q] : rho[q];
+      src += 4.0 * kPi * rin[q] * (1.0 + 3.0 * w[q]);
+    }
+    kK[n - 1] = 1.0 / 3.0 * Kin * Kin + src;
+    for (size_t q = 0; q < nf; ++q) krho[n - 1][q] = Kin * rin[q] * (1.0 + w[q]);
+    std::vector<double> rg(nf);
+    if (n <= 2) {
+      g_phi = phi + (h / 2.0) * kphi[n - 1];
+      g_K = K + (h / 2.0) * kK[n - 1];
+      for (size_t q = 0; q < nf; ++q) rg[q] = rho[q] + (h / 2.0) * krho[n - 1][q];
+    } else if (n == 3) {
+      g_phi = phi + h * kphi[2];
+      g_K = K + h * kK[2];
+      for (size_t q = 0; q < nf; ++q) rg[q] = rho[q] + h * krho[2][q];
+    } else {
+      phi += h / 6.0 * (kphi[0] + 2.0 * kphi[1] + 2.0 * kphi[2] + kphi[3]);
+      K += h / 6.0 * (kK[0] + 2.0 * kK[1] + 2.0 * kK[2] + kK[3]);
+      for (size_t q = 0; q < nf; ++q)
+        rho[q] += h / 6.0 * (krho[0][q] + 2.0 * krho[1][q] + 2.0 * krho[2][q] + krho[3][q]);
+      g_phi = phi;
+      g_K = K;
+      rg = rho;
+    }
+    sum_get(rg);
+  }
+};
+
+const char* kFieldNames[NF] = {"DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIFFgamma22", "DIFFgamma23",
+                               "DIFFgamma33", "DIFFphi",     "A11",         "A12",         "A13",
+                               "A22",         "A23",         "A33",         "DIFFK",       "Gamma1",
+                               "Gamma2",      "Gamma3",      "DIFFalpha"};
+
+// ------------------------------------------------- elementwise kernels --
+template <typename TS>
+__global__ void k_step_init(TS* base, long fstride, long rstride, int rp, int ra, int rf, long n, int normalize) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  auto R = [&](int f, int r) -> TS* { return base + (long)f * fstride + (long)r * rstride; };
+  // RK4Register::stepInit (RK4Register.h:118-127)
+  double v[NF];
+  for (int f = 0; f < NF; ++f) {
+    v[f] = (double)R(f, rp)[x];
+    R(f, rf)[x] = (TS)0;
+  }
+  if (normalize) {
+    // BSSN::set_DIFFgamma_Aij_norm (bssn.cc:61-119)
+    double g11 = v[G11], g12 = v[G12], g13 = v[G13], g22 = v[G22], g23 = v[G23], g33 = v[G33];
+    double omd = -1.0 * (g11 + g22 + g33 - g12 * g12 - g13 * g13 - g23 * g23 + g11 * g22 + g11 * g33 + g22 * g33 -
+                         g23 * g23 * g11 - g13 * g13 * g22 - g12 * g12 * g33 + 2.0 * g12 * g13 * g23 + g11 * g22 * g33);
+    double t = -1.0 * expm1(log1p(-1.0 * omd) / 3.0);
+    g11 = (g11 + t) / (1.0 - t); g22 = (g22 + t) / (1.0 - t); g33 = (g33 + t) / (1.0 - t);
+    g12 = g12 / (1.0 - t); g13 = g13 / (1.0 - t); g23 = g23 / (1.0 - t);
+    v[G11] = g11; v[G12] = g12; v[G13] = g13; v[G22] = g22; v[G23] = g23; v[G33] = g33;
+    double i11 = 1.0 + g22 + g33 - g23 * g23 + g22 * g33, i22 = 1.0 + g11 + g33 - g13 * g13 + g11 * g33;
+    double i33 = 1.0 + g11 + g22 - g12 * g12 + g11 * g22, i12 = g13 * g23 - g12 * (1.0 + g33);
+    double i13 = g12 * g23 - g13 * (1.0 + g22), i23 = g12 * g13 - g23 * (1.0 + g11);
+    double trA = i11 * v[A11] + i22 * v[A22] + i33 * v[A33] + 2.0 * (i12 * v[A12] + i13 * v[A13] + i23 * v[A23]);
+    v[A11] = (v[A11] - 1.0 / 3.0 * (1.0 + g11) * trA) / (1.0 - t);
+    v[A22] = (v[A22] - 1.0 / 3.0 * (1.0 + g22) * trA) / (1.0 - t);
+    v[A33] = (v[A33] - 1.0 / 3.0 * (1.0 + g33) * trA) / (1.0 - t);
+    v[A12] = (v[A12] - 1.0 / 3.0 * g12 * trA) / (1.0 - t);
+    v[A13] = (v[A13] - 1.0 / 3.0 * g13 * trA) / (1.0 - t);
+    v[A23] = (v[A23] - 1.0 / 3.0 * g23 * trA) / (1.0 - t);
+  }
+  for (int f = 0; f < NF; ++f) R(f, ra)[x] = (TS)v[f];
+}
+
+// RK4Register::K{n}Finalize (RK4Register.h:129-176) without the pointer swap
+template <typename TS>
+__global__ void k_finalize(TS* base, long fstride, long rstride, int rp, int rc, int rf, long n, int stage,
+                           double dt) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  for (int f = 0; f < NF; ++f) {
+    TS* P = base + (long)f * fstride + (long)rp * rstride;
+    TS* Cc = base + (long)f * fstride + (long)rc * rstride;
+    TS* F = base + (long)f * fstride + (long)rf * rstride;
+    const double c = (double)Cc[x], p = (double)P[x], fo = (double)F[x];
+    if (stage == 1) { F[x] = (TS)(fo + dt * c / 6.0); Cc[x] = (TS)(p + dt * c / 2.0); }
+    else if (stage == 2) { F[x] = (TS)(fo + dt * c / 3.0); Cc[x] = (TS)(p + dt * c / 2.0); }
+    else if (stage == 3) { F[x] = (TS)(fo + dt * c / 3.0); Cc[x] = (TS)(p + dt * c); }
+    else {
+      const double fn = fo + (dt * c / 6.0 + p);
+      F[x] = (TS)fn; Cc[x] = (TS)fn; P[x] = (TS)fn;
+    }
+  }
+}
+
+template <typename TS>
+__global__ void k_zero(TS* p, long n) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n) p[x] = (TS)0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ context --
+struct cg_ctx {
+  cg_params p{};
+  int prec = 0;
+  size_t ts = 8;          // storage bytes
+  int R = 2, H = 2;
+  KernelShape shape{4, 8, 8};
+  long npts = 0, fstride = 0, rstride = 0;
+  void* regs = nullptr;   // [NF][4][npts]
+  void* DIFFD = nullptr;
+  unsigned* nan_dev = nullptr;
+  unsigned nan_seen = 0;
+  hipStream_t stream = nullptr;
+  Frw frw;
+  double lambda = 0;
+  // register roles (physical slot of each reference register)
+  int sP = 0, sF = 1, sA = 2, sC = 3;
+  bool a_is_p = true;     // after a fused step _a and _f hold the same data as _p
+  std::string err;
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+
+static int fail(cg_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+#define HIPCHK(ctx, expr)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return fail(ctx, CG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static void* reg_ptr(cg_ctx* c, int f, int slot) {
+  return (char*)c->regs + ((long)f * c->fstride + (long)slot * c->rstride) * (long)c->ts;
+}
+
+static int halo_for(int order, double ko) {
+  int R = order / 2;
+  if (ko != 0.0 && order == 2) return 2;
+  if (ko != 0.0 && order == 8) return 5;
+  return R;
+}
+
+// The definitions below take C linkage from their declarations in cosmo_gpu.h.
+
+const char* cg_last_error(const cg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cg_create(const cg_params* params, cg_ctx** out) {
+  if (!params || !out) return CG_ERR_ARG;
+  *out = nullptr;
+  const cg_params& p = *params;
+  if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return CG_ERR_ARG;
+  if (p.stencil_order != 2 && p.stencil_order != 4 && p.stencil_order != 6 && p.stencil_order != 8) return CG_ERR_ARG;
+  if (p.precision != CG_FP64 && p.precision != CG_FP32_SPLIT) return CG_ERR_ARG;
+  if (p.matter != CG_MATTER_NONE && p.matter != CG_MATTER_STATIC) return CG_ERR_ARG;
+  if (p.lapse == CG_LAPSE_CONFORMAL_FLRW) return CG_ERR_ARG;  // needs per-stage K_avg reductions
+  if (!(p.dx > 0) || !std::isfinite(p.dt)) return CG_ERR_ARG;
+  cg_ctx* c = new cg_ctx();
+  c->p = p;
+  c->prec = p.precision;
+  c->ts = p.precision == CG_FP64 ? 8 : 4;
+  c->R = p.stencil_order / 2;
+  c->H = halo_for(p.stencil_order, p.ko_coeff);
+  c->lambda = p.lambda;
+  if (p.tile_i > 0) c->shape = KernelShape{p.tile_i, p.tile_j, p.tile_k};
+  // the single conditional wrap in the halo loader needs n > tile + H - 2
+  const int tdim[3] = {c->shape.ti, c->shape.tj, c->shape.tk}, ndim[3] = {p.nx, p.ny, p.nz};
+  for (int d = 0; d < 3; ++d)
+    if (ndim[d] < c->H || ndim[d] <= tdim[d] + c->H - 2) {
+      int rc = fail(c, CG_ERR_ARG, "grid too small for the stencil tile");
+      *out = c;
+      return rc;
+    }
+  c->npts = (long)p.nx * p.ny * p.nz;
+  const long pad = ((c->npts + 63) / 64) * 64;  // 256/512-byte aligned registers
+  c->rstride = pad;
+  c->fstride = 4 * pad;
+  *out = c;
+  HIPCHK(c, hipSetDevice(p.device));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(c, hipMalloc(&c->regs, (size_t)NF * 4 * pad * c->ts));
+  HIPCHK(c, hipMemsetAsync(c->regs, 0, (size_t)NF * 4 * pad * c->ts, c->stream));
+  HIPCHK(c, hipMalloc(&c->DIFFD, (size_t)pad * c->ts));
+  HIPCHK(c, hipMemsetAsync(c->DIFFD, 0, (size_t)pad * c->ts, c->stream));
+  HIPCHK(c, hipMalloc(&c->nan_dev, sizeof(unsigned)));
+  HIPCHK(c, hipMemsetAsync(c->nan_dev, 0, sizeof(unsigned), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CG_OK;
+}
+
+int cg_destroy(cg_ctx* c) {
+  if (!c) return CG_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+  if (c->regs) (void)hipFree(c->regs);
+  if (c->DIFFD) (void)hipFree(c->DIFFD);
+  if (c->nan_dev) (void)hipFree(c->nan_dev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CG_OK;
+}
+
+// resolve a reference field-map key to a device pointer
+static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err) {
+  *err = 0;
+  std::string s(name ? name : "");
+  if (s == "DIFFD_a") return c->DIFFD;
+  if (s.size() < 3 || s[s.size() - 2] != '_') { *err = 1; return nullptr; }
+  const std::string base = s.substr(0, s.size() - 2);
+  const char r = s.back();
+  for (int f = 0; f < NF; ++f) {
+    if (base != kFieldNames[f]) continue;
+    int slot;
+    if (r == 'p') slot = c->sP;
+    else if (r == 'f') slot = c->a_is_p ? c->sP : c->sF;
+    else if (r == 'a') slot = c->a_is_p ? c->sP : c->sA;
+    else if (r == 'c') slot = c->sC;
+    else { *err = 1; return nullptr; }
+    if (for_write && c->a_is_p && (r == 'a' || r == 'f')) { *err = 2; return nullptr; }
+    return reg_ptr(c, f, slot);
+  }
+  *err = 1;
+  return nullptr;
+}
+
+int cg_field_upload(cg_ctx* c, const char* name, const double* host, size_t n) {
+  if (!c || !host) return CG_ERR_ARG;
+  if ((long)n != c->npts) return fail(c, CG_ERR_ARG, "size mismatch");
+  int e;
+  void* d = resolve(c, name, true, &e);
+  if (!d) return fail(c, e == 2 ? CG_ERR_STATE : CG_ERR_ARG,
+                      e == 2 ? "_a/_f alias _p after a fused step; write _p" : std::string("unknown field ") + name);
+  if (c->prec == CG_FP64) {
+    HIPCHK(c, hipMemcpyAsync(d, host, n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(n);
+    for (size_t x = 0; x < n; ++x) tmp[x] = (float)host[x];
+    HIPCHK(c, hipMemcpyAsync(d, tmp.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return CG_OK;
+}
+
+int cg_field_download(cg_ctx* c, const char* name, double* host, size_t n) {
+  if (!c || !host) return CG_ERR_ARG;
+  if ((long)n != c->npts) return fail(c, CG_ERR_ARG, "size mismatch");
+  int e;
+  void* d = resolve(c, name, false, &e);
+  if (!d) return fail(c, CG_ERR_ARG, std::string("unknown field ") + name);
+  if (c->prec == CG_FP64) {
+    HIPCHK(c, hipMemcpyAsync(host, d, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), d, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t x = 0; x < n; ++x) host[x] = (double)tmp[x];
+  }
+  return CG_OK;
+}
+
+int cg_frw_set(cg_ctx* c, double phi, double K) {
+  if (!c) return CG_ERR_ARG;
+  c->frw.set(phi, K);
+  return CG_OK;
+}
+int cg_frw_add_fluid(cg_ctx* c, double rho, double w) {
+  if (!c) return CG_ERR_ARG;
+  c->frw.add_fluid(rho, w);
+  return CG_OK;
+}
+int cg_frw_get(const cg_ctx* c, double out[4]) {
+  if (!c || !out) return CG_ERR_ARG;
+  out[0] = c->frw.g_phi; out[1] = c->frw.g_K; out[2] = c->frw.g_rho; out[3] = c->frw.g_S;
+  return CG_OK;
+}
+int cg_set_lambda(cg_ctx* c, double lambda) {
+  if (!c) return CG_ERR_ARG;
+  c->lambda = lambda;
+  return CG_OK;
+}
+
+static StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int normalize) {
+  StageArgs a{};
+  a.base = c->regs;
+  a.fstride = c->fstride;
+  a.rstride = c->rstride;
+  a.r_in = rin; a.r_p = rp; a.r_f = rf; a.r_out = rout;
+  a.DIFFD = c->DIFFD;
+  a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
+  a.ntx = (c->p.nx + c->shape.ti - 1) / c->shape.ti;
+  a.nty = (c->p.ny + c->shape.tj - 1) / c->shape.tj;
+  a.ntz = (c->p.nz + c->shape.tk - 1) / c->shape.tk;
+  const int nt = a.ntx * a.nty * a.ntz;
+  a.per_xcd = (nt + 7) / 8;
+  a.stage = stage;
+  a.normalize = normalize;
+  a.matter = c->p.matter;
+  a.lapse = c->p.lapse;
+  a.use_ko = c->p.ko_coeff != 0.0 && (c->p.stencil_order == 2 || c->p.stencil_order == 8);
+  a.use_H = (c->p.a_adj_amp != 0.0 || c->p.k_damping_amp != 0.0);
+  a.inv_dx = 1.0 / c->p.dx;
+  a.inv_dx2 = 1.0 / (c->p.dx * c->p.dx);
+  a.dt = c->p.dt;
+  a.phi_F = c->frw.g_phi; a.K_F = c->frw.g_K; a.rho_F = c->frw.g_rho; a.S_F = c->frw.g_S;
+  a.K_avg = 0.0;
+  a.lambda = c->lambda;
+  const double dx = c->p.dx, ko = c->p.ko_coeff;
+  // KO_dissipation_Q (math.h:36-72): ko*dx^3/64 * sum/dx^4 ; -ko*dx^9/1024 * sum/dx^10
+  a.ko_pref = c->p.stencil_order == 2 ? ko / (64.0 * dx) : c->p.stencil_order == 8 ? -ko / (1024.0 * dx) : 0.0;
+  a.a_adj = c->p.a_adj_amp;
+  a.k_damp = c->p.k_damping_amp;
+  a.H_frw = 2.0 / 3.0 * c->frw.g_K * c->frw.g_K - 16.0 * M_PI * c->frw.g_rho;
+  a.nan_count = c->nan_dev;
+  return a;
+}
+
+static int launch(cg_ctx* c, const StageArgs& a) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+  }
+  int rc = launch_stage(c->prec, c->p.stencil_order, c->H, c->shape, a, c->stream);
+  if (rc != 0) return fail(c, CG_ERR_HIP, std::string("stage launch: ") + hipGetErrorString((hipError_t)rc));
+  if (c->timing) {
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    c->ev.emplace_back(e0, e1);
+  }
+  return CG_OK;
+}
+
+template <typename TS>
+static int step_init_t(cg_ctx* c) {
+  const int bs = 256;
+  const long nb = (c->npts + bs - 1) / bs;
+  hipLaunchKernelGGL(k_step_init<TS>, dim3(nb), dim3(bs), 0, c->stream, (TS*)c->regs, c->fstride, c->rstride, c->sP,
+                     c->sA, c->sF, c->npts, c->p.normalize_metric);
+  HIPCHK(c, hipGetLastError());
+  return CG_OK;
+}
+
+int cg_step_init(cg_ctx* c) {
+  if (!c) return CG_ERR_ARG;
+  if (c->a_is_p) { c->a_is_p = false; c->sA = 2; c->sC = 3; }
+  return c->prec == CG_FP64 ? step_init_t<double>(c) : step_init_t<float>(c);
+}
+
+int cg_rk_evolve(cg_ctx* c) {
+  if (!c) return CG_ERR_ARG;
+  if (c->a_is_p) return fail(c, CG_ERR_STATE, "RKEvolve before stepInit");
+  StageArgs a = make_args(c, 0, c->sA, c->sP, c->sF, c->sC, 0);
+  return launch(c, a);
+}
+
+template <typename TS>
+static int finalize_t(cg_ctx* c, int n) {
+  const int bs = 256;
+  const long nb = (c->npts + bs - 1) / bs;
+  hipLaunchKernelGGL(k_finalize<TS>, dim3(nb), dim3(bs), 0, c->stream, (TS*)c->regs, c->fstride, c->rstride, c->sP,
+                     c->sC, c->sF, c->npts, n, c->p.dt);
+  HIPCHK(c, hipGetLastError());
+  return CG_OK;
+}
+
+int cg_k_finalize(cg_ctx* c, int n) {
+  if (!c || n < 1 || n > 4) return CG_ERR_ARG;
+  if (c->a_is_p) return fail(c, CG_ERR_STATE, "K-finalize before stepInit");
+  c->frw.stage(n, c->p.dt);  // bssn.cc:269: FRW stage first
+  int rc = c->prec == CG_FP64 ? finalize_t<double>(c, n) : finalize_t<float>(c, n);
+  if (rc) return rc;
+  std::swap(c->sA, c->sC);  // swap_a_c
+  return CG_OK;
+}
+
+int cg_rk4_steps(cg_ctx* c, int nsteps) {
+  if (!c || nsteps < 0) return CG_ERR_ARG;
+  // fixed physical slots for the fused path: P=0, F=1, X=2, Y=3
+  c->sP = 0; c->sF = 1;
+  const int X = 2, Y = 3;
+  const bool ko_norm = c->p.ko_coeff != 0.0 && c->p.normalize_metric &&
+                       (c->p.stencil_order == 2 || c->p.stencil_order == 8);
+  for (int s = 0; s < nsteps; ++s) {
+    int rc;
+    if (ko_norm) {
+      // KO needs normalised A_ij at halo points: materialise stepInit first
+      c->a_is_p = false; c->sA = X; c->sC = Y;
+      if ((rc = cg_step_init(c))) return rc;
+      if ((rc = launch(c, make_args(c, 1, X, 0, 1, Y, 0)))) return rc;
+      c->frw.stage(1, c->p.dt);
+      if ((rc = launch(c, make_args(c, 2, Y, 0, 1, X, 0)))) return rc;
+    } else {
+      if ((rc = launch(c, make_args(c, 1, 0, 0, 1, X, c->p.normalize_metric)))) return rc;
+      c->frw.stage(1, c->p.dt);
+      if ((rc = launch(c, make_args(c, 2, X, 0, 1, Y, 0)))) return rc;
+    }
+    c->frw.stage(2, c->p.dt);
+    const int in3 = ko_norm ? X : Y, out3 = ko_norm ? Y : X;
+    if ((rc = launch(c, make_args(c, 3, in3, 0, 1, out3, 0)))) return rc;
+    c->frw.stage(3, c->p.dt);
+    if ((rc = launch(c, make_args(c, 4, out3, 0, 1, 0, 0)))) return rc;
+    c->frw.stage(4, c->p.dt);
+    c->sA = 0; c->sC = out3; c->a_is_p = true;
+  }
+  unsigned nn = 0;
+  HIPCHK(c, hipMemcpyAsync(&nn, c->nan_dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (nn != c->nan_seen) {
+    c->nan_seen = nn;
+    return fail(c, CG_ERR_NAN, "NAN detected in DIFFphi");
+  }
+  return CG_OK;
+}
+
+int cg_sync(cg_ctx* c) {
+  if (!c) return CG_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CG_OK;
+}
+
+int cg_kernel_info(const cg_ctx* c, int* vgprs, int* scratch, int* lds, int* ti, int* tj, int* tk) {
+  if (!c) return CG_ERR_ARG;
+  int rc = stage_kernel_info(c->prec, c->p.stencil_order, c->H, c->shape, vgprs, scratch, lds);
+  *ti = c->shape.ti; *tj = c->shape.tj; *tk = c->shape.tk;
+  return rc ? CG_ERR_HIP : CG_OK;
+}
+
+int cg_set_timing(cg_ctx* c, int on) {
+  if (!c) return CG_ERR_ARG;
+  c->timing = on != 0;
+  return CG_OK;
+}
+
+int cg_stage_time_ms(cg_ctx* c, double* avg, int* count) {
+  if (!c || !avg || !count) return CG_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double tot = 0;
+  int n = 0;
+  for (auto& e : c->ev) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) { tot += ms; ++n; }
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  c->ev.clear();
+  *avg = n ? tot / n : 0.0;
+  *count = n;
+  return CG_OK;
+}
+

@@ -1,0 +1,217 @@
+"""ctypes front end of libcosmo_gpu.so (C ABI: include/cosmo_gpu.h).
+
+``BSSN`` mirrors the public methods of the reference's ``cosmo::BSSN``
+(components/bssn/bssn.h:53-170) that its sims call: ``stepInit``,
+``RKEvolve``, ``K1Finalize``..``K4Finalize``, ``step`` and the ``fields``
+map, with the FRW reference integrator (``frw``) and Lambda attached as in
+the reference.  Errors raise ``CosmoGpuError`` carrying the library's
+message, in place of the reference's integer ``throw``s.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+FIELDS = ["DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIFFgamma22", "DIFFgamma23",
+          "DIFFgamma33", "DIFFphi", "A11", "A12", "A13", "A22", "A23", "A33", "DIFFK",
+          "Gamma1", "Gamma2", "Gamma3", "DIFFalpha"]
+PRECISION = {"fp64": 0, "fp32_split": 1}
+MATTER = {"none": 0, "vacuum": 0, "static": 1, "dust": 2}
+LAPSE = {"": 0, "Static": 0, "Harmonic": 1, "ConformalFLRW": 2}
+ERR_NAN = -10
+
+
+class CosmoGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[cosmo_gpu {code}] {msg}")
+        self.code = code
+
+
+class _Params(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("stencil_order", C.c_int),
+                ("precision", C.c_int), ("dx", C.c_double), ("dt", C.c_double),
+                ("ko_coeff", C.c_double), ("a_adj_amp", C.c_double),
+                ("k_damping_amp", C.c_double), ("normalize_metric", C.c_int),
+                ("lapse", C.c_int), ("matter", C.c_int), ("lambda_", C.c_double),
+                ("device", C.c_int), ("tile_i", C.c_int), ("tile_j", C.c_int),
+                ("tile_k", C.c_int)]
+
+
+@dataclass
+class GpuParams:
+    """Grid and physics parameters (reference macros and config keys)."""
+    n: int | tuple = 32
+    order: int = 4
+    precision: str = "fp64"
+    box: float = 1.0            # H_LEN_FRAC
+    dx: float | None = None
+    dt_frac: float = 0.1
+    ko_coeff: float = 0.0
+    a_adj_amp: float = 0.0
+    k_damping_amp: float = 0.0
+    normalize_metric: int = 1
+    lapse: str = "Static"
+    matter: str = "static"
+    lambda_: float = 0.0
+    device: int = 0
+    tile: tuple | None = None
+
+    def shape(self):
+        return (self.n, self.n, self.n) if np.isscalar(self.n) else tuple(self.n)
+
+    def to_c(self) -> _Params:
+        nx, ny, nz = self.shape()
+        dx = self.box / nx if self.dx is None else self.dx
+        ti, tj, tk = self.tile if self.tile else (0, 0, 0)
+        return _Params(nx, ny, nz, self.order, PRECISION[self.precision], dx, self.dt_frac * dx,
+                       self.ko_coeff, self.a_adj_amp, self.k_damping_amp, self.normalize_metric,
+                       LAPSE[self.lapse], MATTER[self.matter], self.lambda_, self.device,
+                       ti, tj, tk)
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return os.path.join(HERE, "libcosmo_gpu.so")
+
+
+def load_library():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise CosmoGpuError(-1, f"{path} is missing: build it with `python -m cosmograph_amd.build`")
+    L = C.CDLL(path)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+    sig = {
+        "cg_create": ([C.POINTER(_Params), C.POINTER(vp)], C.c_int),
+        "cg_destroy": ([vp], C.c_int),
+        "cg_last_error": ([vp], C.c_char_p),
+        "cg_field_upload": ([vp, C.c_char_p, dp, C.c_size_t], C.c_int),
+        "cg_field_download": ([vp, C.c_char_p, dp, C.c_size_t], C.c_int),
+        "cg_frw_set": ([vp, C.c_double, C.c_double], C.c_int),
+        "cg_frw_add_fluid": ([vp, C.c_double, C.c_double], C.c_int),
+        "cg_frw_get": ([vp, dp], C.c_int),
+        "cg_set_lambda": ([vp, C.c_double], C.c_int),
+        "cg_step_init": ([vp], C.c_int),
+        "cg_rk_evolve": ([vp], C.c_int),
+        "cg_k_finalize": ([vp, C.c_int], C.c_int),
+        "cg_rk4_steps": ([vp, C.c_int], C.c_int),
+        "cg_sync": ([vp], C.c_int),
+        "cg_kernel_info": ([vp, ip, ip, ip, ip, ip, ip], C.c_int),
+        "cg_set_timing": ([vp, C.c_int], C.c_int),
+        "cg_stage_time_ms": ([vp, dp, ip], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes, f.restype = args, res
+    _lib = L
+    return L
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class BSSN:
+    """GPU-resident BSSN state with the reference's method names."""
+
+    def __init__(self, params: GpuParams):
+        self.L = load_library()
+        self.params = params
+        self.shape = params.shape()
+        self.npts = int(np.prod(self.shape))
+        h = C.c_void_p()
+        rc = self.L.cg_create(C.byref(params.to_c()), C.byref(h))
+        self.h = h
+        if rc:
+            msg = self.L.cg_last_error(h).decode() if h.value else "invalid parameters"
+            if h.value:
+                self.L.cg_destroy(h)
+            self.h = None
+            raise CosmoGpuError(rc, msg)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.cg_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc):
+        if rc:
+            raise CosmoGpuError(rc, self.L.cg_last_error(self.h).decode())
+
+    # --- fields map --------------------------------------------------
+    def upload(self, name: str, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        if a.size != self.npts:
+            raise ValueError(f"{name}: expected {self.npts} values, got {a.size}")
+        self._chk(self.L.cg_field_upload(self.h, name.encode(), _dptr(a), a.size))
+
+    def download(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        self._chk(self.L.cg_field_download(self.h, name.encode(), _dptr(out), out.size))
+        return out
+
+    # --- FRW / Lambda --------------------------------------------------
+    def frw_set(self, phi: float, K: float) -> None:
+        self._chk(self.L.cg_frw_set(self.h, phi, K))
+
+    def frw_add_fluid(self, rho: float, w: float) -> None:
+        self._chk(self.L.cg_frw_add_fluid(self.h, rho, w))
+
+    def frw_get(self):
+        out = (C.c_double * 4)()
+        self._chk(self.L.cg_frw_get(self.h, out))
+        return tuple(out)
+
+    def set_lambda(self, lam: float) -> None:
+        self._chk(self.L.cg_set_lambda(self.h, lam))
+
+    # --- RK4 methods (bssn.cc:219-339) -----------------------------------
+    def stepInit(self):
+        self._chk(self.L.cg_step_init(self.h))
+
+    def RKEvolve(self):
+        self._chk(self.L.cg_rk_evolve(self.h))
+
+    def K1Finalize(self):
+        self._chk(self.L.cg_k_finalize(self.h, 1))
+
+    def K2Finalize(self):
+        self._chk(self.L.cg_k_finalize(self.h, 2))
+
+    def K3Finalize(self):
+        self._chk(self.L.cg_k_finalize(self.h, 3))
+
+    def K4Finalize(self):
+        self._chk(self.L.cg_k_finalize(self.h, 4))
+
+    def step(self, nsteps: int = 1):
+        """nsteps whole RK4 steps (stepInit + 4 fused stages each)."""
+        self._chk(self.L.cg_rk4_steps(self.h, nsteps))
+
+    def sync(self):
+        self._chk(self.L.cg_sync(self.h))
+
+    # --- introspection / timing -----------------------------------------
+    def kernel_info(self):
+        v = [C.c_int() for _ in range(6)]
+        self._chk(self.L.cg_kernel_info(self.h, *[C.byref(x) for x in v]))
+        vg, sc, lds, ti, tj, tk = (x.value for x in v)
+        return {"vgprs": vg, "scratch_bytes": sc, "lds_bytes": lds, "tile": (ti, tj, tk)}
+
+    def set_timing(self, on: bool):
+        self._chk(self.L.cg_set_timing(self.h, int(on)))
+
+    def stage_time_ms(self):
+        avg, n = C.c_double(), C.c_int()
+        self._chk(self.L.cg_stage_time_ms(self.h, C.byref(avg), C.byref(n)))
+        return avg.value, n.value

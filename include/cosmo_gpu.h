@@ -1,0 +1,106 @@
+/*
+ * cosmo_gpu.h -- C ABI of the MI355X BSSN hot path (libcosmo_gpu.so).
+ *
+ * The reference (CosmoGRaPH unigrid) has no FFI: its sims call C++ methods
+ * of BSSN, RK4Register, Static, Dust and Lambda in-process.  Each entry point
+ * below replaces one of those methods; the reference method it stands in
+ * for is cited beside it.  Plain C: opaque handle, int status (0 = ok, < 0 =
+ * error, message from cg_last_error), host buffers borrowed for the call
+ * only, device memory owned by the context.  One host thread per context;
+ * work is asynchronous on the context's HIP stream, and the calls that
+ * return data to the host synchronise.
+ *
+ * Field names are the reference's field-map keys (cosmo_macros.h:154-187,
+ * bssn_macros.h:77-112): "<field>_<p|a|c|f>" for the RK4 registers of
+ * DIFFgamma11..DIFFgamma33, DIFFphi, A11..A33, DIFFK, Gamma1..3, DIFFalpha;
+ * "DIFFD_a" for the static-dust density (components/static/static.h:67).
+ * Grid layout is the reference's: index (i*ny + j)*nz + k, periodic.
+ */
+#ifndef COSMO_GPU_H
+#define COSMO_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cg_ctx cg_ctx;
+
+enum { CG_OK = 0, CG_ERR_ARG = -1, CG_ERR_HIP = -2, CG_ERR_STATE = -3, CG_ERR_NAN = -10 };
+enum { CG_FP64 = 0, CG_FP32_SPLIT = 1 };
+enum { CG_MATTER_NONE = 0, CG_MATTER_STATIC = 1, CG_MATTER_DUST = 2 };
+enum { CG_LAPSE_STATIC = 0, CG_LAPSE_HARMONIC = 1, CG_LAPSE_CONFORMAL_FLRW = 2 };
+
+/* Construction parameters.  Mirrors the compile-time macros and config keys
+ * the reference reads: COSMO_N/NX/NY/NZ and STENCIL_ORDER
+ * (cosmo_macros.h:12-40), dx and dt_frac (cosmo.cc:41-42),
+ * KO_damping_coefficient, a_adj_amp, k_damping_amp, normalize_metric
+ * (bssn.cc:15-22), lapse (BSSNGaugeHandler.h:175), the matter type
+ * (simulation_type, cosmo.cc:52-81) and Lambda (lambda.cc:16-24). */
+typedef struct {
+  int nx, ny, nz;
+  int stencil_order;      /* 2, 4, 6, 8 */
+  int precision;          /* CG_FP64 or CG_FP32_SPLIT */
+  double dx, dt;
+  double ko_coeff;
+  double a_adj_amp;
+  double k_damping_amp;
+  int normalize_metric;
+  int lapse;
+  int matter;
+  double lambda;
+  int device;             /* HIP device ordinal */
+  int tile_i, tile_j, tile_k; /* 0 = autotuned/default */
+} cg_params;
+
+/* BSSN::BSSN(ConfigParser*, Fourier*) (components/bssn/bssn.cc:13-44):
+ * allocate all registers, zero-filled. */
+int cg_create(const cg_params* params, cg_ctx** out);
+/* BSSN::~BSSN (bssn.cc:46-54) */
+int cg_destroy(cg_ctx* ctx);
+const char* cg_last_error(const cg_ctx* ctx);
+
+/* Writes / reads through the reference's `fields` map (bssn.h:43):
+ * n must equal nx*ny*nz.  FP32-split contexts convert at the boundary. */
+int cg_field_upload(cg_ctx* ctx, const char* name, const double* host, size_t n);
+int cg_field_download(cg_ctx* ctx, const char* name, double* host, size_t n);
+
+/* FRW<real_t> reference integrator owned by BSSN (bssn.h:46, utils/FRW.h):
+ * set_phi/set_K (FRW.h:48-55), addFluid (FRW.h:60-76), getters (FRW.h:79-83). */
+int cg_frw_set(cg_ctx* ctx, double phi, double K);
+int cg_frw_add_fluid(cg_ctx* ctx, double rho, double w);
+int cg_frw_get(const cg_ctx* ctx, double out_phi_K_rho_S[4]);
+/* Lambda::setLambda (components/Lambda/lambda.cc:16-19) */
+int cg_set_lambda(cg_ctx* ctx, double lambda);
+
+/* Method-level drop-ins (unfused; each streams the grid once):
+ * BSSN::stepInit (bssn.cc:219-226), BSSN::RKEvolve with the matter source
+ * of the context folded in (bssn.cc:231-243 after clearSrc/addBSSNSrc,
+ * static.cc:19-36, lambda.cc:26-43), BSSN::K{n}Finalize (bssn.cc:267-305). */
+int cg_step_init(cg_ctx* ctx);
+int cg_rk_evolve(cg_ctx* ctx);
+int cg_k_finalize(cg_ctx* ctx, int n);
+
+/* One or more whole RK4 steps in the StaticSim sequence
+ * (sims/static.cc:74-170: initStaticStep + runStaticStep, without output),
+ * fused into four stage kernels per step.  Returns CG_ERR_NAN if DIFFphi
+ * turned NaN (the guard of sims/sim.cc:179-183). */
+int cg_rk4_steps(cg_ctx* ctx, int nsteps);
+
+/* Synchronise the context's stream. */
+int cg_sync(cg_ctx* ctx);
+
+/* Kernel resource usage of the stage kernel this context launches. */
+int cg_kernel_info(const cg_ctx* ctx, int* vgprs, int* scratch_bytes, int* lds_bytes,
+                   int* tile_i, int* tile_j, int* tile_k);
+
+/* Average duration of the last `n` stage-kernel launches recorded with HIP
+ * events on the context's stream (ms); enable with cg_set_timing(ctx, 1). */
+int cg_set_timing(cg_ctx* ctx, int on);
+int cg_stage_time_ms(cg_ctx* ctx, double* avg_ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
