@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark: BSSN RK4 point-updates/s at 256^3, FP32 split vs FP64 GPU.
+
+Workload (the 256^3 split-precision configuration): BSSN + static dust on a
+periodic 256^3 grid, box L = 1, dx = 1/256, dt = 0.1 dx, 6th-order centred
+stencils, synchronous gauge, delta ~ P(k) ~ k^-3 on 1 <= |k| <= N/4 with
+rms 1e-2 (seed 9), constraint-solved metric.  A "step" is one whole RK4 step
+(stepInit + 4 fused stage kernels) of the whole grid, inputs resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--n 256] [--order 6]
+
+Prints ONE JSON line.  `value` is the FP32-split rate; the FP64 drop-in is
+timed on the same initial data and reported beside it (`fp64_value`,
+`fp32_over_fp64`), with the relative L2 difference of every field between
+the two after the same number of steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FIELDS = ["DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIFFgamma22", "DIFFgamma23",
+          "DIFFgamma33", "DIFFphi", "A11", "A12", "A13", "A22", "A23", "A33", "DIFFK",
+          "Gamma1", "Gamma2", "Gamma3", "DIFFalpha"]
+HBM_PEAK_GBS = 8000.0
+F_FIELDS, G_FIELDS = 18, 1  # evolved fields, pointwise matter fields read per stage
+
+
+def b_alg_words():
+    """Algorithmic words per point-update for the per-stage-fused 4-register
+    RK4: stage 1 reads p, writes f and a' (3F); stages 2-3 read a, p, f and
+    write f, a' (5F each); stage 4 reads a, p, f and writes p (4F); every
+    stage reads the G pointwise matter fields."""
+    return 17 * F_FIELDS + 4 * G_FIELDS
+
+
+def make_ctx(cg, n, order, precision, ics):
+    fields, frw = ics
+    g = cg.BSSN(cg.GpuParams(n=n, order=order, precision=precision, matter="static"))
+    for name, arr in fields.items():
+        g.upload(name, arr)
+    g.frw_set(frw["phi"], frw["K"])
+    for rho, w in frw["fluids"]:
+        g.frw_add_fluid(rho, w)
+    return g
+
+
+def time_steps(g, warmup, steps):
+    g.step(warmup)
+    g.sync()
+    g.set_timing(True)
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    t1 = time.perf_counter()
+    g.set_timing(False)
+    avg_ms, nlaunch = g.stage_time_ms()
+    return t1 - t0, avg_ms, nlaunch
+
+
+def cpu_baseline(n_sample, order, spectral_index, rms, seed):
+    """The CPU oracle (oracle/liboracle.so, OpenMP) on a bounded sample: one
+    RK4 step of the same physics on an n_sample^3 grid."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from cosmograph_amd.ics import static_dust_ics
+    fields, frw = static_dust_ics(n_sample, spectral_index, rms, seed)
+    s = pyoracle.Sim(pyoracle.make_params(n_sample, order=order))
+    for name, arr in fields.items():
+        s.field(name)[:] = arr
+    s.frw_set(frw["phi"], frw["K"])
+    for rho, w in frw["fluids"]:
+        s.frw_add_fluid(rho, w)
+    t0 = time.perf_counter()
+    s.run_steps(1)
+    dt = time.perf_counter() - t0
+    threads = int(pyoracle.lib().or_num_threads())
+    return {"value": n_sample ** 3 / dt, "unit": "point-updates/s", "cores": threads, "kind": "port",
+            "sample": f"1 RK4 step of the oracle (C restatement, OpenMP, {threads} threads) on a "
+                      f"{n_sample}^3 grid, same physics/order; {dt:.1f} s"}
+
+
+def copy_probe(gib):
+    """One device-to-device copy of a known size through the HIP runtime, so
+    that FETCH_SIZE / WRITE_SIZE can be scaled against known bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = int(gib * (1 << 30))
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(n)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(dst, src, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_probe_bytes": n}))
+
+
+def load_traffic(workload):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh).get("hbm_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--order", type=int, default=6)
+    ap.add_argument("--seed", type=int, default=9)
+    ap.add_argument("--rms", type=float, default=1e-2)
+    ap.add_argument("--spectral-index", type=float, default=-3.0)
+    ap.add_argument("--cpu-sample", type=int, default=128, help="grid side of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run a device-to-device copy of this many GiB (PMC byte-counter calibration)")
+    args = ap.parse_args()
+
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU (slab decomposition with "
+              "halo exchange) is a design only, see DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    import cosmograph_amd as cg
+    n, order = args.n, args.order
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    log(f"[bench] ICs {n}^3 P(k)~k^{args.spectral_index} rms {args.rms} seed {args.seed}")
+    ics = cg.static_dust_ics(n, args.spectral_index, args.rms, args.seed)
+
+    log("[bench] FP32 split")
+    g32 = make_ctx(cg, n, order, "fp32_split", ics)
+    t32, k32_ms, nl32 = time_steps(g32, args.warmup, args.steps)
+    info32 = g32.kernel_info()
+    pts = n ** 3
+    rate32 = pts * args.steps / t32
+
+    res = {}
+    if not args.skip_fp64:
+        log("[bench] FP64 drop-in")
+        g64 = make_ctx(cg, n, order, "fp64", ics)
+        t64, k64_ms, _ = time_steps(g64, args.warmup, args.steps)
+        rate64 = pts * args.steps / t64
+        errs = {}
+        for f in FIELDS:
+            a, b = g32.download(f + "_p"), g64.download(f + "_p")
+            nb = float(np.sqrt(np.sum(b * b)))
+            d = float(np.sqrt(np.sum((a - b) ** 2)))
+            errs[f] = d / nb if nb > 0 else d
+        res = {"fp64_value": rate64, "fp64_ms_per_step": 1e3 * t64 / args.steps,
+               "fp64_stage_kernel_ms": k64_ms, "fp32_over_fp64": rate32 / rate64,
+               "max_rel_l2_fp32_vs_fp64": max(errs.values()),
+               "rel_l2_fp32_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+               "fp64_kernel": g64.kernel_info()}
+        del g64
+
+    words = b_alg_words()
+    bytes_pt_upd = words * 4                       # FP32 storage
+    bytes_per_launch = bytes_pt_upd / 4.0 * pts    # average stage
+    achieved = bytes_per_launch / (k32_ms * 1e-3) / 1e9
+    traffic = load_traffic(f"c3_{n}_o{order}_fp32")
+
+    cpu = None
+    if args.cpu_sample > 0:
+        log(f"[bench] CPU baseline sample {args.cpu_sample}^3")
+        cpu = cpu_baseline(args.cpu_sample, order, args.spectral_index, args.rms, args.seed)
+
+    line = {
+        "metric": "BSSN RK4 point-updates/s at 256^3, FP32-split vs FP64 GPU at <=1e-5 rel L2 err",
+        "value": rate32,
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t32 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 storage+stencils, f64 RK4 accumulation/FRW",
+        "data": "synthetic: seeded Gaussian random field ICs, constraint-solved",
+        "config": {"workload": f"BSSN + static dust, {n}^3 periodic, order {order}, "
+                               f"P(k)~k^{args.spectral_index:g}, rms {args.rms:g}, dt=0.1dx, FP32 split",
+                   "grid": n, "stencil_order": order, "precision": "fp32_split"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+        "b_alg_bytes_per_point_update": bytes_pt_upd,
+        "stage_kernel_avg_ms": k32_ms,
+        "stage_kernel_launches_timed": nl32,
+        "fp32_kernel": info32,
+    }
+    line.update(res)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

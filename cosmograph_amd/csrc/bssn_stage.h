@@ -16,22 +16,26 @@ enum Lapse { LAPSE_STATIC = 0, LAPSE_HARMONIC = 1, LAPSE_CONFORMAL_FLRW = 2 };
 
 // One fused RK4 stage over the whole grid: stencils of the `in` register,
 // BSSN right-hand side, matter sources, and the RK4Register update
-// (utils/RK4Register.h:118-176) in the epilogue.
+// (utils/RK4Register.h:118-176) in the epilogue.  Register pointers are
+// resolved on the host; the DIFFphi registers are always FP64 (see
+// bssn_stage.hip), every other field uses the context's storage type.
 struct StageArgs {
-  void* base;             // registers [field][reg][point], storage type TS
-  long fstride, rstride;  // element strides between fields / registers
-  int r_in, r_p, r_f, r_out;
+  const void* in[NF];     // stencil input (_a register of the stage)
+  const void* p[NF];      // _p register (step start)
+  void* f[NF];            // _f accumulator
+  void* out[NF];          // stage output: next _a, or _p at stage 4, or _c at stage 0
   const void* DIFFD;      // static dust: conserved density perturbation
   int nx, ny, nz;
   int ntx, nty, ntz;      // tile counts per axis
   int per_xcd;            // tiles handled per XCD (XCD-aware block remap)
-  int stage;              // 0: RHS only into r_out; 1..4: fused K{n}Finalize
+  int stage;              // 0: RHS only into out; 1..4: fused K{n}Finalize
   int normalize;          // stage 1: det(gamma)=1 rescale, trace-free A on the fly
   int matter;
   int lapse;
   int use_ko;
   int use_H;              // a_adj_amp or k_damping_amp non-zero
   double inv_dx, inv_dx2, dt;
+  double cf, ca;          // epilogue: f += cf*c ; a' = p + ca*c  (stage 4: p' = f + (cf*c + p))
   double phi_F, K_F, rho_F, S_F;  // FRW values seen by this stage
   double K_avg;
   double lambda;

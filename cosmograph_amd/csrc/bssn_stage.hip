@@ -39,6 +39,17 @@ __device__ __forceinline__ T norm_factor(T g11, T g12, T g13, T g22, T g23, T g3
   return -expm1(log1p(-omd) / T(3));
 }
 
+// element load at a 32-bit element offset from a uniform base pointer, so the
+// address is formed as SGPR base + 32-bit VGPR byte offset
+template <typename T>
+__device__ __forceinline__ T ldo(const T* __restrict__ base, unsigned eoff) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)(eoff * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void sto(T* __restrict__ base, unsigned eoff, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)(eoff * (unsigned)sizeof(T))) = v;
+}
+
 template <typename T>
 __device__ __forceinline__ bool nan_as_float(T v) {
   // numNaNs tests the float-rounded value (utils/math.h:1524-1541)
@@ -47,6 +58,12 @@ __device__ __forceinline__ bool nan_as_float(T v) {
   return (u << 1) > 0xff000000u;
 }
 
+// The conformal metric (DIFFgamma_ij, DIFFphi) is always stored and
+// differentiated in FP64: its second derivatives enter the Ricci tensor with
+// a cancellation of order 1/(k dx)^2 (~1600 for the longest mode at 256^3),
+// so FP32 storage would put its rounding noise straight into A_ij.
+__device__ __host__ constexpr bool metric_field(int f) { return f <= G33 || f == PHI; }
+
 template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK>
 __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   constexpr int NT = TI * TJ * TK;
@@ -54,8 +71,7 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   constexpr int HV = HI * HJ * HK;
   constexpr int SI = HJ * HK, SJ = HK, SK = 1;
   constexpr int NPF = (HV + NT - 1) / NT;
-  __shared__ TC lds[2][HV];
-  __shared__ TC ldn[HV];
+  __shared__ double ldsraw[2][HV];  // double-buffered field tile (FP32 fields use the first half)
 
   const int ntiles = a.ntx * a.nty * a.ntz;
   const int b = blockIdx.x;
@@ -65,13 +81,15 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   const int i0 = tx * TI, j0 = ty * TJ, k0 = tz * TK;
 
   const int tid = threadIdx.x;
+  __builtin_assume(tid >= 0 && tid < NT);
   const int lk = tid % TK, lj = (tid / TK) % TJ, li = tid / (TK * TJ);
   const int px = i0 + li, py = j0 + lj, pz = k0 + lk;
   const bool valid = (px < a.nx) && (py < a.ny) && (pz < a.nz);
-  const long idx = valid ? ((long)px * a.ny + py) * a.nz + pz : 0;
+  const unsigned idx = valid ? (unsigned)((px * a.ny + py) * a.nz + pz) : 0u;
   const int c = ((li + H) * HJ + (lj + H)) * HK + (lk + H);
 
-  // wrapped global offsets of this thread's share of the halo tile
+  // wrapped global offsets of this thread's share of the halo tile; the same
+  // share is used for every field, so per-element data can stay in registers
   unsigned goff[NPF];
 #pragma unroll
   for (int q = 0; q < NPF; ++q) {
@@ -82,74 +100,93 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
     x += x < 0 ? a.nx : 0; x -= x >= a.nx ? a.nx : 0;
     y += y < 0 ? a.ny : 0; y -= y >= a.ny ? a.ny : 0;
     z += z < 0 ? a.nz : 0; z -= z >= a.nz ? a.nz : 0;
-    goff[q] = (unsigned)(((long)x * a.ny + y) * a.nz + z);
+    goff[q] = (unsigned)((x * a.ny + y) * a.nz + z);
   }
 
-  const TS* __restrict__ base = (const TS*)a.base;
-  auto reg = [&](int f, int r) -> const TS* { return base + (long)f * a.fstride + (long)r * a.rstride; };
+  auto INM = [&](int f) -> const double* { return (const double*)a.in[f]; };
+  auto INS = [&](int f) -> const TS* { return (const TS*)a.in[f]; };
   const bool norm = a.normalize != 0;
 
-  // stage-1 normalisation factor at every halo point
+  // stage-1 unit-determinant factor at this thread's halo elements
+  // (bssn.cc:75-99): gamma_ii -> (gamma_ii + t) r, gamma_ij -> gamma_ij r
+  double tq[NPF], rq[NPF];
   if (norm) {
 #pragma unroll
     for (int q = 0; q < NPF; ++q) {
-      const int e = tid + q * NT;
-      if (e < HV) {
-        const unsigned o = goff[q];
-        ldn[e] = norm_factor<TC>((TC)reg(G11, a.r_in)[o], (TC)reg(G12, a.r_in)[o], (TC)reg(G13, a.r_in)[o],
-                                 (TC)reg(G22, a.r_in)[o], (TC)reg(G23, a.r_in)[o], (TC)reg(G33, a.r_in)[o]);
-      }
+      const unsigned o = goff[q];
+      tq[q] = norm_factor<double>(ldo(INM(G11), o), ldo(INM(G12), o), ldo(INM(G13), o), ldo(INM(G22), o),
+                                  ldo(INM(G23), o), ldo(INM(G33), o));
+      rq[q] = 1.0 / (1.0 - tq[q]);
     }
   }
 
+  double pd[NPF];
   TC pf[NPF];
   auto fetch = [&](int f) {
-    const TS* __restrict__ src = reg(f, a.r_in);
+    if (metric_field(f)) {
+      const double* __restrict__ src = INM(f);
 #pragma unroll
-    for (int q = 0; q < NPF; ++q) {
-      const int e = tid + q * NT;
-      pf[q] = (e < HV) ? (TC)src[goff[q]] : TC(0);
+      for (int q = 0; q < NPF; ++q) pd[q] = ldo(src, goff[q]);
+    } else {
+      const TS* __restrict__ src = INS(f);
+#pragma unroll
+      for (int q = 0; q < NPF; ++q) pf[q] = (TC)ldo(src, goff[q]);
     }
   };
-  // metric slot (0..5) or -1: applies the unit-determinant rescale (bssn.cc:94-99)
-  auto store = [&](TC* __restrict__ dst, int gslot) {
+  auto store = [&](int buf, int f) {
+    if (metric_field(f)) {
+      double* __restrict__ dst = ldsraw[buf];
 #pragma unroll
-    for (int q = 0; q < NPF; ++q) {
-      const int e = tid + q * NT;
-      if (e < HV) {
-        TC v = pf[q];
-        if (norm && gslot >= 0) {
-          const TC tn = ldn[e];
-          v = (v + (diag6(gslot) ? tn : TC(0))) / (TC(1) - tn);
+      for (int q = 0; q < NPF; ++q) {
+        const int e = tid + q * NT;
+        if (e < HV) {
+          double v = pd[q];
+          if (norm && f <= G33) v = (v + (diag6(f) ? tq[q] : 0.0)) * rq[q];
+          dst[e] = v;
         }
-        dst[e] = v;
+      }
+    } else {
+      TC* __restrict__ dst = reinterpret_cast<TC*>(ldsraw[buf]);
+#pragma unroll
+      for (int q = 0; q < NPF; ++q) {
+        const int e = tid + q * NT;
+        if (e < HV) dst[e] = pf[q];
       }
     }
   };
 
   // ---------------------------------------------------------- point values
   TC v[NF];
+  double gc[7];  // metric at the point in FP64 (DIFFgamma_ij, DIFFphi)
 #pragma unroll
-  for (int f = 0; f < NF; ++f) v[f] = valid ? (TC)reg(f, a.r_in)[idx] : TC(0);
+  for (int f = 0; f < 6; ++f) gc[f] = ldo(INM(f), idx);  // idx = 0 (in bounds) outside the grid
+  const double phi_c = ldo(INM(PHI), idx);
+  gc[6] = phi_c;
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+    if (!metric_field(f)) v[f] = (TC)ldo(INS(f), idx);
   if (norm) {
     // bssn.cc:94-117 at this point
-    const TC tn = norm_factor<TC>(v[G11], v[G12], v[G13], v[G22], v[G23], v[G33]);
-    const TC den = TC(1) - tn;
+    const double tn = norm_factor<double>(gc[0], gc[1], gc[2], gc[3], gc[4], gc[5]);
+    const double rden = 1.0 / (1.0 - tn);
 #pragma unroll
-    for (int s = 0; s < 6; ++s) v[G11 + s] = (v[G11 + s] + (diag6(s) ? tn : TC(0))) / den;
-    const TC g11 = v[G11], g12 = v[G12], g13 = v[G13], g22 = v[G22], g23 = v[G23], g33 = v[G33];
-    const TC i11 = TC(1) + g22 + g33 - g23 * g23 + g22 * g33;
-    const TC i22 = TC(1) + g11 + g33 - g13 * g13 + g11 * g33;
-    const TC i33 = TC(1) + g11 + g22 - g12 * g12 + g11 * g22;
-    const TC i12 = g13 * g23 - g12 * (TC(1) + g33);
-    const TC i13 = g12 * g23 - g13 * (TC(1) + g22);
-    const TC i23 = g12 * g13 - g23 * (TC(1) + g11);
-    const TC trA = i11 * v[A11] + i22 * v[A22] + i33 * v[A33] +
-                   TC(2) * (i12 * v[A12] + i13 * v[A13] + i23 * v[A23]);
-    const TC third = TC(1.0 / 3.0);
+    for (int s = 0; s < 6; ++s) gc[s] = (gc[s] + (diag6(s) ? tn : 0.0)) * rden;
+    const double g11 = gc[0], g12 = gc[1], g13 = gc[2], g22 = gc[3], g23 = gc[4], g33 = gc[5];
+    const double i11 = 1.0 + g22 + g33 - g23 * g23 + g22 * g33;
+    const double i22 = 1.0 + g11 + g33 - g13 * g13 + g11 * g33;
+    const double i33 = 1.0 + g11 + g22 - g12 * g12 + g11 * g22;
+    const double i12 = g13 * g23 - g12 * (1.0 + g33);
+    const double i13 = g12 * g23 - g13 * (1.0 + g22);
+    const double i23 = g12 * g13 - g23 * (1.0 + g11);
+    const double trA = i11 * v[A11] + i22 * v[A22] + i33 * v[A33] +
+                       2.0 * (i12 * v[A12] + i13 * v[A13] + i23 * v[A23]);
 #pragma unroll
-    for (int s = 0; s < 6; ++s) v[A11 + s] = (v[A11 + s] - third * ((diag6(s) ? TC(1) : TC(0)) + v[G11 + s]) * trA) / den;
+    for (int s = 0; s < 6; ++s)
+      v[A11 + s] = (TC)(((double)v[A11 + s] - (1.0 / 3.0) * ((diag6(s) ? 1.0 : 0.0) + gc[s]) * trA) * rden);
   }
+#pragma unroll
+  for (int f = 0; f < 6; ++f) v[f] = (TC)gc[f];
+  v[PHI] = (TC)phi_c;
 
   // inverse conformal metric from DIFF variables (bssn.cc:447-455)
   TC gi[6];
@@ -167,7 +204,8 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   auto AM = [&](int p, int q) -> TC { return v[A11 + sy(p, q)]; };
 
   // ------------------------------------------------ stencil phase (LDS)
-  const TC idx1 = (TC)a.inv_dx, idx2 = (TC)a.inv_dx2;
+  const double i1 = a.inv_dx, i2 = a.inv_dx2;
+  const TC idx1 = (TC)a.inv_dx;
   TC dg[3][6];      // d_k gamma_ij
   TC T1[6];         // gamma^{kl} d_k d_l gamma_ij
   TC dphi[3], ddphi[6], dal[3], ddal[6], dK[3], dG[3][3];
@@ -175,50 +213,61 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
 #pragma unroll
   for (int f = 0; f < NF; ++f) ko[f] = TC(0);
   const bool use_ko = a.use_ko != 0;
-  const TC kop = (TC)a.ko_pref;
   constexpr int KO_ORDER = (H > R) ? 2 * R : 0;
 
-  // fields staged in order; A_ij only when KO is active
-  constexpr int SEQ[NF] = {G11, G12, G13, G22, G23, G33, PHI, ALPHA, KK, GAM1, GAM2, GAM3,
-                           A11, A12, A13, A22, A23, A33};
-  const int nseq = use_ko ? NF : 12;
+  // stencil fields in order; A_ij only when KO is active
+  constexpr int NSEQ = 18;
+  constexpr int SEQ[NSEQ] = {PHI, G11, G12, G13, G22, G23, G33, ALPHA, KK, GAM1, GAM2, GAM3,
+                             A11, A12, A13, A22, A23, A33};
+  const int nseq = use_ko ? NSEQ : 12;
 
   fetch(SEQ[0]);
-  store(lds[0], 0);
+  store(0, SEQ[0]);
   __syncthreads();
 
 #pragma unroll
-  for (int s = 0; s < NF; ++s) {
+  for (int s = 0; s < NSEQ; ++s) {
     if (s < nseq) {
       const bool more = (s + 1) < nseq;
-      if (more) fetch(SEQ[(s + 1) < NF ? s + 1 : 0]);
-      const TC* __restrict__ L = lds[s & 1];
+      const int fn = SEQ[(s + 1) < NSEQ ? s + 1 : 0];
+      if (more) fetch(fn);
       const int f = SEQ[s];
       if (valid) {
-        if (KO_ORDER && use_ko) ko[f] = kop * ko_sum<KO_ORDER, TC>(L, c, SI, SJ, SK);
-        if (f <= G33 || f == PHI || f == ALPHA) {
-          const TC d0 = fd1<R>(L, c, SI) * idx1, d1 = fd1<R>(L, c, SJ) * idx1, d2 = fd1<R>(L, c, SK) * idx1;
-          const TC e00 = fd2<R>(L, c, SI) * idx2, e11 = fd2<R>(L, c, SJ) * idx2, e22 = fd2<R>(L, c, SK) * idx2;
-          const TC e01 = fdm<R>(L, c, SI, SJ) * idx2, e02 = fdm<R>(L, c, SI, SK) * idx2,
-                   e12 = fdm<R>(L, c, SJ, SK) * idx2;
-          if (f <= G33) {
-            dg[0][f] = d0; dg[1][f] = d1; dg[2][f] = d2;
-            T1[f] = gi[0] * e00 + gi[3] * e11 + gi[5] * e22 + TC(2) * (gi[1] * e01 + gi[2] * e02 + gi[4] * e12);
-          } else if (f == PHI) {
-            dphi[0] = d0; dphi[1] = d1; dphi[2] = d2;
-            ddphi[0] = e00; ddphi[1] = e01; ddphi[2] = e02; ddphi[3] = e11; ddphi[4] = e12; ddphi[5] = e22;
+        if (metric_field(f)) {
+          const double* __restrict__ L = ldsraw[s & 1];
+          if (KO_ORDER && use_ko) ko[f] = (TC)(a.ko_pref * ko_sum<KO_ORDER, double>(L, c, SI, SJ, SK));
+          const double d0 = fd1<R>(L, c, SI) * i1, d1 = fd1<R>(L, c, SJ) * i1, d2 = fd1<R>(L, c, SK) * i1;
+          const double e00 = fd2<R>(L, c, SI) * i2, e11 = fd2<R>(L, c, SJ) * i2, e22 = fd2<R>(L, c, SK) * i2;
+          const double e01 = fdm<R>(L, c, SI, SJ) * i2, e02 = fdm<R>(L, c, SI, SK) * i2,
+                       e12 = fdm<R>(L, c, SJ, SK) * i2;
+          if (f == PHI) {
+            dphi[0] = (TC)d0; dphi[1] = (TC)d1; dphi[2] = (TC)d2;
+            ddphi[0] = (TC)e00; ddphi[1] = (TC)e01; ddphi[2] = (TC)e02;
+            ddphi[3] = (TC)e11; ddphi[4] = (TC)e12; ddphi[5] = (TC)e22;
           } else {
-            dal[0] = d0; dal[1] = d1; dal[2] = d2;
-            ddal[0] = e00; ddal[1] = e01; ddal[2] = e02; ddal[3] = e11; ddal[4] = e12; ddal[5] = e22;
+            dg[0][f] = (TC)d0; dg[1][f] = (TC)d1; dg[2][f] = (TC)d2;
+            T1[f] = gi[0] * (TC)e00 + gi[3] * (TC)e11 + gi[5] * (TC)e22 +
+                    TC(2) * (gi[1] * (TC)e01 + gi[2] * (TC)e02 + gi[4] * (TC)e12);
           }
-        } else if (f == KK) {
-          dK[0] = fd1<R>(L, c, SI) * idx1; dK[1] = fd1<R>(L, c, SJ) * idx1; dK[2] = fd1<R>(L, c, SK) * idx1;
-        } else if (f >= GAM1 && f <= GAM3) {
-          const int kk = f - GAM1;
-          dG[0][kk] = fd1<R>(L, c, SI) * idx1; dG[1][kk] = fd1<R>(L, c, SJ) * idx1; dG[2][kk] = fd1<R>(L, c, SK) * idx1;
+        } else {
+          const TC* __restrict__ L = reinterpret_cast<const TC*>(ldsraw[s & 1]);
+          if (KO_ORDER && use_ko) ko[f] = (TC)a.ko_pref * ko_sum<KO_ORDER, TC>(L, c, SI, SJ, SK);
+          if (f == ALPHA) {
+            const TC idx2 = (TC)a.inv_dx2;
+            dal[0] = fd1<R>(L, c, SI) * idx1; dal[1] = fd1<R>(L, c, SJ) * idx1; dal[2] = fd1<R>(L, c, SK) * idx1;
+            ddal[0] = fd2<R>(L, c, SI) * idx2; ddal[3] = fd2<R>(L, c, SJ) * idx2; ddal[5] = fd2<R>(L, c, SK) * idx2;
+            ddal[1] = fdm<R>(L, c, SI, SJ) * idx2; ddal[2] = fdm<R>(L, c, SI, SK) * idx2;
+            ddal[4] = fdm<R>(L, c, SJ, SK) * idx2;
+          } else if (f == KK) {
+            dK[0] = fd1<R>(L, c, SI) * idx1; dK[1] = fd1<R>(L, c, SJ) * idx1; dK[2] = fd1<R>(L, c, SK) * idx1;
+          } else if (f >= GAM1 && f <= GAM3) {
+            const int kk = f - GAM1;
+            dG[0][kk] = fd1<R>(L, c, SI) * idx1; dG[1][kk] = fd1<R>(L, c, SJ) * idx1;
+            dG[2][kk] = fd1<R>(L, c, SK) * idx1;
+          }
         }
       }
-      if (more) store(lds[(s + 1) & 1], (s + 1) < 6 ? s + 1 : -1);
+      if (more) store((s + 1) & 1, fn);
       __syncthreads();
     }
   }
@@ -228,14 +277,14 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   const TC phiF = (TC)a.phi_F, KF = (TC)a.K_F, rhoF = (TC)a.rho_F, SF = (TC)a.S_F;
   const TC PI = TC(3.14159265358979323846);
   const TC alpha = v[ALPHA] + TC(1);
-  const TC phi = v[PHI] + phiF;
+  const TC phi = (TC)(phi_c + a.phi_F);
   const TC K = v[KK] + KF;
 
   // sources (components/static/static.cc:28-35, components/Lambda/lambda.cc:32-41)
   TC DIFFr = TC(0), DIFFS = TC(0);
   if (a.matter == MATTER_STATIC) {
-    const TC D = (TC)((const TS*)a.DIFFD)[idx];
-    DIFFr = exp(TC(-6) * (v[PHI] + phiF)) * D + rhoF * expm1(TC(-6) * v[PHI]);
+    const TC D = (TC)ldo((const TS*)a.DIFFD, idx);
+    DIFFr = exp((TC)(-6.0 * (phi_c + a.phi_F))) * D + rhoF * expm1(TC(-6) * v[PHI]);
   }
   DIFFr += (TC)a.lambda;
   DIFFS += TC(-3) * (TC)a.lambda;
@@ -415,30 +464,40 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   rhs[ALPHA] = lapse - ko[ALPHA];
 
   // ------------------------------------------------ RK4Register epilogue
-  TS* __restrict__ wbase = (TS*)a.base;
-  auto wreg = [&](int f, int r) -> TS* { return wbase + (long)f * a.fstride + (long)r * a.rstride; };
-  const double hdt = a.dt;
+  // K{n}Finalize (RK4Register.h:129-176): f += cf*c ; a' = p + ca*c, and at
+  // stage 4 p' = f + (cf*c + p).  Accumulation in FP64 for every precision.
   const int st = a.stage;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     const double cc = (double)rhs[f];
-    if (st == 0) {
-      wreg(f, a.r_out)[idx] = (TS)cc;
-    } else if (st == 1) {
-      const double p = (double)reg(f, a.r_p)[idx];
-      wreg(f, a.r_f)[idx] = (TS)(hdt * cc / 6.0);
-      wreg(f, a.r_out)[idx] = (TS)(p + hdt * cc / 2.0);
-    } else if (st == 2 || st == 3) {
-      const double p = (double)reg(f, a.r_p)[idx];
-      const double fo = (double)reg(f, a.r_f)[idx];
-      wreg(f, a.r_f)[idx] = (TS)(fo + hdt * cc / 3.0);
-      wreg(f, a.r_out)[idx] = (TS)(st == 2 ? p + hdt * cc / 2.0 : p + hdt * cc);
+    if (metric_field(f)) {
+      double* __restrict__ o = (double*)a.out[f];
+      if (st == 0) { sto(o, idx, cc); continue; }
+      const double p = ldo((const double*)a.p[f], idx);
+      double* __restrict__ fr = (double*)a.f[f];
+      const double fo = st >= 2 ? ldo((const double*)fr, idx) : 0.0;
+      const double inc = a.cf * cc;
+      if (st == 4) {
+        const double pn = fo + (inc + p);
+        sto(o, idx, pn);
+        if (f == PHI && nan_as_float(pn)) atomicAdd(a.nan_count, 1u);
+      } else {
+        sto(fr, idx, fo + inc);
+        sto(o, idx, p + a.ca * cc);
+      }
     } else {
-      const double p = (double)reg(f, a.r_p)[idx];
-      const double fo = (double)reg(f, a.r_f)[idx];
-      const double pn = fo + (hdt * cc / 6.0 + p);
-      wreg(f, a.r_out)[idx] = (TS)pn;
-      if (f == PHI && nan_as_float(pn)) atomicAdd(a.nan_count, 1u);
+      TS* __restrict__ o = (TS*)a.out[f];
+      if (st == 0) { sto(o, idx, (TS)cc); continue; }
+      const double p = (double)ldo((const TS*)a.p[f], idx);
+      TS* __restrict__ fr = (TS*)a.f[f];
+      const double fo = st >= 2 ? (double)ldo((const TS*)fr, idx) : 0.0;
+      const double inc = a.cf * cc;
+      if (st == 4) {
+        sto(o, idx, (TS)(fo + (inc + p)));
+      } else {
+        sto(fr, idx, (TS)(fo + inc));
+        sto(o, idx, (TS)(p + a.ca * cc));
+      }
     }
   }
 }
@@ -475,7 +534,7 @@ static int info_t(int* vgprs, int* spills, int* lds) {
   CG_SHAPES(X, TS, TC, 2, 2)  \
   CG_SHAPES(X, TS, TC, 3, 3)  \
   CG_SHAPES(X, TS, TC, 4, 4)  \
-  CG_SHAPES(X, TS, TC, 4, 5)
+  X(TS, TC, 4, 5, 4, 8, 8)
 
 #define CG_CASE_LAUNCH(TS, TC, R, H, A, B, C)                                    \
   if (order == 2 * R && halo == H && shape.ti == A && shape.tj == B && shape.tk == C) \

@@ -92,65 +92,72 @@ const char* kFieldNames[NF] = {"DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIF
                                "Gamma2",      "Gamma3",      "DIFFalpha"};
 
 // ------------------------------------------------- elementwise kernels --
+// Register table: slot pointers of every field (DIFFphi always FP64).
+struct RegTable {
+  void* r[NF][4];
+};
+
+// the conformal metric (DIFFgamma_ij, DIFFphi) is stored in FP64 in every mode
+__device__ __host__ inline bool is_metric(int f) { return f <= G33 || f == PHI; }
+
 template <typename TS>
-__global__ void k_step_init(TS* base, long fstride, long rstride, int rp, int ra, int rf, long n, int normalize) {
+__device__ __forceinline__ double ld(const RegTable& t, int f, int s, long x) {
+  return is_metric(f) ? ((const double*)t.r[f][s])[x] : (double)((const TS*)t.r[f][s])[x];
+}
+template <typename TS>
+__device__ __forceinline__ void st(const RegTable& t, int f, int s, long x, double v) {
+  if (is_metric(f)) ((double*)t.r[f][s])[x] = v;
+  else ((TS*)t.r[f][s])[x] = (TS)v;
+}
+
+template <typename TS>
+__global__ void k_step_init(RegTable t, int rp, int ra, int rf, long n, int normalize) {
   const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
-  auto R = [&](int f, int r) -> TS* { return base + (long)f * fstride + (long)r * rstride; };
   // RK4Register::stepInit (RK4Register.h:118-127)
   double v[NF];
   for (int f = 0; f < NF; ++f) {
-    v[f] = (double)R(f, rp)[x];
-    R(f, rf)[x] = (TS)0;
+    v[f] = ld<TS>(t, f, rp, x);
+    st<TS>(t, f, rf, x, 0.0);
   }
   if (normalize) {
     // BSSN::set_DIFFgamma_Aij_norm (bssn.cc:61-119)
     double g11 = v[G11], g12 = v[G12], g13 = v[G13], g22 = v[G22], g23 = v[G23], g33 = v[G33];
     double omd = -1.0 * (g11 + g22 + g33 - g12 * g12 - g13 * g13 - g23 * g23 + g11 * g22 + g11 * g33 + g22 * g33 -
                          g23 * g23 * g11 - g13 * g13 * g22 - g12 * g12 * g33 + 2.0 * g12 * g13 * g23 + g11 * g22 * g33);
-    double t = -1.0 * expm1(log1p(-1.0 * omd) / 3.0);
-    g11 = (g11 + t) / (1.0 - t); g22 = (g22 + t) / (1.0 - t); g33 = (g33 + t) / (1.0 - t);
-    g12 = g12 / (1.0 - t); g13 = g13 / (1.0 - t); g23 = g23 / (1.0 - t);
+    double tn = -1.0 * expm1(log1p(-1.0 * omd) / 3.0);
+    g11 = (g11 + tn) / (1.0 - tn); g22 = (g22 + tn) / (1.0 - tn); g33 = (g33 + tn) / (1.0 - tn);
+    g12 = g12 / (1.0 - tn); g13 = g13 / (1.0 - tn); g23 = g23 / (1.0 - tn);
     v[G11] = g11; v[G12] = g12; v[G13] = g13; v[G22] = g22; v[G23] = g23; v[G33] = g33;
     double i11 = 1.0 + g22 + g33 - g23 * g23 + g22 * g33, i22 = 1.0 + g11 + g33 - g13 * g13 + g11 * g33;
     double i33 = 1.0 + g11 + g22 - g12 * g12 + g11 * g22, i12 = g13 * g23 - g12 * (1.0 + g33);
     double i13 = g12 * g23 - g13 * (1.0 + g22), i23 = g12 * g13 - g23 * (1.0 + g11);
     double trA = i11 * v[A11] + i22 * v[A22] + i33 * v[A33] + 2.0 * (i12 * v[A12] + i13 * v[A13] + i23 * v[A23]);
-    v[A11] = (v[A11] - 1.0 / 3.0 * (1.0 + g11) * trA) / (1.0 - t);
-    v[A22] = (v[A22] - 1.0 / 3.0 * (1.0 + g22) * trA) / (1.0 - t);
-    v[A33] = (v[A33] - 1.0 / 3.0 * (1.0 + g33) * trA) / (1.0 - t);
-    v[A12] = (v[A12] - 1.0 / 3.0 * g12 * trA) / (1.0 - t);
-    v[A13] = (v[A13] - 1.0 / 3.0 * g13 * trA) / (1.0 - t);
-    v[A23] = (v[A23] - 1.0 / 3.0 * g23 * trA) / (1.0 - t);
+    v[A11] = (v[A11] - 1.0 / 3.0 * (1.0 + g11) * trA) / (1.0 - tn);
+    v[A22] = (v[A22] - 1.0 / 3.0 * (1.0 + g22) * trA) / (1.0 - tn);
+    v[A33] = (v[A33] - 1.0 / 3.0 * (1.0 + g33) * trA) / (1.0 - tn);
+    v[A12] = (v[A12] - 1.0 / 3.0 * g12 * trA) / (1.0 - tn);
+    v[A13] = (v[A13] - 1.0 / 3.0 * g13 * trA) / (1.0 - tn);
+    v[A23] = (v[A23] - 1.0 / 3.0 * g23 * trA) / (1.0 - tn);
   }
-  for (int f = 0; f < NF; ++f) R(f, ra)[x] = (TS)v[f];
+  for (int f = 0; f < NF; ++f) st<TS>(t, f, ra, x, v[f]);
 }
 
 // RK4Register::K{n}Finalize (RK4Register.h:129-176) without the pointer swap
 template <typename TS>
-__global__ void k_finalize(TS* base, long fstride, long rstride, int rp, int rc, int rf, long n, int stage,
-                           double dt) {
+__global__ void k_finalize(RegTable t, int rp, int rc, int rf, long n, int stage, double dt) {
   const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
   for (int f = 0; f < NF; ++f) {
-    TS* P = base + (long)f * fstride + (long)rp * rstride;
-    TS* Cc = base + (long)f * fstride + (long)rc * rstride;
-    TS* F = base + (long)f * fstride + (long)rf * rstride;
-    const double c = (double)Cc[x], p = (double)P[x], fo = (double)F[x];
-    if (stage == 1) { F[x] = (TS)(fo + dt * c / 6.0); Cc[x] = (TS)(p + dt * c / 2.0); }
-    else if (stage == 2) { F[x] = (TS)(fo + dt * c / 3.0); Cc[x] = (TS)(p + dt * c / 2.0); }
-    else if (stage == 3) { F[x] = (TS)(fo + dt * c / 3.0); Cc[x] = (TS)(p + dt * c); }
+    const double c = ld<TS>(t, f, rc, x), p = ld<TS>(t, f, rp, x), fo = ld<TS>(t, f, rf, x);
+    if (stage == 1) { st<TS>(t, f, rf, x, fo + dt * c / 6.0); st<TS>(t, f, rc, x, p + dt * c / 2.0); }
+    else if (stage == 2) { st<TS>(t, f, rf, x, fo + dt * c / 3.0); st<TS>(t, f, rc, x, p + dt * c / 2.0); }
+    else if (stage == 3) { st<TS>(t, f, rf, x, fo + dt * c / 3.0); st<TS>(t, f, rc, x, p + dt * c); }
     else {
       const double fn = fo + (dt * c / 6.0 + p);
-      F[x] = (TS)fn; Cc[x] = (TS)fn; P[x] = (TS)fn;
+      st<TS>(t, f, rf, x, fn); st<TS>(t, f, rc, x, fn); st<TS>(t, f, rp, x, fn);
     }
   }
-}
-
-template <typename TS>
-__global__ void k_zero(TS* p, long n) {
-  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x < n) p[x] = (TS)0;
 }
 
 }  // namespace
@@ -163,7 +170,8 @@ struct cg_ctx {
   int R = 2, H = 2;
   KernelShape shape{4, 8, 8};
   long npts = 0, fstride = 0, rstride = 0;
-  void* regs = nullptr;   // [NF][4][npts]
+  void* regs = nullptr;   // [11][4][npts] A_ij, DIFFK, Gamma^i, DIFFalpha in the storage type
+  double* met = nullptr;  // [7][4][npts] DIFFgamma11..33, DIFFphi registers, always FP64
   void* DIFFD = nullptr;
   unsigned* nan_dev = nullptr;
   unsigned nan_seen = 0;
@@ -189,7 +197,15 @@ static int fail(cg_ctx* c, int code, const std::string& msg) {
   } while (0)
 
 static void* reg_ptr(cg_ctx* c, int f, int slot) {
-  return (char*)c->regs + ((long)f * c->fstride + (long)slot * c->rstride) * (long)c->ts;
+  if (is_metric(f)) return c->met + ((long)f * 4 + slot) * c->rstride;  // f = 0..6
+  return (char*)c->regs + ((long)(f - 7) * c->fstride + (long)slot * c->rstride) * (long)c->ts;  // f = 7..17
+}
+
+static RegTable reg_table(cg_ctx* c) {
+  RegTable t;
+  for (int f = 0; f < NF; ++f)
+    for (int s = 0; s < 4; ++s) t.r[f][s] = reg_ptr(c, f, s);
+  return t;
 }
 
 static int halo_for(int order, double ko) {
@@ -236,8 +252,10 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   *out = c;
   HIPCHK(c, hipSetDevice(p.device));
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIPCHK(c, hipMalloc(&c->regs, (size_t)NF * 4 * pad * c->ts));
-  HIPCHK(c, hipMemsetAsync(c->regs, 0, (size_t)NF * 4 * pad * c->ts, c->stream));
+  HIPCHK(c, hipMalloc(&c->regs, (size_t)(NF - 7) * 4 * pad * c->ts));
+  HIPCHK(c, hipMemsetAsync(c->regs, 0, (size_t)(NF - 7) * 4 * pad * c->ts, c->stream));
+  HIPCHK(c, hipMalloc((void**)&c->met, (size_t)7 * 4 * pad * sizeof(double)));
+  HIPCHK(c, hipMemsetAsync(c->met, 0, (size_t)7 * 4 * pad * sizeof(double), c->stream));
   HIPCHK(c, hipMalloc(&c->DIFFD, (size_t)pad * c->ts));
   HIPCHK(c, hipMemsetAsync(c->DIFFD, 0, (size_t)pad * c->ts, c->stream));
   HIPCHK(c, hipMalloc(&c->nan_dev, sizeof(unsigned)));
@@ -252,15 +270,17 @@ int cg_destroy(cg_ctx* c) {
   for (auto& e : c->ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
   if (c->regs) (void)hipFree(c->regs);
   if (c->DIFFD) (void)hipFree(c->DIFFD);
+  if (c->met) (void)hipFree(c->met);
   if (c->nan_dev) (void)hipFree(c->nan_dev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CG_OK;
 }
 
-// resolve a reference field-map key to a device pointer
-static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err) {
+// resolve a reference field-map key to a device pointer and element size
+static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err, size_t* esz) {
   *err = 0;
+  *esz = c->ts;
   std::string s(name ? name : "");
   if (s == "DIFFD_a") return c->DIFFD;
   if (s.size() < 3 || s[s.size() - 2] != '_') { *err = 1; return nullptr; }
@@ -275,6 +295,7 @@ static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err) {
     else if (r == 'c') slot = c->sC;
     else { *err = 1; return nullptr; }
     if (for_write && c->a_is_p && (r == 'a' || r == 'f')) { *err = 2; return nullptr; }
+    if (is_metric(f)) *esz = 8;
     return reg_ptr(c, f, slot);
   }
   *err = 1;
@@ -285,10 +306,11 @@ int cg_field_upload(cg_ctx* c, const char* name, const double* host, size_t n) {
   if (!c || !host) return CG_ERR_ARG;
   if ((long)n != c->npts) return fail(c, CG_ERR_ARG, "size mismatch");
   int e;
-  void* d = resolve(c, name, true, &e);
+  size_t esz;
+  void* d = resolve(c, name, true, &e, &esz);
   if (!d) return fail(c, e == 2 ? CG_ERR_STATE : CG_ERR_ARG,
                       e == 2 ? "_a/_f alias _p after a fused step; write _p" : std::string("unknown field ") + name);
-  if (c->prec == CG_FP64) {
+  if (esz == 8) {
     HIPCHK(c, hipMemcpyAsync(d, host, n * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   } else {
@@ -304,9 +326,10 @@ int cg_field_download(cg_ctx* c, const char* name, double* host, size_t n) {
   if (!c || !host) return CG_ERR_ARG;
   if ((long)n != c->npts) return fail(c, CG_ERR_ARG, "size mismatch");
   int e;
-  void* d = resolve(c, name, false, &e);
+  size_t esz;
+  void* d = resolve(c, name, false, &e, &esz);
   if (!d) return fail(c, CG_ERR_ARG, std::string("unknown field ") + name);
-  if (c->prec == CG_FP64) {
+  if (esz == 8) {
     HIPCHK(c, hipMemcpyAsync(host, d, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   } else {
@@ -341,10 +364,12 @@ int cg_set_lambda(cg_ctx* c, double lambda) {
 
 static StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int normalize) {
   StageArgs a{};
-  a.base = c->regs;
-  a.fstride = c->fstride;
-  a.rstride = c->rstride;
-  a.r_in = rin; a.r_p = rp; a.r_f = rf; a.r_out = rout;
+  for (int f = 0; f < NF; ++f) {
+    a.in[f] = reg_ptr(c, f, rin);
+    a.p[f] = reg_ptr(c, f, rp);
+    a.f[f] = reg_ptr(c, f, rf);
+    a.out[f] = reg_ptr(c, f, rout);
+  }
   a.DIFFD = c->DIFFD;
   a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
   a.ntx = (c->p.nx + c->shape.ti - 1) / c->shape.ti;
@@ -361,6 +386,10 @@ static StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int ro
   a.inv_dx = 1.0 / c->p.dx;
   a.inv_dx2 = 1.0 / (c->p.dx * c->p.dx);
   a.dt = c->p.dt;
+  // RK4Register weights (RK4Register.h:129-176)
+  const double dt = c->p.dt;
+  a.cf = (stage == 1 || stage == 4) ? dt / 6.0 : dt / 3.0;
+  a.ca = (stage == 3) ? dt : dt / 2.0;
   a.phi_F = c->frw.g_phi; a.K_F = c->frw.g_K; a.rho_F = c->frw.g_rho; a.S_F = c->frw.g_S;
   a.K_avg = 0.0;
   a.lambda = c->lambda;
@@ -394,8 +423,8 @@ template <typename TS>
 static int step_init_t(cg_ctx* c) {
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
-  hipLaunchKernelGGL(k_step_init<TS>, dim3(nb), dim3(bs), 0, c->stream, (TS*)c->regs, c->fstride, c->rstride, c->sP,
-                     c->sA, c->sF, c->npts, c->p.normalize_metric);
+  hipLaunchKernelGGL(k_step_init<TS>, dim3(nb), dim3(bs), 0, c->stream, reg_table(c), c->sP, c->sA, c->sF, c->npts,
+                     c->p.normalize_metric);
   HIPCHK(c, hipGetLastError());
   return CG_OK;
 }
@@ -417,8 +446,8 @@ template <typename TS>
 static int finalize_t(cg_ctx* c, int n) {
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
-  hipLaunchKernelGGL(k_finalize<TS>, dim3(nb), dim3(bs), 0, c->stream, (TS*)c->regs, c->fstride, c->rstride, c->sP,
-                     c->sC, c->sF, c->npts, n, c->p.dt);
+  hipLaunchKernelGGL(k_finalize<TS>, dim3(nb), dim3(bs), 0, c->stream, reg_table(c), c->sP, c->sC, c->sF, c->npts, n,
+                     c->p.dt);
   HIPCHK(c, hipGetLastError());
   return CG_OK;
 }

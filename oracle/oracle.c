@@ -175,6 +175,17 @@ void or_destroy(or_sim *s) {
 
 long or_points(const or_sim *s) { return s->npts; }
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+int or_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 /* Reference field-map keys (cosmo_macros.h:154-187): "<field>_<p|a|c|f>" for
  * RK4 registers, "<src>_a" for sources/extras, "DIFFD_a" for static dust.
  * The dust-fluid registers (dust.h:32-35) have no map in the reference; they

@@ -88,6 +88,7 @@ or_sim *or_create(const or_params *p);
 void or_destroy(or_sim *s);
 double *or_field(or_sim *s, const char *name);   /* reference names, e.g. "DIFFphi_p", "DIFFD_a", "D_p" */
 long or_points(const or_sim *s);
+int or_num_threads(void);
 
 /* FRW reference integrator (utils/FRW.h) */
 void or_frw_set(or_sim *s, double phi, double K);

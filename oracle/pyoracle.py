@@ -76,6 +76,7 @@ def lib():
         L.or_field.argtypes = [vp, C.c_char_p]
         L.or_points.restype = C.c_long
         L.or_points.argtypes = [vp]
+        L.or_num_threads.restype = C.c_int
         for name in ["or_bssn_step_init", "or_bssn_rk_evolve", "or_bssn_clear_src",
                      "or_bssn_normalize", "or_bssn_extra_field_data", "or_bssn_prep_output",
                      "or_static_add_src", "or_lambda_add_src", "or_dust_step_init",
