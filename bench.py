@@ -53,7 +53,8 @@ def make_ctx(cg, n, order, precision, ics):
     return g
 
 
-def time_steps(g, warmup, steps):
+def time_steps(g, warmup, steps, autotune=True):
+    tune = g.autotune(3) if autotune else None
     g.step(warmup)
     g.sync()
     g.set_timing(True)
@@ -63,7 +64,7 @@ def time_steps(g, warmup, steps):
     t1 = time.perf_counter()
     g.set_timing(False)
     avg_ms, nlaunch = g.stage_time_ms()
-    return t1 - t0, avg_ms, nlaunch
+    return t1 - t0, avg_ms, nlaunch, tune
 
 
 def cpu_baseline(n_sample, order, spectral_index, rms, seed):
@@ -125,6 +126,7 @@ def main():
     ap.add_argument("--spectral-index", type=float, default=-3.0)
     ap.add_argument("--cpu-sample", type=int, default=128, help="grid side of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -146,8 +148,9 @@ def main():
 
     log("[bench] FP32 split")
     g32 = make_ctx(cg, n, order, "fp32_split", ics)
-    t32, k32_ms, nl32 = time_steps(g32, args.warmup, args.steps)
+    t32, k32_ms, nl32, tune32 = time_steps(g32, args.warmup, args.steps, not args.no_autotune)
     info32 = g32.kernel_info()
+    log(f"[bench] FP32 split: {1e3 * t32 / args.steps:.3f} ms/step, kernel {info32}")
     pts = n ** 3
     rate32 = pts * args.steps / t32
 
@@ -155,7 +158,7 @@ def main():
     if not args.skip_fp64:
         log("[bench] FP64 drop-in")
         g64 = make_ctx(cg, n, order, "fp64", ics)
-        t64, k64_ms, _ = time_steps(g64, args.warmup, args.steps)
+        t64, k64_ms, _, tune64 = time_steps(g64, args.warmup, args.steps, not args.no_autotune)
         rate64 = pts * args.steps / t64
         errs = {}
         for f in FIELDS:
@@ -167,7 +170,7 @@ def main():
                "fp64_stage_kernel_ms": k64_ms, "fp32_over_fp64": rate32 / rate64,
                "max_rel_l2_fp32_vs_fp64": max(errs.values()),
                "rel_l2_fp32_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
-               "fp64_kernel": g64.kernel_info()}
+               "fp64_kernel": g64.kernel_info(), "fp64_autotune": tune64}
         del g64
 
     words = b_alg_words()
@@ -204,6 +207,7 @@ def main():
         "stage_kernel_avg_ms": k32_ms,
         "stage_kernel_launches_timed": nl32,
         "fp32_kernel": info32,
+        "fp32_autotune": tune32,
     }
     line.update(res)
     print(json.dumps(line))

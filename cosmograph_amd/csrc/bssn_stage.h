@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 namespace cg {
 
@@ -45,15 +46,20 @@ struct StageArgs {
   unsigned int* nan_count;
 };
 
-struct KernelShape {
+// One compiled instantiation of the stage kernel.
+struct StageVariant {
+  int prec;               // 0 = FP64, 1 = FP32 split
+  int order, halo;
   int ti, tj, tk;         // tile extents (i slowest, k fastest)
+  int wpe;                // waves per SIMD the register budget allows
+  bool live_alpha;        // false: lapse frozen at 1 (synchronous gauge)
+  int (*launch)(const StageArgs&, hipStream_t);
+  int (*info)(int* vgprs, int* scratch, int* lds);
 };
 
-// Launch the stage kernel for (storage precision, stencil radius, halo).
-// prec: 0 = FP64, 1 = FP32 split.  Returns hipError_t.
-int launch_stage(int prec, int order, int halo, const KernelShape& shape, const StageArgs& a,
-                 hipStream_t stream);
-int stage_kernel_info(int prec, int order, int halo, const KernelShape& shape, int* vgprs,
-                      int* spills, int* lds_bytes);
+std::vector<StageVariant>& stage_variants();
+struct StageVariantReg {
+  explicit StageVariantReg(const StageVariant& v) { stage_variants().push_back(v); }
+};
 
 }  // namespace cg

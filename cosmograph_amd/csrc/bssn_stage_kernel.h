@@ -1,4 +1,5 @@
-// bssn_stage.hip -- fused BSSN RK4 stage kernel for gfx950.
+// bssn_stage_kernel.h -- fused BSSN RK4 stage kernel for gfx950 (template;
+// instantiated by the stage_inst_*.hip translation units).
 //
 // One launch = one RK4 stage over the whole periodic grid:
 //   * a 3D tile of points per workgroup; the tile plus a halo of H points is
@@ -19,6 +20,7 @@
 // TS is the storage type, TC the stencil/algebra type.  FP64 drop-in: both
 // double.  FP32 split: both float, with the RK4 accumulation in double and
 // FRW scalars passed in double.
+#pragma once
 #include <hip/hip_runtime.h>
 
 #include "bssn_stage.h"
@@ -64,14 +66,36 @@ __device__ __forceinline__ bool nan_as_float(T v) {
 // so FP32 storage would put its rounding noise straight into A_ij.
 __device__ __host__ constexpr bool metric_field(int f) { return f <= G33 || f == PHI; }
 
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK>
-__global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
+// order in which fields are staged through LDS:
+// PHI, G11..G33, [ALPHA], KK, GAM1..GAM3, A11..A33 (the A_ij only for KO)
+template <bool LA>
+__device__ __host__ constexpr int stencil_seq(int s) {
+  if (s == 0) return PHI;
+  if (s <= 6) return G11 + (s - 1);
+  if (LA) {
+    if (s == 7) return ALPHA;
+    s -= 1;
+  }
+  if (s == 7) return KK;
+  if (s <= 10) return GAM1 + (s - 8);
+  return A11 + (s - 11 < 5 ? s - 11 : 5);
+}
+
+// WPE: minimum waves per SIMD the register allocation must allow.
+// LIVE_ALPHA = false: synchronous gauge with DIFFalpha identically zero (Static
+// lapse, zero initial lapse perturbation).  Every lapse derivative is then an
+// exact zero, so the lapse stencil, D_iD_j alpha and the lapse registers are
+// skipped; the result is bitwise the same as differentiating zeros.
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LIVE_ALPHA>
+__global__ void __launch_bounds__(TI* TJ* TK) __attribute__((amdgpu_waves_per_eu(WPE)))
+stage_kernel(const StageArgs a) {
   constexpr int NT = TI * TJ * TK;
   constexpr int HJ = TJ + 2 * H, HK = TK + 2 * H, HI = TI + 2 * H;
   constexpr int HV = HI * HJ * HK;
   constexpr int SI = HJ * HK, SJ = HK, SK = 1;
   constexpr int NPF = (HV + NT - 1) / NT;
   __shared__ double ldsraw[2][HV];  // double-buffered field tile (FP32 fields use the first half)
+  __shared__ double ldn[HV];        // stage-1 unit-determinant factor t at every halo point
 
   const int ntiles = a.ntx * a.nty * a.ntz;
   const int b = blockIdx.x;
@@ -107,16 +131,18 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   auto INS = [&](int f) -> const TS* { return (const TS*)a.in[f]; };
   const bool norm = a.normalize != 0;
 
-  // stage-1 unit-determinant factor at this thread's halo elements
-  // (bssn.cc:75-99): gamma_ii -> (gamma_ii + t) r, gamma_ij -> gamma_ij r
-  double tq[NPF], rq[NPF];
+  // stage-1 unit-determinant factor at the halo points (bssn.cc:75-99):
+  // gamma_ii -> (gamma_ii + t)/(1 - t), gamma_ij -> gamma_ij/(1 - t).  Each
+  // thread writes and later reads only its own elements, so no barrier.
   if (norm) {
 #pragma unroll
     for (int q = 0; q < NPF; ++q) {
-      const unsigned o = goff[q];
-      tq[q] = norm_factor<double>(ldo(INM(G11), o), ldo(INM(G12), o), ldo(INM(G13), o), ldo(INM(G22), o),
-                                  ldo(INM(G23), o), ldo(INM(G33), o));
-      rq[q] = 1.0 / (1.0 - tq[q]);
+      const int e = tid + q * NT;
+      if (e < HV) {
+        const unsigned o = goff[q];
+        ldn[e] = norm_factor<double>(ldo(INM(G11), o), ldo(INM(G12), o), ldo(INM(G13), o), ldo(INM(G22), o),
+                                     ldo(INM(G23), o), ldo(INM(G33), o));
+      }
     }
   }
 
@@ -141,7 +167,10 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
         const int e = tid + q * NT;
         if (e < HV) {
           double v = pd[q];
-          if (norm && f <= G33) v = (v + (diag6(f) ? tq[q] : 0.0)) * rq[q];
+          if (norm && f <= G33) {
+            const double tn = ldn[e];
+            v = (v + (diag6(f) ? tn : 0.0)) / (1.0 - tn);
+          }
           dst[e] = v;
         }
       }
@@ -164,7 +193,7 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   gc[6] = phi_c;
 #pragma unroll
   for (int f = 0; f < NF; ++f)
-    if (!metric_field(f)) v[f] = (TC)ldo(INS(f), idx);
+    if (!metric_field(f)) v[f] = (f == ALPHA && !LIVE_ALPHA) ? TC(0) : (TC)ldo(INS(f), idx);
   if (norm) {
     // bssn.cc:94-117 at this point
     const double tn = norm_factor<double>(gc[0], gc[1], gc[2], gc[3], gc[4], gc[5]);
@@ -216,22 +245,26 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   constexpr int KO_ORDER = (H > R) ? 2 * R : 0;
 
   // stencil fields in order; A_ij only when KO is active
-  constexpr int NSEQ = 18;
-  constexpr int SEQ[NSEQ] = {PHI, G11, G12, G13, G22, G23, G33, ALPHA, KK, GAM1, GAM2, GAM3,
-                             A11, A12, A13, A22, A23, A33};
-  const int nseq = use_ko ? NSEQ : 12;
+  constexpr int NSEQ = LIVE_ALPHA ? 18 : 17;
+  const int nseq = use_ko ? NSEQ : NSEQ - 6;
+  if constexpr (!LIVE_ALPHA) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) dal[d] = TC(0);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) ddal[d] = TC(0);
+  }
 
-  fetch(SEQ[0]);
-  store(0, SEQ[0]);
+  fetch(PHI);
+  store(0, PHI);
   __syncthreads();
 
 #pragma unroll
   for (int s = 0; s < NSEQ; ++s) {
     if (s < nseq) {
       const bool more = (s + 1) < nseq;
-      const int fn = SEQ[(s + 1) < NSEQ ? s + 1 : 0];
+      const int fn = stencil_seq<LIVE_ALPHA>((s + 1) < NSEQ ? s + 1 : 0);
       if (more) fetch(fn);
-      const int f = SEQ[s];
+      const int f = stencil_seq<LIVE_ALPHA>(s);
       if (valid) {
         if (metric_field(f)) {
           const double* __restrict__ L = ldsraw[s & 1];
@@ -331,8 +364,10 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   for (int s = 0; s < 6; ++s) DDphi[s] = ddphi[s] - (G[0][s] * dphi[0] + G[1][s] * dphi[1] + G[2][s] * dphi[2]);
 
   // (D_i D_j alpha)^TF and its trace (bssn.cc:604-626)
-  TC DDa[6], DDaTR;
-  {
+  TC DDa[6], DDaTR = TC(0);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) DDa[q] = TC(0);
+  if constexpr (LIVE_ALPHA) {
     TC gild[3];
 #pragma unroll
     for (int I = 0; I < 3; ++I) gild[I] = GI(I, 0) * dphi[0] + GI(I, 1) * dphi[1] + GI(I, 2) * dphi[2];
@@ -469,6 +504,7 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   const int st = a.stage;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
+    if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
     const double cc = (double)rhs[f];
     if (metric_field(f)) {
       double* __restrict__ o = (double*)a.out[f];
@@ -502,20 +538,17 @@ __global__ void __launch_bounds__(TI* TJ* TK) stage_kernel(const StageArgs a) {
   }
 }
 
-// ------------------------------------------------------------ dispatch
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK>
-static int launch_t(const StageArgs& a, hipStream_t s) {
-  const int ntiles = a.ntx * a.nty * a.ntz;
-  const int grid = 8 * a.per_xcd;
-  (void)ntiles;
-  hipLaunchKernelGGL((stage_kernel<TS, TC, R, H, TI, TJ, TK>), dim3(grid), dim3(TI * TJ * TK), 0, s, a);
+// ------------------------------------------------------------ variants
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA>
+int stage_launch_t(const StageArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((stage_kernel<TS, TC, R, H, TI, TJ, TK, WPE, LA>), dim3(8 * a.per_xcd), dim3(TI * TJ * TK), 0, s, a);
   return (int)hipGetLastError();
 }
 
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK>
-static int info_t(int* vgprs, int* spills, int* lds) {
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA>
+int stage_info_t(int* vgprs, int* spills, int* lds) {
   hipFuncAttributes attr;
-  hipError_t e = hipFuncGetAttributes(&attr, (const void*)stage_kernel<TS, TC, R, H, TI, TJ, TK>);
+  hipError_t e = hipFuncGetAttributes(&attr, (const void*)stage_kernel<TS, TC, R, H, TI, TJ, TK, WPE, LA>);
   if (e != hipSuccess) return (int)e;
   *vgprs = attr.numRegs;
   *spills = (int)attr.localSizeBytes;
@@ -523,44 +556,11 @@ static int info_t(int* vgprs, int* spills, int* lds) {
   return 0;
 }
 
-#define CG_SHAPES(X, TS, TC, R, H) \
-  X(TS, TC, R, H, 4, 8, 8)         \
-  X(TS, TC, R, H, 2, 8, 16)        \
-  X(TS, TC, R, H, 4, 4, 16)
-
-#define CG_ORDERS(X, TS, TC)  \
-  CG_SHAPES(X, TS, TC, 1, 1)  \
-  CG_SHAPES(X, TS, TC, 1, 2)  \
-  CG_SHAPES(X, TS, TC, 2, 2)  \
-  CG_SHAPES(X, TS, TC, 3, 3)  \
-  CG_SHAPES(X, TS, TC, 4, 4)  \
-  X(TS, TC, 4, 5, 4, 8, 8)
-
-#define CG_CASE_LAUNCH(TS, TC, R, H, A, B, C)                                    \
-  if (order == 2 * R && halo == H && shape.ti == A && shape.tj == B && shape.tk == C) \
-    return launch_t<TS, TC, R, H, A, B, C>(a, stream);
-#define CG_CASE_INFO(TS, TC, R, H, A, B, C)                                      \
-  if (order == 2 * R && halo == H && shape.ti == A && shape.tj == B && shape.tk == C) \
-    return info_t<TS, TC, R, H, A, B, C>(vgprs, spills, lds_bytes);
-
-int launch_stage(int prec, int order, int halo, const KernelShape& shape, const StageArgs& a,
-                 hipStream_t stream) {
-  if (prec == 0) {
-    CG_ORDERS(CG_CASE_LAUNCH, double, double)
-  } else {
-    CG_ORDERS(CG_CASE_LAUNCH, float, float)
-  }
-  return (int)hipErrorInvalidConfiguration;
-}
-
-int stage_kernel_info(int prec, int order, int halo, const KernelShape& shape, int* vgprs, int* spills,
-                      int* lds_bytes) {
-  if (prec == 0) {
-    CG_ORDERS(CG_CASE_INFO, double, double)
-  } else {
-    CG_ORDERS(CG_CASE_INFO, float, float)
-  }
-  return (int)hipErrorInvalidConfiguration;
-}
+#define CG_REGISTER_STAGE(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA)                                            \
+  static StageVariantReg CG_CAT(reg_, __COUNTER__)(StageVariant{PREC, 2 * R, H, TI, TJ, TK, WPE, LA,          \
+                                                                &stage_launch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>, \
+                                                                &stage_info_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>});
+#define CG_CAT2(a, b) a##b
+#define CG_CAT(a, b) CG_CAT2(a, b)
 
 }  // namespace cg

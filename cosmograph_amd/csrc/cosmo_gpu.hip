@@ -168,7 +168,9 @@ struct cg_ctx {
   int prec = 0;
   size_t ts = 8;          // storage bytes
   int R = 2, H = 2;
-  KernelShape shape{4, 8, 8};
+  int req_ti = 0, req_tj = 0, req_tk = 0;  // requested tile (0 = automatic)
+  const StageVariant* var = nullptr;     // stage kernel in use
+  bool alpha_nonzero = false;             // a non-zero DIFFalpha was uploaded
   long npts = 0, fstride = 0, rstride = 0;
   void* regs = nullptr;   // [11][4][npts] A_ij, DIFFK, Gamma^i, DIFFalpha in the storage type
   double* met = nullptr;  // [7][4][npts] DIFFgamma11..33, DIFFphi registers, always FP64
@@ -215,6 +217,41 @@ static int halo_for(int order, double ko) {
   return R;
 }
 
+// the single conditional wrap in the halo loader needs n > tile + H - 2
+static bool variant_fits(const cg_ctx* c, const StageVariant& v) {
+  const int t[3] = {v.ti, v.tj, v.tk}, n[3] = {c->p.nx, c->p.ny, c->p.nz};
+  for (int d = 0; d < 3; ++d)
+    if (n[d] < v.halo || n[d] <= t[d] + v.halo - 2) return false;
+  return true;
+}
+
+static bool alpha_live(const cg_ctx* c) { return c->p.lapse != CG_LAPSE_STATIC || c->alpha_nonzero; }
+
+static bool variant_matches(const cg_ctx* c, const StageVariant& v) {
+  return v.prec == c->prec && v.order == c->p.stencil_order && v.halo == c->H && v.live_alpha == alpha_live(c) &&
+         variant_fits(c, v);
+}
+
+// default: the requested tile, else the 4x8x8 tile with the register budget
+// of 3 (FP32) / 2 (FP64) waves per SIMD when compiled, else any fitting tile
+static const StageVariant* pick_variant(const cg_ctx* c) {
+  const StageVariant* best = nullptr;
+  int best_score = -1;
+  for (const auto& v : stage_variants()) {
+    if (!variant_matches(c, v)) continue;
+    int score = 0;
+    if (c->req_ti) {
+      if (v.ti != c->req_ti || v.tj != c->req_tj || v.tk != c->req_tk) continue;
+      score += 100;
+    } else if (v.ti == 4 && v.tj == 8 && v.tk == 8) {
+      score += 100;
+    }
+    score += (v.wpe == (c->prec == CG_FP64 ? 2 : 3)) ? 10 : (v.wpe == 1 ? 5 : 0);
+    if (score > best_score) { best = &v; best_score = score; }
+  }
+  return best;
+}
+
 // The definitions below take C linkage from their declarations in cosmo_gpu.h.
 
 const char* cg_last_error(const cg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -236,15 +273,7 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   c->R = p.stencil_order / 2;
   c->H = halo_for(p.stencil_order, p.ko_coeff);
   c->lambda = p.lambda;
-  if (p.tile_i > 0) c->shape = KernelShape{p.tile_i, p.tile_j, p.tile_k};
-  // the single conditional wrap in the halo loader needs n > tile + H - 2
-  const int tdim[3] = {c->shape.ti, c->shape.tj, c->shape.tk}, ndim[3] = {p.nx, p.ny, p.nz};
-  for (int d = 0; d < 3; ++d)
-    if (ndim[d] < c->H || ndim[d] <= tdim[d] + c->H - 2) {
-      int rc = fail(c, CG_ERR_ARG, "grid too small for the stencil tile");
-      *out = c;
-      return rc;
-    }
+  if (p.tile_i > 0) { c->req_ti = p.tile_i; c->req_tj = p.tile_j; c->req_tk = p.tile_k; }
   c->npts = (long)p.nx * p.ny * p.nz;
   const long pad = ((c->npts + 63) / 64) * 64;  // 256/512-byte aligned registers
   c->rstride = pad;
@@ -310,6 +339,9 @@ int cg_field_upload(cg_ctx* c, const char* name, const double* host, size_t n) {
   void* d = resolve(c, name, true, &e, &esz);
   if (!d) return fail(c, e == 2 ? CG_ERR_STATE : CG_ERR_ARG,
                       e == 2 ? "_a/_f alias _p after a fused step; write _p" : std::string("unknown field ") + name);
+  if (!strncmp(name, "DIFFalpha_", 10))
+    for (size_t x = 0; x < n; ++x)
+      if (host[x] != 0.0) { c->alpha_nonzero = true; break; }
   if (esz == 8) {
     HIPCHK(c, hipMemcpyAsync(d, host, n * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -372,11 +404,7 @@ static StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int ro
   }
   a.DIFFD = c->DIFFD;
   a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
-  a.ntx = (c->p.nx + c->shape.ti - 1) / c->shape.ti;
-  a.nty = (c->p.ny + c->shape.tj - 1) / c->shape.tj;
-  a.ntz = (c->p.nz + c->shape.tk - 1) / c->shape.tk;
-  const int nt = a.ntx * a.nty * a.ntz;
-  a.per_xcd = (nt + 7) / 8;
+
   a.stage = stage;
   a.normalize = normalize;
   a.matter = c->p.matter;
@@ -403,14 +431,23 @@ static StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int ro
   return a;
 }
 
-static int launch(cg_ctx* c, const StageArgs& a) {
+static int launch(cg_ctx* c, StageArgs a) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     HIPCHK(c, hipEventRecord(e0, c->stream));
   }
-  int rc = launch_stage(c->prec, c->p.stencil_order, c->H, c->shape, a, c->stream);
+  const bool live = alpha_live(c);
+  if (!c->var || c->var->live_alpha != live) {
+    c->var = pick_variant(c);
+    if (!c->var) return fail(c, CG_ERR_ARG, "no stage kernel variant fits this grid/order");
+  }
+  a.ntx = (c->p.nx + c->var->ti - 1) / c->var->ti;
+  a.nty = (c->p.ny + c->var->tj - 1) / c->var->tj;
+  a.ntz = (c->p.nz + c->var->tk - 1) / c->var->tk;
+  a.per_xcd = (a.ntx * a.nty * a.ntz + 7) / 8;
+  int rc = c->var->launch(a, c->stream);
   if (rc != 0) return fail(c, CG_ERR_HIP, std::string("stage launch: ") + hipGetErrorString((hipError_t)rc));
   if (c->timing) {
     HIPCHK(c, hipEventRecord(e1, c->stream));
@@ -507,11 +544,63 @@ int cg_sync(cg_ctx* c) {
   return CG_OK;
 }
 
-int cg_kernel_info(const cg_ctx* c, int* vgprs, int* scratch, int* lds, int* ti, int* tj, int* tk) {
+int cg_kernel_info(const cg_ctx* cc, int* vgprs, int* scratch, int* lds, int* ti, int* tj, int* tk) {
+  cg_ctx* c = const_cast<cg_ctx*>(cc);
   if (!c) return CG_ERR_ARG;
-  int rc = stage_kernel_info(c->prec, c->p.stencil_order, c->H, c->shape, vgprs, scratch, lds);
-  *ti = c->shape.ti; *tj = c->shape.tj; *tk = c->shape.tk;
+  if (!c->var || c->var->live_alpha != alpha_live(c)) c->var = pick_variant(c);
+  if (!c->var) return fail(c, CG_ERR_ARG, "no stage kernel variant fits this grid/order");
+  int rc = c->var->info(vgprs, scratch, lds);
+  *ti = c->var->ti; *tj = c->var->tj; *tk = c->var->tk;
   return rc ? CG_ERR_HIP : CG_OK;
+}
+
+int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
+  if (!c || reps < 1) return CG_ERR_ARG;
+  if (!c->a_is_p) return fail(c, CG_ERR_STATE, "autotune between method-level stages would clobber _a/_c");
+  // Times real stage-2-shaped launches (stencils of _p, writes into the dead
+  // _f and spare registers), so the evolved state is untouched.
+  std::string rep = "[";
+  const StageVariant* best = nullptr;
+  double best_ms = 1e30;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  const bool timing = c->timing;
+  c->timing = false;
+  for (const auto& v : stage_variants()) {
+    if (!variant_matches(c, v)) continue;
+    c->var = &v;
+    StageArgs a = make_args(c, 2, 0, 0, 1, 3, 0);
+    int rc = launch(c, a);
+    if (rc) { c->timing = timing; return rc; }
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r)
+      if ((rc = launch(c, a))) { c->timing = timing; return rc; }
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    const double avg = ms / reps;
+    int vg = 0, sc = 0, ld = 0;
+    v.info(&vg, &sc, &ld);
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"ms\":%.4f,\"vgprs\":%d,\"scratch\":%d}",
+             rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, avg, vg, sc);
+    rep += buf;
+    if (avg < best_ms) { best_ms = avg; best = &v; }
+  }
+  rep += "]";
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->timing = timing;
+  c->var = best;
+  if (report && report_len) {
+    strncpy(report, rep.c_str(), report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  if (!best) return fail(c, CG_ERR_ARG, "no stage kernel variant fits this grid/order");
+  c->req_ti = best->ti; c->req_tj = best->tj; c->req_tk = best->tk;
+  return CG_OK;
 }
 
 int cg_set_timing(cg_ctx* c, int on) {

@@ -108,6 +108,7 @@ def load_library():
         "cg_sync": ([vp], C.c_int),
         "cg_kernel_info": ([vp, ip, ip, ip, ip, ip, ip], C.c_int),
         "cg_set_timing": ([vp, C.c_int], C.c_int),
+        "cg_autotune": ([vp, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
         "cg_stage_time_ms": ([vp, dp, ip], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -207,6 +208,13 @@ class BSSN:
         self._chk(self.L.cg_kernel_info(self.h, *[C.byref(x) for x in v]))
         vg, sc, lds, ti, tj, tk = (x.value for x in v)
         return {"vgprs": vg, "scratch_bytes": sc, "lds_bytes": lds, "tile": (ti, tj, tk)}
+
+    def autotune(self, reps: int = 3):
+        """Time every compiled stage-kernel variant for this grid, keep the fastest."""
+        import json
+        buf = C.create_string_buffer(16384)
+        self._chk(self.L.cg_autotune(self.h, reps, buf, len(buf)))
+        return json.loads(buf.value.decode() or "[]")
 
     def set_timing(self, on: bool):
         self._chk(self.L.cg_set_timing(self.h, int(on)))

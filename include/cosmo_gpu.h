@@ -95,6 +95,12 @@ int cg_sync(cg_ctx* ctx);
 int cg_kernel_info(const cg_ctx* ctx, int* vgprs, int* scratch_bytes, int* lds_bytes,
                    int* tile_i, int* tile_j, int* tile_k);
 
+/* Project autotuner: times every compiled stage-kernel variant (tile shape x
+ * register budget) that fits this grid on the context's own data and keeps
+ * the fastest.  The evolved state is untouched; must be called between whole
+ * steps.  `report` (optional) receives a JSON list of the timings. */
+int cg_autotune(cg_ctx* ctx, int reps, char* report, size_t report_len);
+
 /* Average duration of the last `n` stage-kernel launches recorded with HIP
  * events on the context's stream (ms); enable with cg_set_timing(ctx, 1). */
 int cg_set_timing(cg_ctx* ctx, int on);

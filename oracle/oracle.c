@@ -706,6 +706,67 @@ long or_num_nans(or_sim *s, const char *name) {
   return n;
 }
 
+/* reductions exported for the bitwise checks against utils/math.h:
+ * 0 average (1359-1371), 1 volume_average (1382-1393), 2 conformal_average
+ * (1403-1415), 3 min (1506-1518), 4 max (1491-1503), 5 numNaNs (1524-1541) */
+double or_reduction(or_sim *s, int kind, const double *f, const double *dphi, double phiF) {
+  const long n = s->npts;
+  double acc = 0.0;
+  switch (kind) {
+  case 0:
+    for (long x = 0; x < n; ++x) acc += f[x];
+    return acc / (double)n;
+  case 1:
+    for (long x = 0; x < n; ++x) acc += exp(6.0 * (dphi[x] + phiF));
+    return acc / (double)n;
+  case 2: return conformal_average(s, f, dphi, phiF);
+  case 3: {
+    double m = f[0];
+    for (long x = 0; x < n; ++x)
+      if (f[x] < m) m = f[x];
+    return m;
+  }
+  case 4: {
+    double m = f[0];
+    for (long x = 0; x < n; ++x)
+      if (f[x] > m) m = f[x];
+    return m;
+  }
+  default: {
+    long c = 0;
+    for (long x = 0; x < n; ++x) {
+      union { float v; uint32_t u; } u;
+      u.v = (float)f[x];
+      if ((u.u << 1) > 0xff000000u) c++;
+    }
+    return (double)c;
+  }
+  }
+}
+
+/* RK4Register arithmetic on a standalone register set, x' = c0 cos x + c1 x
+ * (the same ODE the reference driver integrates with utils/RK4Register.h);
+ * with c0 = c1 = 0 it integrates x' = exp(-x), the ODE of tests/rk4.cc. */
+void or_rk4_demo(const double *x0, long n, double h, double c0, double c1, int nsteps, double *traj,
+                 double *last_stage_a) {
+  double *r[4];
+  for (int q = 0; q < 4; ++q) r[q] = zalloc(n);
+  for (long x = 0; x < n; ++x) r[OR_P][x] = x0[x];
+  for (int st = 0; st < nsteps; ++st) {
+    reg_step_init(r, n);
+    for (int k = 1; k <= 4; ++k) {
+      for (long x = 0; x < n; ++x) {
+        double a = r[OR_A][x];
+        if (st == nsteps - 1) last_stage_a[(long)(k - 1) * n + x] = a;
+        r[OR_C][x] = (c0 == 0.0 && c1 == 0.0) ? exp(-a) : cos(a) * c0 + c1 * a;
+      }
+      reg_finalize(r, n, k, h);
+    }
+    for (long x = 0; x < n; ++x) traj[(long)st * n + x] = r[OR_P][x];
+  }
+  for (int q = 0; q < 4; ++q) free(r[q]);
+}
+
 /* ---------------------------------------------------- BSSN methods --- */
 /* set_DIFFgamma_Aij_norm, bssn.cc:61-119 (on _a only) */
 void or_bssn_normalize(or_sim *s) {

@@ -120,6 +120,11 @@ void or_dust_add_src(or_sim *s);
  * sims/dust.cc:53-161) without output; returns 10 on a NaN (sims/sim.cc:179). */
 int or_run_steps(or_sim *s, int nsteps);
 
+/* reductions (utils/math.h) and RK4Register arithmetic, for bitwise checks */
+double or_reduction(or_sim *s, int kind, const double *f, const double *dphi, double phiF);
+void or_rk4_demo(const double *x0, long n, double h, double c0, double c1, int nsteps, double *traj,
+                 double *last_stage_a);
+
 /* point algebra on caller-filled inputs (for brute-force identity tests) */
 void or_point_compute(const or_params *p, or_point *bd);
 void or_point_momentum(const or_point *bd, double M[3]);

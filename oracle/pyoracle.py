@@ -98,6 +98,9 @@ def lib():
         L.or_point_momentum.argtypes = [C.POINTER(Point), dp]
         L.or_stencil.argtypes = [vp, dp, C.c_int, C.c_int, C.c_int, C.c_long, C.c_long, C.c_long]
         L.or_stencil.restype = C.c_double
+        L.or_reduction.argtypes = [vp, C.c_int, dp, dp, C.c_double]
+        L.or_reduction.restype = C.c_double
+        L.or_rk4_demo.argtypes = [dp, C.c_long, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]
         _lib = L
     return _lib
 

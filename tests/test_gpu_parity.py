@@ -123,3 +123,32 @@ def test_fp32_split_random_state(oracle_mod, gpu):
         ref = o.field(f + "_p")
         err = rel_l2(g.download(f + "_p"), ref)
         assert err <= 1e-5, (f, err)
+
+
+def test_autotuned_variant_parity(oracle_mod, gpu):
+    """Whatever variant the autotuner picks must give the same evolution."""
+    shape = (32, 32, 32)
+    _say("case autotune")
+    st = smooth_random_state(shape, seed=11, with_alpha=False)
+    o, g = setup_pair(oracle_mod, gpu, {"n": shape, "order": 4}, state=st)
+    report = g.autotune(2)
+    assert len(report) >= 1
+    o.run_steps(3)
+    g.step(3)
+    for f in FIELDS:
+        assert rel_l2(g.download(f + "_p"), o.field(f + "_p")) <= 1e-12, f
+
+
+@pytest.mark.parametrize("tile,precision,tol", [((2, 8, 16), "fp64", 1e-12), ((4, 4, 16), "fp64", 1e-12),
+                                                ((8, 8, 8), "fp32_split", 1e-5)])
+def test_tile_variants_parity(oracle_mod, gpu, tile, precision, tol):
+    shape = (24, 24, 40)
+    _say(f"case tile {tile} {precision}")
+    st = smooth_random_state(shape, seed=12)
+    o, g = setup_pair(oracle_mod, gpu, {"n": shape, "order": 6, "precision": precision}, state=st,
+                      gpu_kw={"tile": tile})
+    assert g.kernel_info()["tile"] == tile
+    o.run_steps(2)
+    g.step(2)
+    for f in FIELDS:
+        assert rel_l2(g.download(f + "_p"), o.field(f + "_p")) <= tol, f
