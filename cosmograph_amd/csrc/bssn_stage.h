@@ -44,7 +44,16 @@ struct StageArgs {
   double a_adj, k_damp;
   double H_frw;           // 2/3 K_F^2 - 16 pi rho_F (for the split-precision H)
   unsigned int* nan_count;
+  double* cons;           // constraint mode: 17 per-point FP64 arrays (see CONS_*)
+  long cons_stride;
 };
+
+// per-point outputs of the constraint mode (BSSN::setConstraintCalcs,
+// bssn.cc:965-1071): value, scale and value/scale of H, M, G, A, S, then the
+// ricci_a and AijAij_a arrays of prepBSSNOutput (sims/sim.cc:214-231)
+enum { CONS_H = 0, CONS_M = 3, CONS_G = 6, CONS_A = 9, CONS_S = 12, CONS_RICCI = 15, CONS_AIJAIJ = 16,
+       CONS_N = 17 };
+int launch_constraints(int prec, int order, bool live_alpha, const StageArgs& a, hipStream_t s);
 
 // One compiled instantiation of the stage kernel.
 struct StageVariant {

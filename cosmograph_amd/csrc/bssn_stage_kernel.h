@@ -86,7 +86,10 @@ __device__ __host__ constexpr int stencil_seq(int s) {
 // lapse, zero initial lapse perturbation).  Every lapse derivative is then an
 // exact zero, so the lapse stencil, D_iD_j alpha and the lapse registers are
 // skipped; the result is bitwise the same as differentiating zeros.
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LIVE_ALPHA>
+// MODE 0: RK4 stage.  MODE 1: constraint diagnostics (TC = double), which
+// also differentiates A_ij for the momentum constraint and writes per-point
+// values instead of the RK4 epilogue.
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LIVE_ALPHA, int MODE = 0>
 __global__ void __launch_bounds__(TI* TJ* TK) __attribute__((amdgpu_waves_per_eu(WPE)))
 stage_kernel(const StageArgs a) {
   constexpr int NT = TI * TJ * TK;
@@ -238,6 +241,7 @@ stage_kernel(const StageArgs a) {
   TC dg[3][6];      // d_k gamma_ij
   TC T1[6];         // gamma^{kl} d_k d_l gamma_ij
   TC dphi[3], ddphi[6], dal[3], ddal[6], dK[3], dG[3][3];
+  TC dA[3][6];      // d_j A_kl, constraint mode only
   TC ko[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) ko[f] = TC(0);
@@ -246,7 +250,7 @@ stage_kernel(const StageArgs a) {
 
   // stencil fields in order; A_ij only when KO is active
   constexpr int NSEQ = LIVE_ALPHA ? 18 : 17;
-  const int nseq = use_ko ? NSEQ : NSEQ - 6;
+  const int nseq = (use_ko || MODE == 1) ? NSEQ : NSEQ - 6;
   if constexpr (!LIVE_ALPHA) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) dal[d] = TC(0);
@@ -297,6 +301,9 @@ stage_kernel(const StageArgs a) {
             const int kk = f - GAM1;
             dG[0][kk] = fd1<R>(L, c, SI) * idx1; dG[1][kk] = fd1<R>(L, c, SJ) * idx1;
             dG[2][kk] = fd1<R>(L, c, SK) * idx1;
+          } else if (MODE == 1 && f >= A11 && f <= A33) {
+            const int q = f - A11;
+            dA[0][q] = fd1<R>(L, c, SI) * idx1; dA[1][q] = fd1<R>(L, c, SJ) * idx1; dA[2][q] = fd1<R>(L, c, SK) * idx1;
           }
         }
       }
@@ -443,7 +450,7 @@ stage_kernel(const StageArgs a) {
 
   // Hamiltonian constraint (bssn.cc:1073-1084), only when a damping term uses it
   TC Hc = TC(0);
-  if (a.use_H) {
+  if (a.use_H || MODE == 1) {
     const TC rr = DIFFr + rhoF;
     if constexpr (sizeof(TC) == 8) {
       Hc = -exp(TC(5) * phi) / TC(8) * (ricci + TC(2.0 / 3.0) * K * K - AijAij - TC(16) * PI * rr);
@@ -452,6 +459,78 @@ stage_kernel(const StageArgs a) {
       const TC dK2 = TC(2.0 / 3.0) * (TC(2) * KF + v[KK]) * v[KK];
       Hc = -exp(TC(5) * phi) / TC(8) * (ricci + ((TC)a.H_frw + dK2) - AijAij - TC(16) * PI * DIFFr);
     }
+  }
+
+  if constexpr (MODE == 1) {
+    // setConstraintCalcs (bssn.cc:965-1071) per-point quantities, FP64
+    double* __restrict__ co = a.cons;
+    const long cs = a.cons_stride;
+    auto put = [&](int q, double val) { co[(long)q * cs + idx] = val; };
+    const double rr = (double)DIFFr + a.rho_F;
+    const double e5 = exp(5.0 * (double)phi), e6 = exp(6.0 * (double)phi);
+    // hamiltonianConstraintScale (bssn.cc:1086-1100)
+    const double Hs = (e5 / 8.0) * sqrt((double)ricci * ricci + (double)AijAij * AijAij +
+                                        pow(2.0 / 3.0 * (double)K * K, 2) + pow(16.0 * 3.14159265358979323846 * rr, 2));
+    put(CONS_H, Hc); put(CONS_H + 1, Hs); put(CONS_H + 2, Hc / Hs);
+    // BSSN_MI / BSSN_MI_SCALE (bssn_macros.h:487-527), S_i = 0 for static dust
+    double M2 = 0, Ms2 = 0;
+#pragma unroll
+    for (int I = 0; I < 3; ++I) {
+      double t6 = 0, td = 0;
+#pragma unroll
+      for (int n = 0; n < 3; ++n)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          t6 += GI(m, n) * AM(m, I) * dphi[n];
+          td += GI(m, n) * dA[n][sy(m, I)];
+        }
+#pragma unroll
+      for (int m = 0; m < 3; ++m) td -= Gd[m] * AM(m, I);
+#pragma unroll
+      for (int n = 0; n < 3; ++n)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) td -= GL[m][sy(n, I)] * Ac[sy(m, n)];
+      const double MI = e6 * (-2.0 / 3.0 * dK[I] + 6.0 * t6 + td);
+      const double MS = e6 * (fabs(2.0 / 3.0 * dK[I]) + 6.0 * fabs(t6) + fabs(td));
+      M2 += MI * MI;
+      Ms2 += MS * MS;
+    }
+    const double Mv = sqrt(M2), Msc = sqrt(Ms2);
+    put(CONS_M, Mv); put(CONS_M + 1, Msc); put(CONS_M + 2, Mv / Msc);
+    // BSSN_GI_CALC / BSSN_GI_SCALE (bssn_macros.h:530-536)
+    double G2 = 0, Gs2 = 0;
+#pragma unroll
+    for (int I = 0; I < 3; ++I) {
+      const double tr = gi[0] * G[I][0] + gi[3] * G[I][3] + gi[5] * G[I][5] +
+                        2.0 * (gi[1] * G[I][1] + gi[2] * G[I][2] + gi[4] * G[I][4]);
+      const double gc_ = v[GAM1 + I] - tr;
+      const double gs_ = fabs((double)v[GAM1 + I]) +
+                         fabs(gi[0] * G[I][0] - gi[3] * G[I][3] - gi[5] * G[I][5] -
+                              2.0 * (gi[1] * G[I][1] + gi[2] * G[I][2] + gi[4] * G[I][4]));
+      G2 += gc_ * gc_;
+      Gs2 += gs_ * gs_;
+    }
+    put(CONS_G, sqrt(G2)); put(CONS_G + 1, sqrt(Gs2)); put(CONS_G + 2, sqrt(G2) / sqrt(Gs2));
+    // AijTFConstraintCalc / Scale (bssn.cc:1182-1192)
+    const double At = gi[0] * v[A11] + gi[3] * v[A22] + gi[5] * v[A33] +
+                      2.0 * (gi[1] * v[A12] + gi[2] * v[A13] + gi[4] * v[A23]);
+    const double As = fabs(gi[0] * v[A11]) + fabs(gi[3] * v[A22]) + fabs(gi[5] * v[A33]) +
+                      2.0 * (fabs(gi[1] * v[A12]) + fabs(gi[2] * v[A13]) + fabs(gi[4] * v[A23]));
+    put(CONS_A, At); put(CONS_A + 1, As); put(CONS_A + 2, At / As);
+    // unitDetConstraintCalc / Scale (bssn.cc:1199-1219)
+    {
+      const double g11 = v[G11], g12 = v[G12], g13 = v[G13], g22 = v[G22], g23 = v[G23], g33 = v[G33];
+      const double Sv = fabs(g11 + g22 + g33 - g12 * g12 - g13 * g13 - g23 * g23 + g11 * g22 + g11 * g33 + g22 * g33 -
+                             g23 * g23 * g11 - g13 * g13 * g22 - g12 * g12 * g33 + 2.0 * g12 * g13 * g23 +
+                             g11 * g22 * g33);
+      const double Ss = fabs(g11) + fabs(g22) + fabs(g33) + g12 * g12 + g13 * g13 + g23 * g23 + fabs(g11 * g22) +
+                        fabs(g11 * g33) + fabs(g22 * g33) + g23 * g23 * g11 + g13 * g13 * g22 + g12 * g12 * g33 +
+                        2.0 * fabs(g12 * g13 * g23) + fabs(g11 * g22 * g33);
+      put(CONS_S, Sv); put(CONS_S + 1, Ss); put(CONS_S + 2, Sv / Ss);
+    }
+    put(CONS_RICCI, ricci);
+    put(CONS_AIJAIJ, AijAij);
+    return;
   }
 
   // ------------------------------------------------ right-hand sides
@@ -554,6 +633,12 @@ int stage_info_t(int* vgprs, int* spills, int* lds) {
   *spills = (int)attr.localSizeBytes;
   *lds = (int)attr.sharedSizeBytes;
   return 0;
+}
+
+template <typename TS, int R, bool LA>
+int cons_launch_t(const StageArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((stage_kernel<TS, double, R, R, 4, 8, 8, 1, LA, 1>), dim3(8 * a.per_xcd), dim3(256), 0, s, a);
+  return (int)hipGetLastError();
 }
 
 #define CG_REGISTER_STAGE(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA)                                            \

@@ -160,6 +160,87 @@ __global__ void k_finalize(RegTable t, int rp, int rc, int rf, long n, int stage
   }
 }
 
+// ---------------------------------------------- constraint statistics --
+// Deterministic FP64 reductions of the per-point constraint values: each
+// block reduces a fixed chunk, a single block then combines the partials in
+// block order.  Max uses `>` so that NaN values never win, like the
+// reference's `if(fabs(C_val) > max_C)` (bssn_macros.h:581-585).
+constexpr int kRedThreads = 256;
+constexpr int kStats = 6;  // sum val, sum scale, sum scaled, sum val^2, max|val|, max|scaled|
+
+__device__ __forceinline__ double red_max(double a, double b) { return b > a ? b : a; }
+
+__global__ void k_cons_partial(const double* __restrict__ cons, long stride, long n, double* __restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  const long per = (n + gridDim.x - 1) / gridDim.x;
+  const long lo = (long)blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+  for (int c = 0; c < 5; ++c) {
+    const double* val = cons + (long)(3 * c) * stride;
+    const double* sca = val + stride;
+    const double* scd = val + 2 * stride;
+    double acc[kStats] = {0, 0, 0, 0, 0, 0};
+    for (long x = lo + threadIdx.x; x < hi; x += kRedThreads) {
+      const double v = val[x], sc = sca[x], sd = scd[x];
+      acc[0] += v; acc[1] += sc; acc[2] += sd; acc[3] += v * v;
+      acc[4] = red_max(acc[4], fabs(v)); acc[5] = red_max(acc[5], fabs(sd));
+    }
+    for (int q = 0; q < kStats; ++q) {
+      sh[threadIdx.x] = acc[q];
+      __syncthreads();
+      for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+          sh[threadIdx.x] = q < 4 ? sh[threadIdx.x] + sh[threadIdx.x + w] : red_max(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) part[((long)blockIdx.x * 5 + c) * kStats + q] = sh[0];
+      __syncthreads();
+    }
+  }
+}
+
+// second pass: sum (val - mean)^2 and (scaled - mean_scaled)^2
+__global__ void k_cons_var(const double* __restrict__ cons, long stride, long n, const double* __restrict__ means,
+                           double* __restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  const long per = (n + gridDim.x - 1) / gridDim.x;
+  const long lo = (long)blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+  for (int c = 0; c < 5; ++c) {
+    const double* val = cons + (long)(3 * c) * stride;
+    const double* scd = val + 2 * stride;
+    const double m0 = means[2 * c], m1 = means[2 * c + 1];
+    double a0 = 0, a1 = 0;
+    for (long x = lo + threadIdx.x; x < hi; x += kRedThreads) {
+      const double d0 = val[x] - m0, d1 = scd[x] - m1;
+      a0 += d0 * d0;
+      a1 += d1 * d1;
+    }
+    for (int q = 0; q < 2; ++q) {
+      sh[threadIdx.x] = q ? a1 : a0;
+      __syncthreads();
+      for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) part[((long)blockIdx.x * 5 + c) * 2 + q] = sh[0];
+      __syncthreads();
+    }
+  }
+}
+
+// one block: combine nb partial rows of width w in block order; columns whose
+// index mod kStats is set in maxmask take the max, the others the sum
+__global__ void k_combine(const double* __restrict__ part, int nb, int w, unsigned maxmask, double* __restrict__ out) {
+  for (int col = threadIdx.x; col < w; col += blockDim.x) {
+    const bool ismax = (maxmask >> (col % kStats)) & 1u;
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) {
+      const double v = part[(long)b * w + col];
+      acc = ismax ? red_max(acc, v) : acc + v;
+    }
+    out[col] = acc;
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ context --
@@ -171,6 +252,9 @@ struct cg_ctx {
   int req_ti = 0, req_tj = 0, req_tk = 0;  // requested tile (0 = automatic)
   const StageVariant* var = nullptr;     // stage kernel in use
   bool alpha_nonzero = false;             // a non-zero DIFFalpha was uploaded
+  double* cons = nullptr;                 // [CONS_N][pad] per-point constraint values (lazy)
+  double* cons_part = nullptr;            // reduction partials
+  bool cons_valid = false;                // ricci_a / AijAij_a filled
   long npts = 0, fstride = 0, rstride = 0;
   void* regs = nullptr;   // [11][4][npts] A_ij, DIFFK, Gamma^i, DIFFalpha in the storage type
   double* met = nullptr;  // [7][4][npts] DIFFgamma11..33, DIFFphi registers, always FP64
@@ -300,6 +384,8 @@ int cg_destroy(cg_ctx* c) {
   if (c->regs) (void)hipFree(c->regs);
   if (c->DIFFD) (void)hipFree(c->DIFFD);
   if (c->met) (void)hipFree(c->met);
+  if (c->cons) (void)hipFree(c->cons);
+  if (c->cons_part) (void)hipFree(c->cons_part);
   if (c->nan_dev) (void)hipFree(c->nan_dev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -312,6 +398,11 @@ static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err, size
   *esz = c->ts;
   std::string s(name ? name : "");
   if (s == "DIFFD_a") return c->DIFFD;
+  if (s == "ricci_a" || s == "AijAij_a") {
+    if (for_write || !c->cons_valid) { *err = 3; return nullptr; }
+    *esz = 8;
+    return c->cons + (long)(s == "ricci_a" ? CONS_RICCI : CONS_AIJAIJ) * c->rstride;
+  }
   if (s.size() < 3 || s[s.size() - 2] != '_') { *err = 1; return nullptr; }
   const std::string base = s.substr(0, s.size() - 2);
   const char r = s.back();
@@ -360,7 +451,8 @@ int cg_field_download(cg_ctx* c, const char* name, double* host, size_t n) {
   int e;
   size_t esz;
   void* d = resolve(c, name, false, &e, &esz);
-  if (!d) return fail(c, CG_ERR_ARG, std::string("unknown field ") + name);
+  if (!d) return fail(c, e == 3 ? CG_ERR_STATE : CG_ERR_ARG,
+                      e == 3 ? std::string(name) + " is filled by cg_constraints" : std::string("unknown field ") + name);
   if (esz == 8) {
     HIPCHK(c, hipMemcpyAsync(host, d, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -552,6 +644,82 @@ int cg_kernel_info(const cg_ctx* cc, int* vgprs, int* scratch, int* lds, int* ti
   int rc = c->var->info(vgprs, scratch, lds);
   *ti = c->var->ti; *tj = c->var->tj; *tk = c->var->tk;
   return rc ? CG_ERR_HIP : CG_OK;
+}
+
+int cg_constraints(cg_ctx* c, double H[8], double M[8], double G[7], double A[7], double S[7]) {
+  if (!c) return CG_ERR_ARG;
+  const long pad = c->rstride;
+  const int nb = 1024;
+  if (!c->cons) {
+    HIPCHK(c, hipMalloc((void**)&c->cons, (size_t)CONS_N * pad * sizeof(double)));
+    HIPCHK(c, hipMalloc((void**)&c->cons_part, (size_t)nb * 5 * kStats * sizeof(double) + 64 * sizeof(double)));
+  }
+  // evaluated on _a as the reference's output step sees it: after stepInit,
+  // i.e. the normalised _p when the last call was a fused step.  The momentum
+  // constraint differentiates A_ij, so the normalised register is
+  // materialised (into the spare stage register and the dead _f) rather than
+  // normalised on the fly.
+  int rin = c->sA;
+  if (c->a_is_p) {
+    rin = (c->sC == 3) ? 2 : 3;  // the spare of the two stage registers
+    const int bs = 256;
+    const long nbk = (c->npts + bs - 1) / bs;
+    if (c->prec == CG_FP64)
+      hipLaunchKernelGGL(k_step_init<double>, dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin, c->sF,
+                         c->npts, c->p.normalize_metric);
+    else
+      hipLaunchKernelGGL(k_step_init<float>, dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin, c->sF,
+                         c->npts, c->p.normalize_metric);
+    HIPCHK(c, hipGetLastError());
+  }
+  StageArgs a = make_args(c, 0, rin, c->sP, c->sF, c->sC, 0);
+  a.cons = c->cons;
+  a.cons_stride = pad;
+  a.ntx = (c->p.nx + 3) / 4;
+  a.nty = (c->p.ny + 7) / 8;
+  a.ntz = (c->p.nz + 7) / 8;
+  a.per_xcd = (a.ntx * a.nty * a.ntz + 7) / 8;
+  if (c->p.nx <= 4 + c->R - 2 || c->p.ny <= 8 + c->R - 2 || c->p.nz <= 8 + c->R - 2)
+    return fail(c, CG_ERR_ARG, "grid too small for the constraint kernel tile");
+  int rc = launch_constraints(c->prec, c->p.stencil_order, alpha_live(c), a, c->stream);
+  if (rc) return fail(c, CG_ERR_HIP, std::string("constraint launch: ") + hipGetErrorString((hipError_t)rc));
+  c->cons_valid = true;
+  double* part = c->cons_part;
+  double* fin = part + (size_t)nb * 5 * kStats;  // 30 + 10 + 10 doubles
+  hipLaunchKernelGGL(k_cons_partial, dim3(nb), dim3(kRedThreads), 0, c->stream, c->cons, pad, c->npts, part);
+  hipLaunchKernelGGL(k_combine, dim3(1), dim3(64), 0, c->stream, part, nb, 5 * kStats, 0x30u, fin);
+  HIPCHK(c, hipGetLastError());
+  double tot[5 * kStats];
+  HIPCHK(c, hipMemcpyAsync(tot, fin, sizeof tot, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double N = (double)c->npts;
+  double means[10];
+  for (int q = 0; q < 5; ++q) {
+    means[2 * q] = tot[q * kStats + 0] / N;      // BSSN_NORMALIZE_MEAN
+    means[2 * q + 1] = tot[q * kStats + 2] / N;
+  }
+  HIPCHK(c, hipMemcpyAsync(fin + 30, means, sizeof means, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_cons_var, dim3(nb), dim3(kRedThreads), 0, c->stream, c->cons, pad, c->npts, fin + 30, part);
+  hipLaunchKernelGGL(k_combine, dim3(1), dim3(64), 0, c->stream, part, nb, 10, 0u, fin + 40);
+  HIPCHK(c, hipGetLastError());
+  double var[10];
+  HIPCHK(c, hipMemcpyAsync(var, fin + 40, sizeof var, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double* outs[5] = {H, M, G, A, S};
+  const double dx3 = c->p.dx * c->p.dx * c->p.dx;
+  for (int q = 0; q < 5; ++q) {
+    double* o = outs[q];
+    if (!o) continue;
+    o[0] = means[2 * q];
+    o[1] = sqrt(var[2 * q] / (N - 1.0));           // BSSN_NORMALIZE_STDEV
+    o[2] = tot[q * kStats + 4];
+    o[3] = tot[q * kStats + 1];                    // mean_scale is left as a sum
+    o[4] = means[2 * q + 1];
+    o[5] = sqrt(var[2 * q + 1] / (N - 1.0));
+    o[6] = tot[q * kStats + 5];
+    if (q < 2) o[7] = sqrt(tot[q * kStats + 3] * dx3);  // H_L2, M_L2 (bssn.cc:989, 994, 1067)
+  }
+  return CG_OK;
 }
 
 int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {

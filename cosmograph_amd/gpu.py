@@ -109,6 +109,7 @@ def load_library():
         "cg_kernel_info": ([vp, ip, ip, ip, ip, ip, ip], C.c_int),
         "cg_set_timing": ([vp, C.c_int], C.c_int),
         "cg_autotune": ([vp, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
+        "cg_constraints": ([vp, dp, dp, dp, dp, dp], C.c_int),
         "cg_stage_time_ms": ([vp, dp, ip], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -198,6 +199,13 @@ class BSSN:
     def step(self, nsteps: int = 1):
         """nsteps whole RK4 steps (stepInit + 4 fused stages each)."""
         self._chk(self.L.cg_rk4_steps(self.h, nsteps))
+
+    def setConstraintCalcs(self):
+        """BSSN::setConstraintCalcs: dict of the H, M, G, A, S statistic arrays."""
+        H, M = (C.c_double * 8)(), (C.c_double * 8)()
+        G, A, S = (C.c_double * 7)(), (C.c_double * 7)(), (C.c_double * 7)()
+        self._chk(self.L.cg_constraints(self.h, H, M, G, A, S))
+        return {"H": list(H), "M": list(M), "G": list(G), "A": list(A), "S": list(S)}
 
     def sync(self):
         self._chk(self.L.cg_sync(self.h))

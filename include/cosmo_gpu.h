@@ -88,6 +88,15 @@ int cg_k_finalize(cg_ctx* ctx, int n);
  * turned NaN (the guard of sims/sim.cc:179-183). */
 int cg_rk4_steps(cg_ctx* ctx, int nsteps);
 
+/* BSSN::setConstraintCalcs (bssn.cc:965-1071) on the _a register as the
+ * reference's output step sees it: mean, stdev, max, sum of scales, mean
+ * scaled, stdev scaled, max scaled of the Hamiltonian (H), momentum (M),
+ * Christoffel (G), trace-free A (A) and unit-determinant (S) constraints;
+ * H[7], M[7] = sqrt(sum C^2 dx^3).  FP64 arithmetic and deterministic FP64
+ * reductions in every precision.  Also fills "ricci_a" and "AijAij_a" for
+ * cg_field_download (prepBSSNOutput, sims/sim.cc:214-231). */
+int cg_constraints(cg_ctx* ctx, double H[8], double M[8], double G[7], double A[7], double S[7]);
+
 /* Synchronise the context's stream. */
 int cg_sync(cg_ctx* ctx);
 

@@ -152,3 +152,67 @@ def test_tile_variants_parity(oracle_mod, gpu, tile, precision, tol):
     g.step(2)
     for f in FIELDS:
         assert rel_l2(g.download(f + "_p"), o.field(f + "_p")) <= tol, f
+
+
+def _oracle_output_step(oracle_mod, o):
+    """The reference's output step sees _a after initStaticStep (stepInit +
+    sources, sims/static.cc:74-82)."""
+    L = oracle_mod.lib()
+    o.stepInit()
+    o.clearSrc()
+    L.or_static_add_src(o.h)
+    L.or_lambda_add_src(o.h)
+    return o.constraints()
+
+
+def _check_stats(got, want, rel):
+    for c in "HMGAS":
+        g, w = np.array(got[c]), np.array(want[c])
+        if abs(w[2]) < 1e-14:  # constraint satisfied to roundoff: compare magnitudes only
+            assert abs(g[2]) < 1e-14, (c, g[2])
+            continue
+        scale = max(abs(w[2]), 1e-300)
+        # mean / stdev / max of values, sum of scales, scaled statistics, L2
+        assert abs(g[0] - w[0]) <= rel * scale, (c, "mean", g[0], w[0])
+        for q in (1, 2, 3, 5, 6) + ((7,) if c in "HM" else ()):
+            assert abs(g[q] - w[q]) <= rel * max(abs(w[q]), 1e-300), (c, q, g[q], w[q])
+        if np.isfinite(w[4]):
+            assert abs(g[4] - w[4]) <= rel * max(abs(w[6]), 1e-300), (c, "mean_scaled", g[4], w[4])
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_constraints_vs_oracle(oracle_mod, gpu, order):
+    shape = (20, 18, 24)
+    _say(f"case constraints order={order}")
+    st = smooth_random_state(shape, seed=21 + order)
+    o, g = setup_pair(oracle_mod, gpu, {"n": shape, "order": order}, state=st)
+    want = _oracle_output_step(oracle_mod, o)
+    got = g.setConstraintCalcs()
+    _check_stats(got, want, 1e-10)
+    np.testing.assert_allclose(g.download("ricci_a"), o.field("ricci_a"), rtol=1e-11, atol=1e-11 * np.abs(o.field("ricci_a")).max())
+
+
+def test_constraints_after_steps_config1(oracle_mod, gpu):
+    from cosmograph_amd.ics import static_dust_ics
+    _say("case constraints config1 after 10 steps")
+    ics = static_dust_ics(32, -2.0, 1e-3, 9)
+    o, g = setup_pair(oracle_mod, gpu, {"n": 32, "order": 4}, ics=ics)
+    o.run_steps(10)
+    g.step(10)
+    _check_stats(g.setConstraintCalcs(), _oracle_output_step(oracle_mod, o), 1e-10)
+
+
+def test_constraints_fp32_split(oracle_mod, gpu):
+    shape = (24, 24, 24)
+    _say("case constraints fp32")
+    st = smooth_random_state(shape, seed=31, with_alpha=False)
+    o, g = setup_pair(oracle_mod, gpu, {"n": shape, "order": 4, "precision": "fp32_split"}, state=st)
+    want = _oracle_output_step(oracle_mod, o)
+    got = g.setConstraintCalcs()
+    for c in "HMG":
+        for q in (2, 6) + ((7,) if c in "HM" else ()):
+            assert abs(got[c][q] - want[c][q]) <= 1e-5 * abs(want[c][q]), (c, q)
+    # algebraic constraints hold to the storage precision: tr A_ij of FP32
+    # A_ij (|A| ~ 1e-3), det(gamma) - 1 of the FP64 metric
+    assert got["A"][2] < 1e-8
+    assert got["S"][2] < 1e-12
