@@ -177,7 +177,7 @@ def main():
     bytes_pt_upd = words * 4                       # FP32 storage
     bytes_per_launch = bytes_pt_upd / 4.0 * pts    # average stage
     achieved = bytes_per_launch / (k32_ms * 1e-3) / 1e9
-    traffic = load_traffic(f"c3_{n}_o{order}_fp32")
+    traffic = load_traffic(f"bssn_static_dust_{n}_o{order}_fp32")
 
     cpu = None
     if args.cpu_sample > 0:
