@@ -38,7 +38,9 @@ struct StageArgs {
   double inv_dx, inv_dx2, dt;
   double cf, ca;          // epilogue: f += cf*c ; a' = p + ca*c  (stage 4: p' = f + (cf*c + p))
   double phi_F, K_F, rho_F, S_F;  // FRW values seen by this stage
-  double K_avg;
+  const double* ext_in;   // [K_min, K_avg, avg_vol, rho_avg] seen by this stage (device)
+  double* ext_part;       // per-wave partial sums of setExtraFieldData over the output (null: off)
+  double phi_F_next;      // FRW phi after this stage's FRW step (weights of the averages)
   double lambda;
   double ko_pref;         // KO prefactor (sign included), per unit stencil sum
   double a_adj, k_damp;
@@ -54,6 +56,48 @@ struct StageArgs {
 enum { CONS_H = 0, CONS_M = 3, CONS_G = 6, CONS_A = 9, CONS_S = 12, CONS_RICCI = 15, CONS_AIJAIJ = 16,
        CONS_N = 17 };
 int launch_constraints(int prec, int order, bool live_alpha, const StageArgs& a, hipStream_t s);
+
+// BSSN::setExtraFieldData (bssn.cc:165-170) partial sums, one row per
+// workgroup: sum w*DIFFK, sum w, sum w*DIFFphi, sum w*DIFFr, min DIFFK with
+// w = exp(6 (DIFFphi + phi_FRW)) (utils/math.h:1382-1415, 1506-1518)
+enum { EXT_SWK = 0, EXT_SW, EXT_SWPHI, EXT_SWR, EXT_KMIN, EXT_W };
+enum { EXTV_KMIN = 0, EXTV_KAVG, EXTV_AVGVOL, EXTV_RHOAVG };
+
+// deterministic wavefront sum / min (fixed butterfly order)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double o = __shfl_xor(v, m, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+// One partial row per workgroup (fixed order: butterfly within each wave,
+// waves in index order).  Every thread of the block must call this: it holds
+// a barrier.
+__device__ __forceinline__ void block_ext_put(double* part, long row, bool on, double phi, double K, double rho,
+                                              double phiF) {
+  __shared__ double sh[16][EXT_W];
+  const double w = on ? exp(6.0 * (phi + phiF)) : 0.0;
+  const double s0 = wave_sum(on ? w * K : 0.0), s1 = wave_sum(w), s2 = wave_sum(on ? w * phi : 0.0),
+               s3 = wave_sum(on ? w * rho : 0.0), mn = wave_min(on ? K : __builtin_huge_val());
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[wv][EXT_SWK] = s0; sh[wv][EXT_SW] = s1; sh[wv][EXT_SWPHI] = s2; sh[wv][EXT_SWR] = s3; sh[wv][EXT_KMIN] = mn;
+  }
+  __syncthreads();
+  if (threadIdx.x < EXT_W) {
+    const int q = threadIdx.x, nw = (int)(blockDim.x >> 6);
+    double acc = sh[0][q];
+    for (int k = 1; k < nw; ++k) acc = q == EXT_KMIN ? (sh[k][q] < acc ? sh[k][q] : acc) : acc + sh[k][q];
+    part[row * EXT_W + q] = acc;
+  }
+}
 
 // One compiled instantiation of the stage kernel.
 struct StageVariant {

@@ -311,309 +311,326 @@ stage_kernel(const StageArgs a) {
       __syncthreads();
     }
   }
-  if (!valid) return;
+  // Threads outside the grid (ragged edge tiles) skip the algebra but stay
+  // alive for the block-wide setExtraFieldData reduction at the end.
+  double phi_new = 0.0, K_new = 0.0, rho_new = 0.0;  // stage output and source seen by the reduction
+  if (valid) {
+    // ---------------------------------------------------------- algebra
+    const TC phiF = (TC)a.phi_F, KF = (TC)a.K_F, rhoF = (TC)a.rho_F, SF = (TC)a.S_F;
+    const TC PI = TC(3.14159265358979323846);
+    const TC alpha = v[ALPHA] + TC(1);
+    const TC phi = (TC)(phi_c + a.phi_F);
+    const TC K = v[KK] + KF;
 
-  // ---------------------------------------------------------- algebra
-  const TC phiF = (TC)a.phi_F, KF = (TC)a.K_F, rhoF = (TC)a.rho_F, SF = (TC)a.S_F;
-  const TC PI = TC(3.14159265358979323846);
-  const TC alpha = v[ALPHA] + TC(1);
-  const TC phi = (TC)(phi_c + a.phi_F);
-  const TC K = v[KK] + KF;
-
-  // sources (components/static/static.cc:28-35, components/Lambda/lambda.cc:32-41)
-  TC DIFFr = TC(0), DIFFS = TC(0);
-  if (a.matter == MATTER_STATIC) {
-    const TC D = (TC)ldo((const TS*)a.DIFFD, idx);
-    DIFFr = exp((TC)(-6.0 * (phi_c + a.phi_F))) * D + rhoF * expm1(TC(-6) * v[PHI]);
-  }
-  DIFFr += (TC)a.lambda;
-  DIFFS += TC(-3) * (TC)a.lambda;
-
-  // A^ij and A_ij A^ij (bssn.cc:463-472)
-  TC Ac[6];
-#pragma unroll
-  for (int I = 0; I < 3; ++I)
-#pragma unroll
-    for (int J = I; J < 3; ++J) {
-      TC acc = TC(0);
-#pragma unroll
-      for (int p = 0; p < 3; ++p) {
-        const TC w = GI(I, p);
-        acc += w * (GI(J, 0) * AM(p, 0) + GI(J, 1) * AM(p, 1) + GI(J, 2) * AM(p, 2));
-      }
-      Ac[sy(I, J)] = acc;
+    // sources (components/static/static.cc:28-35, components/Lambda/lambda.cc:32-41)
+    TC DIFFr = TC(0), DIFFS = TC(0);
+    if (a.matter == MATTER_STATIC) {
+      const TC D = (TC)ldo((const TS*)a.DIFFD, idx);
+      DIFFr = exp((TC)(-6.0 * (phi_c + a.phi_F))) * D + rhoF * expm1(TC(-6) * v[PHI]);
     }
-  const TC AijAij = Ac[0] * v[A11] + Ac[3] * v[A22] + Ac[5] * v[A33] +
-                    TC(2) * (Ac[1] * v[A12] + Ac[2] * v[A13] + Ac[4] * v[A23]);
+    DIFFr += (TC)a.lambda;
+    DIFFS += TC(-3) * (TC)a.lambda;
 
-  // Christoffels (bssn_macros.h:221-229):  GL_ijk = 1/2(d_j g_ki + d_k g_ji - d_i g_jk),
-  // G^i_jk = gamma^{im} GL_mjk
-  TC GL[3][6], G[3][6];
-#pragma unroll
-  for (int I = 0; I < 3; ++I)
-#pragma unroll
-    for (int J = 0; J < 3; ++J)
-#pragma unroll
-      for (int Kx = J; Kx < 3; ++Kx)
-        GL[I][sy(J, Kx)] = TC(0.5) * (dg[J][sy(Kx, I)] + dg[Kx][sy(J, I)] - dg[I][sy(J, Kx)]);
-#pragma unroll
-  for (int I = 0; I < 3; ++I)
-#pragma unroll
-    for (int s = 0; s < 6; ++s) G[I][s] = GI(I, 0) * GL[0][s] + GI(I, 1) * GL[1][s] + GI(I, 2) * GL[2][s];
-  TC Gd[3];
-#pragma unroll
-  for (int I = 0; I < 3; ++I)
-    Gd[I] = G[I][0] * gi[0] + G[I][3] * gi[3] + G[I][5] * gi[5] + TC(2) * (G[I][1] * gi[1] + G[I][2] * gi[2] + G[I][4] * gi[4]);
-
-  // D_i D_j phi (bssn.cc:592-602)
-  TC DDphi[6];
-#pragma unroll
-  for (int s = 0; s < 6; ++s) DDphi[s] = ddphi[s] - (G[0][s] * dphi[0] + G[1][s] * dphi[1] + G[2][s] * dphi[2]);
-
-  // (D_i D_j alpha)^TF and its trace (bssn.cc:604-626)
-  TC DDa[6], DDaTR = TC(0);
-#pragma unroll
-  for (int q = 0; q < 6; ++q) DDa[q] = TC(0);
-  if constexpr (LIVE_ALPHA) {
-    TC gild[3];
-#pragma unroll
-    for (int I = 0; I < 3; ++I) gild[I] = GI(I, 0) * dphi[0] + GI(I, 1) * dphi[1] + GI(I, 2) * dphi[2];
-#pragma unroll
+    // A^ij and A_ij A^ij (bssn.cc:463-472)
+    TC Ac[6];
+  #pragma unroll
     for (int I = 0; I < 3; ++I)
-#pragma unroll
+  #pragma unroll
       for (int J = I; J < 3; ++J) {
-        const int s = sy(I, J);
         TC acc = TC(0);
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-          acc += (G[m][s] + TC(2) * ((m == I ? dphi[J] : TC(0)) + (m == J ? dphi[I] : TC(0)) - GM(I, J) * gild[m])) * dal[m];
-        DDa[s] = ddal[s] - acc;
+  #pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          const TC w = GI(I, p);
+          acc += w * (GI(J, 0) * AM(p, 0) + GI(J, 1) * AM(p, 1) + GI(J, 2) * AM(p, 2));
+        }
+        Ac[sy(I, J)] = acc;
       }
-    DDaTR = gi[0] * DDa[0] + gi[3] * DDa[3] + gi[5] * DDa[5] + TC(2) * (gi[1] * DDa[1] + gi[2] * DDa[2] + gi[4] * DDa[4]);
-#pragma unroll
-    for (int s = 0; s < 6; ++s) DDa[s] -= TC(1.0 / 3.0) * (v[G11 + s] + (diag6(s) ? TC(1) : TC(0))) * DDaTR;
-  }
-  const TC em4phi = exp(TC(-4) * phi);
-  DDaTR *= em4phi;
+    const TC AijAij = Ac[0] * v[A11] + Ac[3] * v[A22] + Ac[5] * v[A33] +
+                      TC(2) * (Ac[1] * v[A12] + Ac[2] * v[A13] + Ac[4] * v[A23]);
 
-  // conformal Ricci (bssn_macros.h:248-277).  TERM4 is regrouped as
-  //   sum_{K,M} X[K][I][M] Y[J][K][M] + (I<->J),
-  //   X[K][I][M] = gamma^{ML} G^K_{LI},  Y[J][K][M] = GL_{J,KM} + 1/2 GL_{K,MJ}
-  TC Z[3][3];
-#pragma unroll
-  for (int I = 0; I < 3; ++I)
-#pragma unroll
-    for (int J = 0; J < 3; ++J) Z[I][J] = TC(0);
-#pragma unroll
-  for (int Kx = 0; Kx < 3; ++Kx) {
-    TC X[3][3], Y[3][3];
-#pragma unroll
+    // Christoffels (bssn_macros.h:221-229):  GL_ijk = 1/2(d_j g_ki + d_k g_ji - d_i g_jk),
+    // G^i_jk = gamma^{im} GL_mjk
+    TC GL[3][6], G[3][6];
+  #pragma unroll
     for (int I = 0; I < 3; ++I)
-#pragma unroll
-      for (int M = 0; M < 3; ++M)
-        X[I][M] = GI(M, 0) * G[Kx][sy(0, I)] + GI(M, 1) * G[Kx][sy(1, I)] + GI(M, 2) * G[Kx][sy(2, I)];
-#pragma unroll
-    for (int J = 0; J < 3; ++J)
-#pragma unroll
-      for (int M = 0; M < 3; ++M) Y[J][M] = GL[J][sy(Kx, M)] + TC(0.5) * GL[Kx][sy(M, J)];
-#pragma unroll
+  #pragma unroll
+      for (int J = 0; J < 3; ++J)
+  #pragma unroll
+        for (int Kx = J; Kx < 3; ++Kx)
+          GL[I][sy(J, Kx)] = TC(0.5) * (dg[J][sy(Kx, I)] + dg[Kx][sy(J, I)] - dg[I][sy(J, Kx)]);
+  #pragma unroll
     for (int I = 0; I < 3; ++I)
-#pragma unroll
-      for (int J = 0; J < 3; ++J) Z[I][J] += X[I][0] * Y[J][0] + X[I][1] * Y[J][1] + X[I][2] * Y[J][2];
-  }
-  TC ric[6];
-  {
-    // phi part (bssn.cc:639-655)
-    const TC expression =
-        gi[0] * (DDphi[0] + TC(2) * dphi[0] * dphi[0]) + gi[3] * (DDphi[3] + TC(2) * dphi[1] * dphi[1]) +
-        gi[5] * (DDphi[5] + TC(2) * dphi[2] * dphi[2]) +
-        TC(2) * (gi[1] * (DDphi[1] + TC(2) * dphi[0] * dphi[1]) + gi[2] * (DDphi[2] + TC(2) * dphi[0] * dphi[2]) +
-                 gi[4] * (DDphi[4] + TC(2) * dphi[1] * dphi[2]));
-#pragma unroll
+  #pragma unroll
+      for (int s = 0; s < 6; ++s) G[I][s] = GI(I, 0) * GL[0][s] + GI(I, 1) * GL[1][s] + GI(I, 2) * GL[2][s];
+    TC Gd[3];
+  #pragma unroll
     for (int I = 0; I < 3; ++I)
-#pragma unroll
+      Gd[I] = G[I][0] * gi[0] + G[I][3] * gi[3] + G[I][5] * gi[5] + TC(2) * (G[I][1] * gi[1] + G[I][2] * gi[2] + G[I][4] * gi[4]);
+
+    // D_i D_j phi (bssn.cc:592-602)
+    TC DDphi[6];
+  #pragma unroll
+    for (int s = 0; s < 6; ++s) DDphi[s] = ddphi[s] - (G[0][s] * dphi[0] + G[1][s] * dphi[1] + G[2][s] * dphi[2]);
+
+    // (D_i D_j alpha)^TF and its trace (bssn.cc:604-626)
+    TC DDa[6], DDaTR = TC(0);
+  #pragma unroll
+    for (int q = 0; q < 6; ++q) DDa[q] = TC(0);
+    if constexpr (LIVE_ALPHA) {
+      TC gild[3];
+  #pragma unroll
+      for (int I = 0; I < 3; ++I) gild[I] = GI(I, 0) * dphi[0] + GI(I, 1) * dphi[1] + GI(I, 2) * dphi[2];
+  #pragma unroll
+      for (int I = 0; I < 3; ++I)
+  #pragma unroll
+        for (int J = I; J < 3; ++J) {
+          const int s = sy(I, J);
+          TC acc = TC(0);
+  #pragma unroll
+          for (int m = 0; m < 3; ++m)
+            acc += (G[m][s] + TC(2) * ((m == I ? dphi[J] : TC(0)) + (m == J ? dphi[I] : TC(0)) - GM(I, J) * gild[m])) * dal[m];
+          DDa[s] = ddal[s] - acc;
+        }
+      DDaTR = gi[0] * DDa[0] + gi[3] * DDa[3] + gi[5] * DDa[5] + TC(2) * (gi[1] * DDa[1] + gi[2] * DDa[2] + gi[4] * DDa[4]);
+  #pragma unroll
+      for (int s = 0; s < 6; ++s) DDa[s] -= TC(1.0 / 3.0) * (v[G11 + s] + (diag6(s) ? TC(1) : TC(0))) * DDaTR;
+    }
+    const TC em4phi = exp(TC(-4) * phi);
+    DDaTR *= em4phi;
+
+    // conformal Ricci (bssn_macros.h:248-277).  TERM4 is regrouped as
+    //   sum_{K,M} X[K][I][M] Y[J][K][M] + (I<->J),
+    //   X[K][I][M] = gamma^{ML} G^K_{LI},  Y[J][K][M] = GL_{J,KM} + 1/2 GL_{K,MJ}
+    TC Z[3][3];
+  #pragma unroll
+    for (int I = 0; I < 3; ++I)
+  #pragma unroll
+      for (int J = 0; J < 3; ++J) Z[I][J] = TC(0);
+  #pragma unroll
+    for (int Kx = 0; Kx < 3; ++Kx) {
+      TC X[3][3], Y[3][3];
+  #pragma unroll
+      for (int I = 0; I < 3; ++I)
+  #pragma unroll
+        for (int M = 0; M < 3; ++M)
+          X[I][M] = GI(M, 0) * G[Kx][sy(0, I)] + GI(M, 1) * G[Kx][sy(1, I)] + GI(M, 2) * G[Kx][sy(2, I)];
+  #pragma unroll
+      for (int J = 0; J < 3; ++J)
+  #pragma unroll
+        for (int M = 0; M < 3; ++M) Y[J][M] = GL[J][sy(Kx, M)] + TC(0.5) * GL[Kx][sy(M, J)];
+  #pragma unroll
+      for (int I = 0; I < 3; ++I)
+  #pragma unroll
+        for (int J = 0; J < 3; ++J) Z[I][J] += X[I][0] * Y[J][0] + X[I][1] * Y[J][1] + X[I][2] * Y[J][2];
+    }
+    TC ric[6];
+    {
+      // phi part (bssn.cc:639-655)
+      const TC expression =
+          gi[0] * (DDphi[0] + TC(2) * dphi[0] * dphi[0]) + gi[3] * (DDphi[3] + TC(2) * dphi[1] * dphi[1]) +
+          gi[5] * (DDphi[5] + TC(2) * dphi[2] * dphi[2]) +
+          TC(2) * (gi[1] * (DDphi[1] + TC(2) * dphi[0] * dphi[1]) + gi[2] * (DDphi[2] + TC(2) * dphi[0] * dphi[2]) +
+                   gi[4] * (DDphi[4] + TC(2) * dphi[1] * dphi[2]));
+  #pragma unroll
+      for (int I = 0; I < 3; ++I)
+  #pragma unroll
+        for (int J = I; J < 3; ++J) {
+          const int s = sy(I, J);
+          TC t2 = TC(0), t3 = TC(0);
+  #pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            t2 += GM(k, I) * dG[J][k] + GM(k, J) * dG[I][k];
+            t3 += Gd[k] * (GL[I][sy(J, k)] + GL[J][sy(I, k)]);
+          }
+          const TC U = TC(-0.5) * T1[s] + TC(0.5) * t2 + TC(0.5) * t3 + (Z[I][J] + Z[J][I]);
+          ric[s] = U - TC(2) * (DDphi[s] - TC(2) * dphi[I] * dphi[J] + GM(I, J) * expression);
+        }
+    }
+    const TC ricci = em4phi * (ric[0] * gi[0] + ric[3] * gi[3] + ric[5] * gi[5] +
+                               TC(2) * (ric[1] * gi[1] + ric[2] * gi[2] + ric[4] * gi[4]));
+    const TC e4phi = exp(TC(4) * phi);
+
+    // Hamiltonian constraint (bssn.cc:1073-1084), only when a damping term uses it
+    TC Hc = TC(0);
+    if (a.use_H || MODE == 1) {
+      const TC rr = DIFFr + rhoF;
+      if constexpr (sizeof(TC) == 8) {
+        Hc = -exp(TC(5) * phi) / TC(8) * (ricci + TC(2.0 / 3.0) * K * K - AijAij - TC(16) * PI * rr);
+      } else {
+        // split precision: (2/3)K^2 - 16 pi rho in difference form around FRW
+        const TC dK2 = TC(2.0 / 3.0) * (TC(2) * KF + v[KK]) * v[KK];
+        Hc = -exp(TC(5) * phi) / TC(8) * (ricci + ((TC)a.H_frw + dK2) - AijAij - TC(16) * PI * DIFFr);
+      }
+    }
+
+    if constexpr (MODE == 1) {
+      // setConstraintCalcs (bssn.cc:965-1071) per-point quantities, FP64
+      double* __restrict__ co = a.cons;
+      const long cs = a.cons_stride;
+      auto put = [&](int q, double val) { co[(long)q * cs + idx] = val; };
+      const double rr = (double)DIFFr + a.rho_F;
+      const double e5 = exp(5.0 * (double)phi), e6 = exp(6.0 * (double)phi);
+      // hamiltonianConstraintScale (bssn.cc:1086-1100)
+      const double Hs = (e5 / 8.0) * sqrt((double)ricci * ricci + (double)AijAij * AijAij +
+                                          pow(2.0 / 3.0 * (double)K * K, 2) + pow(16.0 * 3.14159265358979323846 * rr, 2));
+      put(CONS_H, Hc); put(CONS_H + 1, Hs); put(CONS_H + 2, Hc / Hs);
+      // BSSN_MI / BSSN_MI_SCALE (bssn_macros.h:487-527), S_i = 0 for static dust
+      double M2 = 0, Ms2 = 0;
+  #pragma unroll
+      for (int I = 0; I < 3; ++I) {
+        double t6 = 0, td = 0;
+  #pragma unroll
+        for (int n = 0; n < 3; ++n)
+  #pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            t6 += GI(m, n) * AM(m, I) * dphi[n];
+            td += GI(m, n) * dA[n][sy(m, I)];
+          }
+  #pragma unroll
+        for (int m = 0; m < 3; ++m) td -= Gd[m] * AM(m, I);
+  #pragma unroll
+        for (int n = 0; n < 3; ++n)
+  #pragma unroll
+          for (int m = 0; m < 3; ++m) td -= GL[m][sy(n, I)] * Ac[sy(m, n)];
+        const double MI = e6 * (-2.0 / 3.0 * dK[I] + 6.0 * t6 + td);
+        const double MS = e6 * (fabs(2.0 / 3.0 * dK[I]) + 6.0 * fabs(t6) + fabs(td));
+        M2 += MI * MI;
+        Ms2 += MS * MS;
+      }
+      const double Mv = sqrt(M2), Msc = sqrt(Ms2);
+      put(CONS_M, Mv); put(CONS_M + 1, Msc); put(CONS_M + 2, Mv / Msc);
+      // BSSN_GI_CALC / BSSN_GI_SCALE (bssn_macros.h:530-536)
+      double G2 = 0, Gs2 = 0;
+  #pragma unroll
+      for (int I = 0; I < 3; ++I) {
+        const double tr = gi[0] * G[I][0] + gi[3] * G[I][3] + gi[5] * G[I][5] +
+                          2.0 * (gi[1] * G[I][1] + gi[2] * G[I][2] + gi[4] * G[I][4]);
+        const double gc_ = v[GAM1 + I] - tr;
+        const double gs_ = fabs((double)v[GAM1 + I]) +
+                           fabs(gi[0] * G[I][0] - gi[3] * G[I][3] - gi[5] * G[I][5] -
+                                2.0 * (gi[1] * G[I][1] + gi[2] * G[I][2] + gi[4] * G[I][4]));
+        G2 += gc_ * gc_;
+        Gs2 += gs_ * gs_;
+      }
+      put(CONS_G, sqrt(G2)); put(CONS_G + 1, sqrt(Gs2)); put(CONS_G + 2, sqrt(G2) / sqrt(Gs2));
+      // AijTFConstraintCalc / Scale (bssn.cc:1182-1192)
+      const double At = gi[0] * v[A11] + gi[3] * v[A22] + gi[5] * v[A33] +
+                        2.0 * (gi[1] * v[A12] + gi[2] * v[A13] + gi[4] * v[A23]);
+      const double As = fabs(gi[0] * v[A11]) + fabs(gi[3] * v[A22]) + fabs(gi[5] * v[A33]) +
+                        2.0 * (fabs(gi[1] * v[A12]) + fabs(gi[2] * v[A13]) + fabs(gi[4] * v[A23]));
+      put(CONS_A, At); put(CONS_A + 1, As); put(CONS_A + 2, At / As);
+      // unitDetConstraintCalc / Scale (bssn.cc:1199-1219)
+      {
+        const double g11 = v[G11], g12 = v[G12], g13 = v[G13], g22 = v[G22], g23 = v[G23], g33 = v[G33];
+        const double Sv = fabs(g11 + g22 + g33 - g12 * g12 - g13 * g13 - g23 * g23 + g11 * g22 + g11 * g33 + g22 * g33 -
+                               g23 * g23 * g11 - g13 * g13 * g22 - g12 * g12 * g33 + 2.0 * g12 * g13 * g23 +
+                               g11 * g22 * g33);
+        const double Ss = fabs(g11) + fabs(g22) + fabs(g33) + g12 * g12 + g13 * g13 + g23 * g23 + fabs(g11 * g22) +
+                          fabs(g11 * g33) + fabs(g22 * g33) + g23 * g23 * g11 + g13 * g13 * g22 + g12 * g12 * g33 +
+                          2.0 * fabs(g12 * g13 * g23) + fabs(g11 * g22 * g33);
+        put(CONS_S, Sv); put(CONS_S + 1, Ss); put(CONS_S + 2, Sv / Ss);
+      }
+      put(CONS_RICCI, ricci);
+      put(CONS_AIJAIJ, AijAij);
+      return;
+    }
+
+    // ------------------------------------------------ right-hand sides
+    TC rhs[NF];
+    const TC dt = (TC)a.dt;
+    const TC adj = (TC)a.a_adj;
+  #pragma unroll
+    for (int s = 0; s < 6; ++s)
+      rhs[G11 + s] = TC(-2) * alpha * v[A11 + s] + TC(0.5) * adj * dt * Hc * v[G11 + s] - ko[G11 + s];
+    // bssn_macros.h:309-338
+  #pragma unroll
+    for (int I = 0; I < 3; ++I)
+  #pragma unroll
       for (int J = I; J < 3; ++J) {
         const int s = sy(I, J);
-        TC t2 = TC(0), t3 = TC(0);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          t2 += GM(k, I) * dG[J][k] + GM(k, J) * dG[I][k];
-          t3 += Gd[k] * (GL[I][sy(J, k)] + GL[J][sy(I, k)]);
-        }
-        const TC U = TC(-0.5) * T1[s] + TC(0.5) * t2 + TC(0.5) * t3 + (Z[I][J] + Z[J][I]);
-        ric[s] = U - TC(2) * (DDphi[s] - TC(2) * dphi[I] * dphi[J] + GM(I, J) * expression);
+        TC AA = TC(0);
+  #pragma unroll
+        for (int m = 0; m < 3; ++m)
+          AA += AM(m, I) * (GI(m, 0) * AM(0, J) + GI(m, 1) * AM(1, J) + GI(m, 2) * AM(2, J));
+        const TC ricTF = ric[s] - TC(1.0 / 3.0) * e4phi * GM(I, J) * ricci;
+        const TC STF = TC(0);  // static dust and Lambda have no trace-free stress
+        rhs[A11 + s] = em4phi * (alpha * (ricTF - TC(8) * PI * STF) - DDa[s]) +
+                       alpha * (K * v[A11 + s] - TC(2) * AA) - adj * dt * v[A11 + s] * Hc - ko[A11 + s];
       }
-  }
-  const TC ricci = em4phi * (ric[0] * gi[0] + ric[3] * gi[3] + ric[5] * gi[5] +
-                             TC(2) * (ric[1] * gi[1] + ric[2] * gi[2] + ric[4] * gi[4]));
-  const TC e4phi = exp(TC(4) * phi);
-
-  // Hamiltonian constraint (bssn.cc:1073-1084), only when a damping term uses it
-  TC Hc = TC(0);
-  if (a.use_H || MODE == 1) {
-    const TC rr = DIFFr + rhoF;
-    if constexpr (sizeof(TC) == 8) {
-      Hc = -exp(TC(5) * phi) / TC(8) * (ricci + TC(2.0 / 3.0) * K * K - AijAij - TC(16) * PI * rr);
-    } else {
-      // split precision: (2/3)K^2 - 16 pi rho in difference form around FRW
-      const TC dK2 = TC(2.0 / 3.0) * (TC(2) * KF + v[KK]) * v[KK];
-      Hc = -exp(TC(5) * phi) / TC(8) * (ricci + ((TC)a.H_frw + dK2) - AijAij - TC(16) * PI * DIFFr);
-    }
-  }
-
-  if constexpr (MODE == 1) {
-    // setConstraintCalcs (bssn.cc:965-1071) per-point quantities, FP64
-    double* __restrict__ co = a.cons;
-    const long cs = a.cons_stride;
-    auto put = [&](int q, double val) { co[(long)q * cs + idx] = val; };
-    const double rr = (double)DIFFr + a.rho_F;
-    const double e5 = exp(5.0 * (double)phi), e6 = exp(6.0 * (double)phi);
-    // hamiltonianConstraintScale (bssn.cc:1086-1100)
-    const double Hs = (e5 / 8.0) * sqrt((double)ricci * ricci + (double)AijAij * AijAij +
-                                        pow(2.0 / 3.0 * (double)K * K, 2) + pow(16.0 * 3.14159265358979323846 * rr, 2));
-    put(CONS_H, Hc); put(CONS_H + 1, Hs); put(CONS_H + 2, Hc / Hs);
-    // BSSN_MI / BSSN_MI_SCALE (bssn_macros.h:487-527), S_i = 0 for static dust
-    double M2 = 0, Ms2 = 0;
-#pragma unroll
+    // bssn_macros.h:340-361 (S_i = 0 for static dust / Lambda)
+  #pragma unroll
     for (int I = 0; I < 3; ++I) {
-      double t6 = 0, td = 0;
-#pragma unroll
-      for (int n = 0; n < 3; ++n)
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-          t6 += GI(m, n) * AM(m, I) * dphi[n];
-          td += GI(m, n) * dA[n][sy(m, I)];
+      const TC second = G[I][0] * Ac[0] + G[I][3] * Ac[3] + G[I][5] * Ac[5] +
+                        TC(2) * (G[I][1] * Ac[1] + G[I][2] * Ac[2] + G[I][4] * Ac[4]) +
+                        TC(6) * (Ac[sy(I, 0)] * dphi[0] + Ac[sy(I, 1)] * dphi[1] + Ac[sy(I, 2)] * dphi[2]);
+      rhs[GAM1 + I] = TC(-2) * (Ac[sy(I, 0)] * dal[0] + Ac[sy(I, 1)] * dal[1] + Ac[sy(I, 2)] * dal[2]) +
+                      TC(2) * alpha * (second - TC(2.0 / 3.0) * (GI(I, 0) * dK[0] + GI(I, 1) * dK[1] + GI(I, 2) * dK[2])) -
+                      ko[GAM1 + I];
+    }
+    // bssn.cc:766-795
+    rhs[KK] = -DDaTR + alpha * (TC(1.0 / 3.0) * v[KK] * (v[KK] + TC(2) * KF) + AijAij) +
+              TC(4) * PI * alpha * (DIFFr + DIFFS) + TC(4) * PI * v[ALPHA] * (rhoF + SF) -
+              (TC)a.k_damp * Hc * exp(TC(-5) * phi) - ko[KK];
+    // bssn.cc:797-820
+    rhs[PHI] = TC(0.1) * adj * dt * Hc - TC(1.0 / 6.0) * (alpha * v[KK] + v[ALPHA] * KF) - ko[PHI];
+    // BSSNGaugeHandler.cc:16-35
+    TC lapse = TC(0);
+    if (a.lapse == LAPSE_HARMONIC) lapse = -alpha * K;
+    else if (a.lapse == LAPSE_CONFORMAL_FLRW) lapse = TC(-1.0 / 3.0) * alpha * (TC)a.ext_in[EXTV_KAVG];
+    rhs[ALPHA] = lapse - ko[ALPHA];
+
+    // ------------------------------------------------ RK4Register epilogue
+    // K{n}Finalize (RK4Register.h:129-176): f += cf*c ; a' = p + ca*c, and at
+    // stage 4 p' = f + (cf*c + p).  Accumulation in FP64 for every precision.
+    const int st = a.stage;
+    rho_new = (double)DIFFr;
+  #pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
+      const double cc = (double)rhs[f];
+      if (metric_field(f)) {
+        double* __restrict__ o = (double*)a.out[f];
+        if (st == 0) {
+          sto(o, idx, cc);
+          continue;
         }
-#pragma unroll
-      for (int m = 0; m < 3; ++m) td -= Gd[m] * AM(m, I);
-#pragma unroll
-      for (int n = 0; n < 3; ++n)
-#pragma unroll
-        for (int m = 0; m < 3; ++m) td -= GL[m][sy(n, I)] * Ac[sy(m, n)];
-      const double MI = e6 * (-2.0 / 3.0 * dK[I] + 6.0 * t6 + td);
-      const double MS = e6 * (fabs(2.0 / 3.0 * dK[I]) + 6.0 * fabs(t6) + fabs(td));
-      M2 += MI * MI;
-      Ms2 += MS * MS;
-    }
-    const double Mv = sqrt(M2), Msc = sqrt(Ms2);
-    put(CONS_M, Mv); put(CONS_M + 1, Msc); put(CONS_M + 2, Mv / Msc);
-    // BSSN_GI_CALC / BSSN_GI_SCALE (bssn_macros.h:530-536)
-    double G2 = 0, Gs2 = 0;
-#pragma unroll
-    for (int I = 0; I < 3; ++I) {
-      const double tr = gi[0] * G[I][0] + gi[3] * G[I][3] + gi[5] * G[I][5] +
-                        2.0 * (gi[1] * G[I][1] + gi[2] * G[I][2] + gi[4] * G[I][4]);
-      const double gc_ = v[GAM1 + I] - tr;
-      const double gs_ = fabs((double)v[GAM1 + I]) +
-                         fabs(gi[0] * G[I][0] - gi[3] * G[I][3] - gi[5] * G[I][5] -
-                              2.0 * (gi[1] * G[I][1] + gi[2] * G[I][2] + gi[4] * G[I][4]));
-      G2 += gc_ * gc_;
-      Gs2 += gs_ * gs_;
-    }
-    put(CONS_G, sqrt(G2)); put(CONS_G + 1, sqrt(Gs2)); put(CONS_G + 2, sqrt(G2) / sqrt(Gs2));
-    // AijTFConstraintCalc / Scale (bssn.cc:1182-1192)
-    const double At = gi[0] * v[A11] + gi[3] * v[A22] + gi[5] * v[A33] +
-                      2.0 * (gi[1] * v[A12] + gi[2] * v[A13] + gi[4] * v[A23]);
-    const double As = fabs(gi[0] * v[A11]) + fabs(gi[3] * v[A22]) + fabs(gi[5] * v[A33]) +
-                      2.0 * (fabs(gi[1] * v[A12]) + fabs(gi[2] * v[A13]) + fabs(gi[4] * v[A23]));
-    put(CONS_A, At); put(CONS_A + 1, As); put(CONS_A + 2, At / As);
-    // unitDetConstraintCalc / Scale (bssn.cc:1199-1219)
-    {
-      const double g11 = v[G11], g12 = v[G12], g13 = v[G13], g22 = v[G22], g23 = v[G23], g33 = v[G33];
-      const double Sv = fabs(g11 + g22 + g33 - g12 * g12 - g13 * g13 - g23 * g23 + g11 * g22 + g11 * g33 + g22 * g33 -
-                             g23 * g23 * g11 - g13 * g13 * g22 - g12 * g12 * g33 + 2.0 * g12 * g13 * g23 +
-                             g11 * g22 * g33);
-      const double Ss = fabs(g11) + fabs(g22) + fabs(g33) + g12 * g12 + g13 * g13 + g23 * g23 + fabs(g11 * g22) +
-                        fabs(g11 * g33) + fabs(g22 * g33) + g23 * g23 * g11 + g13 * g13 * g22 + g12 * g12 * g33 +
-                        2.0 * fabs(g12 * g13 * g23) + fabs(g11 * g22 * g33);
-      put(CONS_S, Sv); put(CONS_S + 1, Ss); put(CONS_S + 2, Sv / Ss);
-    }
-    put(CONS_RICCI, ricci);
-    put(CONS_AIJAIJ, AijAij);
-    return;
-  }
-
-  // ------------------------------------------------ right-hand sides
-  TC rhs[NF];
-  const TC dt = (TC)a.dt;
-  const TC adj = (TC)a.a_adj;
-#pragma unroll
-  for (int s = 0; s < 6; ++s)
-    rhs[G11 + s] = TC(-2) * alpha * v[A11 + s] + TC(0.5) * adj * dt * Hc * v[G11 + s] - ko[G11 + s];
-  // bssn_macros.h:309-338
-#pragma unroll
-  for (int I = 0; I < 3; ++I)
-#pragma unroll
-    for (int J = I; J < 3; ++J) {
-      const int s = sy(I, J);
-      TC AA = TC(0);
-#pragma unroll
-      for (int m = 0; m < 3; ++m)
-        AA += AM(m, I) * (GI(m, 0) * AM(0, J) + GI(m, 1) * AM(1, J) + GI(m, 2) * AM(2, J));
-      const TC ricTF = ric[s] - TC(1.0 / 3.0) * e4phi * GM(I, J) * ricci;
-      const TC STF = TC(0);  // static dust and Lambda have no trace-free stress
-      rhs[A11 + s] = em4phi * (alpha * (ricTF - TC(8) * PI * STF) - DDa[s]) +
-                     alpha * (K * v[A11 + s] - TC(2) * AA) - adj * dt * v[A11 + s] * Hc - ko[A11 + s];
-    }
-  // bssn_macros.h:340-361 (S_i = 0 for static dust / Lambda)
-#pragma unroll
-  for (int I = 0; I < 3; ++I) {
-    const TC second = G[I][0] * Ac[0] + G[I][3] * Ac[3] + G[I][5] * Ac[5] +
-                      TC(2) * (G[I][1] * Ac[1] + G[I][2] * Ac[2] + G[I][4] * Ac[4]) +
-                      TC(6) * (Ac[sy(I, 0)] * dphi[0] + Ac[sy(I, 1)] * dphi[1] + Ac[sy(I, 2)] * dphi[2]);
-    rhs[GAM1 + I] = TC(-2) * (Ac[sy(I, 0)] * dal[0] + Ac[sy(I, 1)] * dal[1] + Ac[sy(I, 2)] * dal[2]) +
-                    TC(2) * alpha * (second - TC(2.0 / 3.0) * (GI(I, 0) * dK[0] + GI(I, 1) * dK[1] + GI(I, 2) * dK[2])) -
-                    ko[GAM1 + I];
-  }
-  // bssn.cc:766-795
-  rhs[KK] = -DDaTR + alpha * (TC(1.0 / 3.0) * v[KK] * (v[KK] + TC(2) * KF) + AijAij) +
-            TC(4) * PI * alpha * (DIFFr + DIFFS) + TC(4) * PI * v[ALPHA] * (rhoF + SF) -
-            (TC)a.k_damp * Hc * exp(TC(-5) * phi) - ko[KK];
-  // bssn.cc:797-820
-  rhs[PHI] = TC(0.1) * adj * dt * Hc - TC(1.0 / 6.0) * (alpha * v[KK] + v[ALPHA] * KF) - ko[PHI];
-  // BSSNGaugeHandler.cc:16-35
-  TC lapse = TC(0);
-  if (a.lapse == LAPSE_HARMONIC) lapse = -alpha * K;
-  else if (a.lapse == LAPSE_CONFORMAL_FLRW) lapse = TC(-1.0 / 3.0) * alpha * (TC)a.K_avg;
-  rhs[ALPHA] = lapse - ko[ALPHA];
-
-  // ------------------------------------------------ RK4Register epilogue
-  // K{n}Finalize (RK4Register.h:129-176): f += cf*c ; a' = p + ca*c, and at
-  // stage 4 p' = f + (cf*c + p).  Accumulation in FP64 for every precision.
-  const int st = a.stage;
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
-    const double cc = (double)rhs[f];
-    if (metric_field(f)) {
-      double* __restrict__ o = (double*)a.out[f];
-      if (st == 0) { sto(o, idx, cc); continue; }
-      const double p = ldo((const double*)a.p[f], idx);
-      double* __restrict__ fr = (double*)a.f[f];
-      const double fo = st >= 2 ? ldo((const double*)fr, idx) : 0.0;
-      const double inc = a.cf * cc;
-      if (st == 4) {
-        const double pn = fo + (inc + p);
-        sto(o, idx, pn);
-        if (f == PHI && nan_as_float(pn)) atomicAdd(a.nan_count, 1u);
+        const double p = ldo((const double*)a.p[f], idx);
+        double* __restrict__ fr = (double*)a.f[f];
+        const double fo = st >= 2 ? ldo((const double*)fr, idx) : 0.0;
+        const double inc = a.cf * cc;
+        const double nv = st == 4 ? fo + (inc + p) : p + a.ca * cc;
+        if (f == PHI) phi_new = nv;
+        sto(o, idx, nv);
+        if (st == 4) {
+          if (f == PHI && nan_as_float(nv)) atomicAdd(a.nan_count, 1u);
+        } else {
+          sto(fr, idx, fo + inc);
+        }
       } else {
-        sto(fr, idx, fo + inc);
-        sto(o, idx, p + a.ca * cc);
-      }
-    } else {
-      TS* __restrict__ o = (TS*)a.out[f];
-      if (st == 0) { sto(o, idx, (TS)cc); continue; }
-      const double p = (double)ldo((const TS*)a.p[f], idx);
-      TS* __restrict__ fr = (TS*)a.f[f];
-      const double fo = st >= 2 ? (double)ldo((const TS*)fr, idx) : 0.0;
-      const double inc = a.cf * cc;
-      if (st == 4) {
-        sto(o, idx, (TS)(fo + (inc + p)));
-      } else {
-        sto(fr, idx, (TS)(fo + inc));
-        sto(o, idx, (TS)(p + a.ca * cc));
+        TS* __restrict__ o = (TS*)a.out[f];
+        if (st == 0) {
+          sto(o, idx, (TS)cc);
+          continue;
+        }
+        const double p = (double)ldo((const TS*)a.p[f], idx);
+        TS* __restrict__ fr = (TS*)a.f[f];
+        const double fo = st >= 2 ? (double)ldo((const TS*)fr, idx) : 0.0;
+        const double inc = a.cf * cc;
+        const TS nv = st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc);
+        if (f == KK) K_new = (double)nv;
+        sto(o, idx, nv);
+        if (st != 4) sto(fr, idx, (TS)(fo + inc));
       }
     }
+  }  // valid
+
+  // setExtraFieldData after K{n}Finalize (bssn.cc:267-305): the averages of
+  // the new _a weighted with the new FRW phi; DIFFr is this stage's source.
+  // Only the ConformalFLRW lapse reads them inside a step, so only the
+  // live-lapse variants carry the reduction (the host computes the values
+  // of the last stage on demand otherwise).
+  if constexpr (LIVE_ALPHA) {
+    if (a.ext_part && a.stage >= 1) block_ext_put(a.ext_part, t, valid, phi_new, K_new, rho_new, a.phi_F_next);
   }
 }
 

@@ -110,16 +110,25 @@ __device__ __forceinline__ void st(const RegTable& t, int f, int s, long x, doub
   else ((TS*)t.r[f][s])[x] = (TS)v;
 }
 
+// Elementwise kernels run 256 threads per block; threads past the end keep
+// going with point 0 and store nothing, so whole blocks reach the
+// setExtraFieldData reduction (one partial row per block, see bssn_stage.h).
 template <typename TS>
-__global__ void k_step_init(RegTable t, int rp, int ra, int rf, long n, int normalize) {
-  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
+__global__ void k_step_init(RegTable t, int rp, int ra, int rf, long n, int normalize, double* ext_part,
+                            double phiF) {
+  const long xg = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = xg < n;
+  const long x = valid ? xg : 0;
   // RK4Register::stepInit (RK4Register.h:118-127)
   double v[NF];
   for (int f = 0; f < NF; ++f) {
     v[f] = ld<TS>(t, f, rp, x);
-    st<TS>(t, f, rf, x, 0.0);
+    if (valid) st<TS>(t, f, rf, x, 0.0);
   }
+  // BSSN::stepInit calls setExtraFieldData before the normalisation, which
+  // leaves DIFFphi and DIFFK alone (bssn.cc:219-226); DIFFr is not touched
+  if (ext_part) block_ext_put(ext_part, blockIdx.x, valid, v[PHI], v[KK], 0.0, phiF);
+  if (!valid) return;
   if (normalize) {
     // BSSN::set_DIFFgamma_Aij_norm (bssn.cc:61-119)
     double g11 = v[G11], g12 = v[G12], g13 = v[G13], g22 = v[G22], g23 = v[G23], g33 = v[G33];
@@ -143,21 +152,117 @@ __global__ void k_step_init(RegTable t, int rp, int ra, int rf, long n, int norm
   for (int f = 0; f < NF; ++f) st<TS>(t, f, ra, x, v[f]);
 }
 
-// RK4Register::K{n}Finalize (RK4Register.h:129-176) without the pointer swap
+// sources of the stage whose _a is given (static.cc:28-35, lambda.cc:32-41)
+struct SrcArgs {
+  const void* DIFFD;
+  int matter;
+  double phi_F, rho_F, lambda;
+};
+
 template <typename TS>
-__global__ void k_finalize(RegTable t, int rp, int rc, int rf, long n, int stage, double dt) {
-  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
+__device__ __forceinline__ double diff_r(const SrcArgs& s, double dphi, long x) {
+  double r = 0.0;
+  if (s.matter == MATTER_STATIC)
+    r = exp(-6.0 * (dphi + s.phi_F)) * (double)((const TS*)s.DIFFD)[x] + s.rho_F * expm1(-6.0 * dphi);
+  return r + s.lambda;
+}
+
+// RK4Register::K{n}Finalize (RK4Register.h:129-176) without the pointer swap,
+// then setExtraFieldData (bssn.cc:267-305) of the new _a: ra is the _a of the
+// finished stage, whose DIFFphi gives that stage's DIFFr
+template <typename TS>
+__global__ void k_finalize(RegTable t, int rp, int rc, int rf, int ra, long n, int stage, double dt,
+                           double* ext_part, SrcArgs src, double phiF_next) {
+  const long xg = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = xg < n;
+  const long x = valid ? xg : 0;
+  double phi_new = 0.0, K_new = 0.0;
+  const double phi_old = ld<TS>(t, PHI, ra, x);
   for (int f = 0; f < NF; ++f) {
     const double c = ld<TS>(t, f, rc, x), p = ld<TS>(t, f, rp, x), fo = ld<TS>(t, f, rf, x);
-    if (stage == 1) { st<TS>(t, f, rf, x, fo + dt * c / 6.0); st<TS>(t, f, rc, x, p + dt * c / 2.0); }
-    else if (stage == 2) { st<TS>(t, f, rf, x, fo + dt * c / 3.0); st<TS>(t, f, rc, x, p + dt * c / 2.0); }
-    else if (stage == 3) { st<TS>(t, f, rf, x, fo + dt * c / 3.0); st<TS>(t, f, rc, x, p + dt * c); }
-    else {
-      const double fn = fo + (dt * c / 6.0 + p);
-      st<TS>(t, f, rf, x, fn); st<TS>(t, f, rc, x, fn); st<TS>(t, f, rp, x, fn);
+    double fn, cn;
+    if (stage == 1) { fn = fo + dt * c / 6.0; cn = p + dt * c / 2.0; }
+    else if (stage == 2) { fn = fo + dt * c / 3.0; cn = p + dt * c / 2.0; }
+    else if (stage == 3) { fn = fo + dt * c / 3.0; cn = p + dt * c; }
+    else { fn = fo + (dt * c / 6.0 + p); cn = fn; }
+    if (f == PHI) phi_new = cn;
+    if (f == KK) K_new = is_metric(f) ? cn : (double)(TS)cn;
+    if (!valid) continue;
+    st<TS>(t, f, rf, x, fn);
+    st<TS>(t, f, rc, x, cn);
+    if (stage == 4) st<TS>(t, f, rp, x, fn);
+  }
+  if (ext_part) block_ext_put(ext_part, blockIdx.x, valid, phi_new, K_new, diff_r<TS>(src, phi_old, x), phiF_next);
+}
+
+// setExtraFieldData after a fused step's K4Finalize: the new _p, and the
+// stage-4 input register for that stage's DIFFr
+template <typename TS>
+__global__ void k_ext_post(const double* __restrict__ phi, const TS* __restrict__ K,
+                           const double* __restrict__ phi_in, long n, double* ext_part, SrcArgs src, double phiF) {
+  const long xg = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = xg < n;
+  const long x = valid ? xg : 0;
+  block_ext_put(ext_part, blockIdx.x, valid, phi[x], (double)K[x], diff_r<TS>(src, phi_in[x], x), phiF);
+}
+
+// setExtraFieldData over a register alone (DIFFr part left out): used when
+// the averages are needed before any stage has produced them
+template <typename TS>
+__global__ void k_ext_state(const double* __restrict__ phi, const TS* __restrict__ K, long n, double* ext_part,
+                            double phiF) {
+  const long xg = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = xg < n;
+  const long x = valid ? xg : 0;
+  block_ext_put(ext_part, blockIdx.x, valid, phi[x], (double)K[x], 0.0, phiF);
+}
+
+// Deterministic combine of partial rows: block b reduces the contiguous
+// range [b*per, (b+1)*per) with a fixed per-thread stride and a fixed tree;
+// with gridDim.x == 1 and ext != null it also forms the four averages of
+// setExtraFieldData (math.h:1382-1415: sum/POINTS/vol, vol = sum w/POINTS).
+constexpr int kExtThreads = 256;
+constexpr int kExtMid = 64;  // first-level combine blocks
+__global__ void k_ext_reduce(const double* __restrict__ in, long rows, double* __restrict__ out, double* ext,
+                             double npts, int keep_rho) {
+  __shared__ double sh[EXT_W][kExtThreads];
+  const long per = (rows + gridDim.x - 1) / gridDim.x;
+  const long lo = (long)blockIdx.x * per, hi = lo + per < rows ? lo + per : rows;
+  double acc[EXT_W] = {0, 0, 0, 0, __builtin_huge_val()};
+  for (long r = lo + threadIdx.x; r < hi; r += kExtThreads)
+    for (int q = 0; q < EXT_W; ++q) {
+      const double v = in[r * EXT_W + q];
+      acc[q] = q == EXT_KMIN ? (v < acc[q] ? v : acc[q]) : acc[q] + v;
+    }
+  for (int q = 0; q < EXT_W; ++q) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = kExtThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int q = 0; q < EXT_W; ++q) {
+        const double o = sh[q][threadIdx.x + w];
+        sh[q][threadIdx.x] = q == EXT_KMIN ? (o < sh[q][threadIdx.x] ? o : sh[q][threadIdx.x]) : sh[q][threadIdx.x] + o;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < EXT_W; ++q) out[(long)blockIdx.x * EXT_W + q] = sh[q][0];
+    if (ext) {
+      const double vol = sh[EXT_SW][0] / npts;
+      ext[EXTV_KMIN] = sh[EXT_KMIN][0];
+      ext[EXTV_KAVG] = sh[EXT_SWK][0] / npts / vol;
+      ext[EXTV_AVGVOL] = exp(6.0 * (sh[EXT_SWPHI][0] / npts / vol));
+      if (!keep_rho) ext[EXTV_RHOAVG] = sh[EXT_SWR][0] / npts / vol;
     }
   }
+}
+
+// numNaNs (math.h:1524-1541): NaN test of the float-rounded value
+template <typename TS>
+__global__ void k_num_nans(const TS* __restrict__ f, long n, unsigned long long* count) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  const float v = (float)f[x];
+  if ((__float_as_uint(v) << 1) > 0xff000000u) atomicAdd(count, 1ull);
 }
 
 // ---------------------------------------------- constraint statistics --
@@ -261,6 +366,22 @@ struct cg_ctx {
   void* DIFFD = nullptr;
   unsigned* nan_dev = nullptr;
   unsigned nan_seen = 0;
+  // setExtraFieldData (bssn.cc:165-170): [K_min, K_avg, avg_vol, rho_avg] on
+  // the device (read by the ConformalFLRW lapse of the next stage), the
+  // per-block partial rows, and the first-level combine rows
+  double* ext = nullptr;
+  double* ext_part = nullptr;
+  double* ext_mid = nullptr;
+  long ext_rows = 0;
+  long last_rows = 0;                     // partial rows written by the last stage launch
+  bool ext_fresh = false;                 // ext holds the averages of the current _p
+  // after a fused step without the ConformalFLRW lapse the averages of its
+  // K4Finalize are formed on demand from _p and the stage-4 input register
+  bool ext_lazy = false;
+  int ext_lazy_in = 0;
+  SrcArgs ext_lazy_src{};
+  double ext_lazy_phiF = 0;
+  unsigned long long* nan_cnt = nullptr;  // numNaNs counter
   hipStream_t stream = nullptr;
   Frw frw;
   double lambda = 0;
@@ -281,6 +402,8 @@ static int fail(cg_ctx* c, int code, const std::string& msg) {
     hipError_t e_ = (expr);                                                            \
     if (e_ != hipSuccess) return fail(ctx, CG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+static int ext_resolve(cg_ctx* c);
 
 static void* reg_ptr(cg_ctx* c, int f, int slot) {
   if (is_metric(f)) return c->met + ((long)f * 4 + slot) * c->rstride;  // f = 0..6
@@ -348,7 +471,8 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   if (p.stencil_order != 2 && p.stencil_order != 4 && p.stencil_order != 6 && p.stencil_order != 8) return CG_ERR_ARG;
   if (p.precision != CG_FP64 && p.precision != CG_FP32_SPLIT) return CG_ERR_ARG;
   if (p.matter != CG_MATTER_NONE && p.matter != CG_MATTER_STATIC) return CG_ERR_ARG;
-  if (p.lapse == CG_LAPSE_CONFORMAL_FLRW) return CG_ERR_ARG;  // needs per-stage K_avg reductions
+  if (p.lapse != CG_LAPSE_STATIC && p.lapse != CG_LAPSE_HARMONIC && p.lapse != CG_LAPSE_CONFORMAL_FLRW)
+    return CG_ERR_ARG;
   if (!(p.dx > 0) || !std::isfinite(p.dt)) return CG_ERR_ARG;
   cg_ctx* c = new cg_ctx();
   c->p = p;
@@ -373,6 +497,19 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   HIPCHK(c, hipMemsetAsync(c->DIFFD, 0, (size_t)pad * c->ts, c->stream));
   HIPCHK(c, hipMalloc(&c->nan_dev, sizeof(unsigned)));
   HIPCHK(c, hipMemsetAsync(c->nan_dev, 0, sizeof(unsigned), c->stream));
+  // partial rows: one per tile of any compiled stage variant, or one per
+  // 256-point block of the elementwise kernels
+  long rows = (c->npts + 255) / 256;
+  for (const auto& v : stage_variants()) {
+    const long nt = (long)((p.nx + v.ti - 1) / v.ti) * ((p.ny + v.tj - 1) / v.tj) * ((p.nz + v.tk - 1) / v.tk);
+    rows = nt > rows ? nt : rows;
+  }
+  c->ext_rows = rows;
+  HIPCHK(c, hipMalloc((void**)&c->ext_part, (size_t)rows * EXT_W * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->ext_mid, (size_t)(kExtMid + 1) * EXT_W * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->ext, 4 * sizeof(double)));
+  HIPCHK(c, hipMemsetAsync(c->ext, 0, 4 * sizeof(double), c->stream));
+  HIPCHK(c, hipMalloc((void**)&c->nan_cnt, sizeof(unsigned long long)));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CG_OK;
 }
@@ -387,6 +524,10 @@ int cg_destroy(cg_ctx* c) {
   if (c->cons) (void)hipFree(c->cons);
   if (c->cons_part) (void)hipFree(c->cons_part);
   if (c->nan_dev) (void)hipFree(c->nan_dev);
+  if (c->ext) (void)hipFree(c->ext);
+  if (c->ext_part) (void)hipFree(c->ext_part);
+  if (c->ext_mid) (void)hipFree(c->ext_mid);
+  if (c->nan_cnt) (void)hipFree(c->nan_cnt);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CG_OK;
@@ -430,6 +571,8 @@ int cg_field_upload(cg_ctx* c, const char* name, const double* host, size_t n) {
   void* d = resolve(c, name, true, &e, &esz);
   if (!d) return fail(c, e == 2 ? CG_ERR_STATE : CG_ERR_ARG,
                       e == 2 ? "_a/_f alias _p after a fused step; write _p" : std::string("unknown field ") + name);
+  if (int rc = ext_resolve(c)) return rc;  // the averages of the state before this write
+  c->ext_fresh = false;
   if (!strncmp(name, "DIFFalpha_", 10))
     for (size_t x = 0; x < n; ++x)
       if (host[x] != 0.0) { c->alpha_nonzero = true; break; }
@@ -468,6 +611,7 @@ int cg_field_download(cg_ctx* c, const char* name, double* host, size_t n) {
 int cg_frw_set(cg_ctx* c, double phi, double K) {
   if (!c) return CG_ERR_ARG;
   c->frw.set(phi, K);
+  c->ext_fresh = false;
   return CG_OK;
 }
 int cg_frw_add_fluid(cg_ctx* c, double rho, double w) {
@@ -511,7 +655,9 @@ static StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int ro
   a.cf = (stage == 1 || stage == 4) ? dt / 6.0 : dt / 3.0;
   a.ca = (stage == 3) ? dt : dt / 2.0;
   a.phi_F = c->frw.g_phi; a.K_F = c->frw.g_K; a.rho_F = c->frw.g_rho; a.S_F = c->frw.g_S;
-  a.K_avg = 0.0;
+  a.ext_in = c->ext;
+  a.ext_part = c->p.lapse == CG_LAPSE_CONFORMAL_FLRW ? c->ext_part : nullptr;
+  a.phi_F_next = c->frw.g_phi;
   a.lambda = c->lambda;
   const double dx = c->p.dx, ko = c->p.ko_coeff;
   // KO_dissipation_Q (math.h:36-72): ko*dx^3/64 * sum/dx^4 ; -ko*dx^9/1024 * sum/dx^10
@@ -539,6 +685,8 @@ static int launch(cg_ctx* c, StageArgs a) {
   a.nty = (c->p.ny + c->var->tj - 1) / c->var->tj;
   a.ntz = (c->p.nz + c->var->tk - 1) / c->var->tk;
   a.per_xcd = (a.ntx * a.nty * a.ntz + 7) / 8;
+  c->last_rows = (long)a.ntx * a.nty * a.ntz;
+  if (c->last_rows > c->ext_rows) return fail(c, CG_ERR_STATE, "extra-field partial buffer too small");
   int rc = c->var->launch(a, c->stream);
   if (rc != 0) return fail(c, CG_ERR_HIP, std::string("stage launch: ") + hipGetErrorString((hipError_t)rc));
   if (c->timing) {
@@ -548,18 +696,69 @@ static int launch(cg_ctx* c, StageArgs a) {
   return CG_OK;
 }
 
+// combine `rows` partial rows into c->ext (two deterministic passes)
+static int ext_combine(cg_ctx* c, long rows, bool keep_rho) {
+  const double n = (double)c->npts;
+  hipLaunchKernelGGL(k_ext_reduce, dim3(kExtMid), dim3(kExtThreads), 0, c->stream, c->ext_part, rows, c->ext_mid,
+                     (double*)nullptr, n, 0);
+  hipLaunchKernelGGL(k_ext_reduce, dim3(1), dim3(kExtThreads), 0, c->stream, c->ext_mid, (long)kExtMid,
+                     c->ext_mid + (long)kExtMid * EXT_W, c->ext, n, keep_rho ? 1 : 0);
+  HIPCHK(c, hipGetLastError());
+  return CG_OK;
+}
+
+// setExtraFieldData of _p alone (rho_avg kept), when no stage has produced it
+static int ext_from_state(cg_ctx* c) {
+  const int bs = 256;
+  const long nb = (c->npts + bs - 1) / bs;
+  const double* phi = (const double*)reg_ptr(c, PHI, c->sP);
+  if (c->prec == CG_FP64)
+    hipLaunchKernelGGL(k_ext_state<double>, dim3(nb), dim3(bs), 0, c->stream, phi,
+                       (const double*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi);
+  else
+    hipLaunchKernelGGL(k_ext_state<float>, dim3(nb), dim3(bs), 0, c->stream, phi,
+                       (const float*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi);
+  HIPCHK(c, hipGetLastError());
+  int rc = ext_combine(c, nb, true);
+  if (!rc) c->ext_fresh = true;
+  return rc;
+}
+
+// form the pending K4Finalize averages of the last fused step
+static int ext_resolve(cg_ctx* c) {
+  if (!c->ext_lazy) return CG_OK;
+  c->ext_lazy = false;
+  const int bs = 256;
+  const long nb = (c->npts + bs - 1) / bs;
+  const double* phi = (const double*)reg_ptr(c, PHI, c->sP);
+  const double* phi_in = (const double*)reg_ptr(c, PHI, c->ext_lazy_in);
+  if (c->prec == CG_FP64)
+    hipLaunchKernelGGL(k_ext_post<double>, dim3(nb), dim3(bs), 0, c->stream, phi,
+                       (const double*)reg_ptr(c, KK, c->sP), phi_in, c->npts, c->ext_part, c->ext_lazy_src,
+                       c->ext_lazy_phiF);
+  else
+    hipLaunchKernelGGL(k_ext_post<float>, dim3(nb), dim3(bs), 0, c->stream, phi,
+                       (const float*)reg_ptr(c, KK, c->sP), phi_in, c->npts, c->ext_part, c->ext_lazy_src,
+                       c->ext_lazy_phiF);
+  HIPCHK(c, hipGetLastError());
+  return ext_combine(c, nb, false);
+}
+
 template <typename TS>
 static int step_init_t(cg_ctx* c) {
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
   hipLaunchKernelGGL(k_step_init<TS>, dim3(nb), dim3(bs), 0, c->stream, reg_table(c), c->sP, c->sA, c->sF, c->npts,
-                     c->p.normalize_metric);
+                     c->p.normalize_metric, c->ext_part, c->frw.g_phi);
   HIPCHK(c, hipGetLastError());
-  return CG_OK;
+  int rc = ext_combine(c, nb, true);
+  if (!rc) c->ext_fresh = true;
+  return rc;
 }
 
 int cg_step_init(cg_ctx* c) {
   if (!c) return CG_ERR_ARG;
+  c->ext_lazy = false;  // superseded by this stepInit's averages
   if (c->a_is_p) { c->a_is_p = false; c->sA = 2; c->sC = 3; }
   return c->prec == CG_FP64 ? step_init_t<double>(c) : step_init_t<float>(c);
 }
@@ -572,22 +771,26 @@ int cg_rk_evolve(cg_ctx* c) {
 }
 
 template <typename TS>
-static int finalize_t(cg_ctx* c, int n) {
+static int finalize_t(cg_ctx* c, int n, const SrcArgs& src) {
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
-  hipLaunchKernelGGL(k_finalize<TS>, dim3(nb), dim3(bs), 0, c->stream, reg_table(c), c->sP, c->sC, c->sF, c->npts, n,
-                     c->p.dt);
+  hipLaunchKernelGGL(k_finalize<TS>, dim3(nb), dim3(bs), 0, c->stream, reg_table(c), c->sP, c->sC, c->sF, c->sA,
+                     c->npts, n, c->p.dt, c->ext_part, src, c->frw.g_phi);
   HIPCHK(c, hipGetLastError());
-  return CG_OK;
+  return ext_combine(c, nb, false);
 }
 
 int cg_k_finalize(cg_ctx* c, int n) {
   if (!c || n < 1 || n > 4) return CG_ERR_ARG;
   if (c->a_is_p) return fail(c, CG_ERR_STATE, "K-finalize before stepInit");
+  c->ext_lazy = false;
+  // the finished stage's sources were formed with the FRW values before this step
+  const SrcArgs src{c->DIFFD, c->p.matter, c->frw.g_phi, c->frw.g_rho, c->lambda};
   c->frw.stage(n, c->p.dt);  // bssn.cc:269: FRW stage first
-  int rc = c->prec == CG_FP64 ? finalize_t<double>(c, n) : finalize_t<float>(c, n);
+  int rc = c->prec == CG_FP64 ? finalize_t<double>(c, n, src) : finalize_t<float>(c, n, src);
   if (rc) return rc;
   std::swap(c->sA, c->sC);  // swap_a_c
+  c->ext_fresh = (n == 4);
   return CG_OK;
 }
 
@@ -598,27 +801,46 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
   const int X = 2, Y = 3;
   const bool ko_norm = c->p.ko_coeff != 0.0 && c->p.normalize_metric &&
                        (c->p.stencil_order == 2 || c->p.stencil_order == 8);
+  // one fused stage: sources and RHS with the current FRW values, then the
+  // FRW stage step (bssn.cc:269), whose phi weights the averages of the output
+  const bool flrw = c->p.lapse == CG_LAPSE_CONFORMAL_FLRW;
+  SrcArgs src4{};
+  auto stage = [&](int n, int rin, int rout, int normalize) -> int {
+    StageArgs a = make_args(c, n, rin, 0, 1, rout, normalize);
+    src4 = SrcArgs{c->DIFFD, c->p.matter, c->frw.g_phi, c->frw.g_rho, c->lambda};
+    c->frw.stage(n, c->p.dt);
+    a.phi_F_next = c->frw.g_phi;
+    int rc = launch(c, a);
+    return (rc || !flrw) ? rc : ext_combine(c, c->last_rows, false);
+  };
+  c->ext_lazy = false;
   for (int s = 0; s < nsteps; ++s) {
     int rc;
+    c->sP = 0; c->sF = 1;
     if (ko_norm) {
       // KO needs normalised A_ij at halo points: materialise stepInit first
       c->a_is_p = false; c->sA = X; c->sC = Y;
       if ((rc = cg_step_init(c))) return rc;
-      if ((rc = launch(c, make_args(c, 1, X, 0, 1, Y, 0)))) return rc;
-      c->frw.stage(1, c->p.dt);
-      if ((rc = launch(c, make_args(c, 2, Y, 0, 1, X, 0)))) return rc;
+      if ((rc = stage(1, X, Y, 0))) return rc;
+      if ((rc = stage(2, Y, X, 0))) return rc;
     } else {
-      if ((rc = launch(c, make_args(c, 1, 0, 0, 1, X, c->p.normalize_metric)))) return rc;
-      c->frw.stage(1, c->p.dt);
-      if ((rc = launch(c, make_args(c, 2, X, 0, 1, Y, 0)))) return rc;
+      // stepInit's setExtraFieldData equals the last K4Finalize's unless the
+      // state changed in between; only the ConformalFLRW lapse reads it
+      if (flrw && !c->ext_fresh && (rc = ext_from_state(c))) return rc;
+      if ((rc = stage(1, 0, X, c->p.normalize_metric))) return rc;
+      if ((rc = stage(2, X, Y, 0))) return rc;
     }
-    c->frw.stage(2, c->p.dt);
     const int in3 = ko_norm ? X : Y, out3 = ko_norm ? Y : X;
-    if ((rc = launch(c, make_args(c, 3, in3, 0, 1, out3, 0)))) return rc;
-    c->frw.stage(3, c->p.dt);
-    if ((rc = launch(c, make_args(c, 4, out3, 0, 1, 0, 0)))) return rc;
-    c->frw.stage(4, c->p.dt);
+    if ((rc = stage(3, in3, out3, 0))) return rc;
+    if ((rc = stage(4, out3, 0, 0))) return rc;
     c->sA = 0; c->sC = out3; c->a_is_p = true;
+    c->ext_fresh = flrw;
+    if (!flrw) {
+      c->ext_lazy = true;
+      c->ext_lazy_in = out3;
+      c->ext_lazy_src = src4;
+      c->ext_lazy_phiF = c->frw.g_phi;
+    }
   }
   unsigned nn = 0;
   HIPCHK(c, hipMemcpyAsync(&nn, c->nan_dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
@@ -627,6 +849,35 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
     c->nan_seen = nn;
     return fail(c, CG_ERR_NAN, "NAN detected in DIFFphi");
   }
+  return CG_OK;
+}
+
+int cg_extra_field_data(cg_ctx* c, double out[4]) {
+  if (!c || !out) return CG_ERR_ARG;
+  if (int rc = ext_resolve(c)) return rc;
+  HIPCHK(c, hipMemcpyAsync(out, c->ext, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CG_OK;
+}
+
+int cg_num_nans(cg_ctx* c, const char* name, long long* count) {
+  if (!c || !count) return CG_ERR_ARG;
+  int e;
+  size_t esz;
+  void* d = resolve(c, name, false, &e, &esz);
+  if (!d) return fail(c, CG_ERR_ARG, std::string("unknown field ") + (name ? name : ""));
+  HIPCHK(c, hipMemsetAsync(c->nan_cnt, 0, sizeof(unsigned long long), c->stream));
+  const int bs = 256;
+  const long nb = (c->npts + bs - 1) / bs;
+  if (esz == 8)
+    hipLaunchKernelGGL(k_num_nans<double>, dim3(nb), dim3(bs), 0, c->stream, (const double*)d, c->npts, c->nan_cnt);
+  else
+    hipLaunchKernelGGL(k_num_nans<float>, dim3(nb), dim3(bs), 0, c->stream, (const float*)d, c->npts, c->nan_cnt);
+  HIPCHK(c, hipGetLastError());
+  unsigned long long n = 0;
+  HIPCHK(c, hipMemcpyAsync(&n, c->nan_cnt, sizeof n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *count = (long long)n;
   return CG_OK;
 }
 
@@ -666,10 +917,10 @@ int cg_constraints(cg_ctx* c, double H[8], double M[8], double G[7], double A[7]
     const long nbk = (c->npts + bs - 1) / bs;
     if (c->prec == CG_FP64)
       hipLaunchKernelGGL(k_step_init<double>, dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin, c->sF,
-                         c->npts, c->p.normalize_metric);
+                         c->npts, c->p.normalize_metric, (double*)nullptr, 0.0);
     else
       hipLaunchKernelGGL(k_step_init<float>, dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin, c->sF,
-                         c->npts, c->p.normalize_metric);
+                         c->npts, c->p.normalize_metric, (double*)nullptr, 0.0);
     HIPCHK(c, hipGetLastError());
   }
   StageArgs a = make_args(c, 0, rin, c->sP, c->sF, c->sC, 0);
@@ -725,6 +976,7 @@ int cg_constraints(cg_ctx* c, double H[8], double M[8], double G[7], double A[7]
 int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
   if (!c || reps < 1) return CG_ERR_ARG;
   if (!c->a_is_p) return fail(c, CG_ERR_STATE, "autotune between method-level stages would clobber _a/_c");
+  if (int rc = ext_resolve(c)) return rc;
   // Times real stage-2-shaped launches (stencils of _p, writes into the dead
   // _f and spare registers), so the evolved state is untouched.
   std::string rep = "[";

@@ -111,6 +111,8 @@ def load_library():
         "cg_autotune": ([vp, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
         "cg_constraints": ([vp, dp, dp, dp, dp, dp], C.c_int),
         "cg_stage_time_ms": ([vp, dp, ip], C.c_int),
+        "cg_extra_field_data": ([vp, dp], C.c_int),
+        "cg_num_nans": ([vp, C.c_char_p, C.POINTER(C.c_longlong)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -206,6 +208,17 @@ class BSSN:
         G, A, S = (C.c_double * 7)(), (C.c_double * 7)(), (C.c_double * 7)()
         self._chk(self.L.cg_constraints(self.h, H, M, G, A, S))
         return {"H": list(H), "M": list(M), "G": list(G), "A": list(A), "S": list(S)}
+
+    def extra_field_data(self):
+        """BSSN::setExtraFieldData results: K_min, K_avg, avg_vol, rho_avg."""
+        out = (C.c_double * 4)()
+        self._chk(self.L.cg_extra_field_data(self.h, out))
+        return dict(zip(("K_min", "K_avg", "avg_vol", "rho_avg"), tuple(out)))
+
+    def numNaNs(self, name: str) -> int:
+        n = C.c_longlong()
+        self._chk(self.L.cg_num_nans(self.h, name.encode(), C.byref(n)))
+        return n.value
 
     def sync(self):
         self._chk(self.L.cg_sync(self.h))

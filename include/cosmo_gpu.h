@@ -97,6 +97,20 @@ int cg_rk4_steps(cg_ctx* ctx, int nsteps);
  * cg_field_download (prepBSSNOutput, sims/sim.cc:214-231). */
 int cg_constraints(cg_ctx* ctx, double H[8], double M[8], double G[7], double A[7], double S[7]);
 
+/* BSSN::setExtraFieldData results (bssn.cc:165-170; the public members
+ * K_min, K_avg, avg_vol, rho_avg of bssn.h:48): out = {K_min, K_avg,
+ * avg_vol, rho_avg} as of the last stepInit / K{n}Finalize, i.e. after a
+ * fused step those of its K4Finalize.  K_avg is the conformal average of
+ * DIFFK (the perturbation, math.h:1403-1415), which the ConformalFLRW lapse
+ * reads on the device every stage.  rho_avg averages the source DIFFr of the
+ * stage that produced the state; a stepInit with no stage since creation
+ * keeps the previous value (0 initially, like a zeroed DIFFr_a). */
+int cg_extra_field_data(cg_ctx* ctx, double out[4]);
+
+/* numNaNs (utils/math.h:1524-1541) of one field-map array: number of points
+ * whose float-rounded value is NaN (CosmoSim::simNumNaNs, sims/sim.cc). */
+int cg_num_nans(cg_ctx* ctx, const char* name, long long* count);
+
 /* Synchronise the context's stream. */
 int cg_sync(cg_ctx* ctx);
 

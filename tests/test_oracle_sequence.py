@@ -57,3 +57,46 @@ def test_homogeneous_static_dust_matches_rk4(oracle_mod, lam):
     assert np.ptp(s.field("DIFFK_p")) == 0
     for f_ in ("A11_p", "DIFFgamma12_p", "Gamma3_p", "DIFFalpha_p"):
         assert np.abs(s.field(f_)).max() < 1e-15
+
+
+def test_homogeneous_conformal_flrw_lapse_matches_rk4(oracle_mod):
+    """ConformalFLRW lapse (BSSNGaugeHandler.cc:24-27): d_t DIFFalpha =
+    -alpha K_avg / 3, with K_avg the conformal average of DIFFK over the
+    stage's _a (setExtraFieldData after the previous finalize, bssn.cc:165-170).
+    On uniform data K_avg equals DIFFK of the stage input, so the sequence is
+    classical RK4 of the 6-variable ODE; a K_avg one stage stale, or the full
+    K averaged instead of the perturbation, fails at O(dt^2) or O(1)."""
+    n = 8
+    rho0 = 3.0 / (8.0 * math.pi)
+    s = oracle_mod.Sim(oracle_mod.make_params(n, order=4, lapse="ConformalFLRW"))
+    dphi0, dK0, da0, D = 2e-3, -5e-3, 1e-3, 4e-3
+    s.field("DIFFphi_p")[:] = dphi0
+    s.field("DIFFK_p")[:] = dK0
+    s.field("DIFFalpha_p")[:] = da0
+    s.field("DIFFD_a")[:] = D
+    s.frw_set(0.0, -3.0)
+    s.frw_add_fluid(rho0, 0.0)
+    h = s.p.dt
+
+    def f(y):
+        phiF, KF, rhoF, dphi, dK, da = y
+        alpha = 1.0 + da
+        DIFFr = math.exp(-6 * (dphi + phiF)) * D + rhoF * math.expm1(-6 * dphi)
+        return np.array([-KF / 6, KF * KF / 3 + 4 * math.pi * rhoF, KF * rhoF,
+                         -(alpha * dK + da * KF) / 6,
+                         alpha * dK * (dK + 2 * KF) / 3 + 4 * math.pi * alpha * DIFFr + 4 * math.pi * da * rhoF,
+                         -alpha * dK / 3])
+
+    want = rk4_reference([0.0, -3.0, rho0, dphi0, dK0, da0], f, h, 8)
+    for step in range(8):
+        s.run_steps(1)
+        phiF, KF, rhoF, _ = s.frw_get()
+        got = [phiF, KF, rhoF, s.field("DIFFphi_p")[1, 2, 3], s.field("DIFFK_p")[4, 0, 7],
+               s.field("DIFFalpha_p")[5, 5, 1]]
+        np.testing.assert_allclose(got, want[step], rtol=1e-13, atol=1e-17)
+        # setExtraFieldData after K4Finalize: averages of the new state
+        ex = s.extra()
+        dK, dphi = want[step][4], want[step][3]
+        assert ex["K_min"] == s.field("DIFFK_p").min()
+        assert abs(ex["K_avg"] - dK) <= 1e-13 * abs(dK)
+        assert abs(ex["avg_vol"] - math.exp(6 * dphi)) <= 1e-13
