@@ -21,10 +21,14 @@ enum Lapse { LAPSE_STATIC = 0, LAPSE_HARMONIC = 1, LAPSE_CONFORMAL_FLRW = 2 };
 // resolved on the host; the DIFFphi registers are always FP64 (see
 // bssn_stage.hip), every other field uses the context's storage type.
 struct StageArgs {
-  const void* in[NF];     // stencil input (_a register of the stage)
-  const void* p[NF];      // _p register (step start)
-  void* f[NF];            // _f accumulator
-  void* out[NF];          // stage output: next _a, or _p at stage 4, or _c at stage 0
+  // Registers: field f, slot s lives at met + (f*4 + s)*rstride (f < 7,
+  // DIFFgamma_ij and DIFFphi, FP64) or regs + ((f-7)*4 + s)*rstride (storage
+  // type).  The slots of this stage: stencil input (_a), _p (step start), _f
+  // accumulator, output (next _a, or _p at stage 4, or _c at stage 0).
+  double* met;
+  void* regs;
+  long rstride;
+  int s_in, s_p, s_f, s_out;
   const void* DIFFD;      // static dust: conserved density perturbation
   int nx, ny, nz;
   int ntx, nty, ntz;      // tile counts per axis
@@ -48,6 +52,23 @@ struct StageArgs {
   unsigned int* nan_count;
   double* cons;           // constraint mode: 17 per-point FP64 arrays (see CONS_*)
   long cons_stride;
+};
+
+// FRW values one stage sees (utils/FRW.h getters), the split-precision
+// background part of H, and the FRW phi after the stage's FRW step
+struct FrwV {
+  double phi, K, rho, S, H, phi_next;
+};
+constexpr int FRW_N = 6;
+
+// One launch over the tiles of many grids (batched ensemble).
+struct BatchArgs {
+  const StageArgs* __restrict__ tab;  // [nreal] arguments of each grid for this stage
+  const int* __restrict__ tstart;     // [nreal] first global tile of each grid
+  const double* __restrict__ frw;     // [nreal][FRW_N] FRW values of this stage
+  int nreal;
+  int total;                          // tiles of all grids
+  int per_xcd;                        // tiles per XCD (XCD-aware block remap)
 };
 
 // per-point outputs of the constraint mode (BSSN::setConstraintCalcs,
@@ -108,6 +129,7 @@ struct StageVariant {
   bool live_alpha;        // false: lapse frozen at 1 (synchronous gauge)
   int (*launch)(const StageArgs&, hipStream_t);
   int (*info)(int* vgprs, int* scratch, int* lds);
+  int (*launch_batch)(const BatchArgs&, hipStream_t);  // null: no batched kernel compiled
 };
 
 std::vector<StageVariant>& stage_variants();

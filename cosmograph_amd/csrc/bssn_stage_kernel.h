@@ -89,9 +89,12 @@ __device__ __host__ constexpr int stencil_seq(int s) {
 // MODE 0: RK4 stage.  MODE 1: constraint diagnostics (TC = double), which
 // also differentiates A_ij for the momentum constraint and writes per-point
 // values instead of the RK4 epilogue.
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LIVE_ALPHA, int MODE = 0>
-__global__ void __launch_bounds__(TI* TJ* TK) __attribute__((amdgpu_waves_per_eu(WPE)))
-stage_kernel(const StageArgs a) {
+//
+// stage_body is one tile t (< a.ntx*a.nty*a.ntz) of one grid; fv holds the
+// FRW values of the stage.  Every thread of the block must call it (it holds
+// barriers).
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, bool LIVE_ALPHA, int MODE>
+__device__ __forceinline__ void stage_body(const StageArgs& a, const int t, const FrwV fv) {
   constexpr int NT = TI * TJ * TK;
   constexpr int HJ = TJ + 2 * H, HK = TK + 2 * H, HI = TI + 2 * H;
   constexpr int HV = HI * HJ * HK;
@@ -100,10 +103,6 @@ stage_kernel(const StageArgs a) {
   __shared__ double ldsraw[2][HV];  // double-buffered field tile (FP32 fields use the first half)
   __shared__ double ldn[HV];        // stage-1 unit-determinant factor t at every halo point
 
-  const int ntiles = a.ntx * a.nty * a.ntz;
-  const int b = blockIdx.x;
-  const int t = (b & 7) * a.per_xcd + (b >> 3);  // consecutive tiles stay on one XCD
-  if (t >= ntiles) return;
   const int tz = t % a.ntz, ty = (t / a.ntz) % a.nty, tx = t / (a.ntz * a.nty);
   const int i0 = tx * TI, j0 = ty * TJ, k0 = tz * TK;
 
@@ -130,8 +129,10 @@ stage_kernel(const StageArgs a) {
     goff[q] = (unsigned)((x * a.ny + y) * a.nz + z);
   }
 
-  auto INM = [&](int f) -> const double* { return (const double*)a.in[f]; };
-  auto INS = [&](int f) -> const TS* { return (const TS*)a.in[f]; };
+  auto RM = [&](int f, int slot) -> double* { return a.met + ((long)f * 4 + slot) * a.rstride; };
+  auto RS = [&](int f, int slot) -> TS* { return (TS*)a.regs + ((long)(f - 7) * 4 + slot) * a.rstride; };
+  auto INM = [&](int f) -> const double* { return RM(f, a.s_in); };
+  auto INS = [&](int f) -> const TS* { return RS(f, a.s_in); };
   const bool norm = a.normalize != 0;
 
   // stage-1 unit-determinant factor at the halo points (bssn.cc:75-99):
@@ -316,17 +317,17 @@ stage_kernel(const StageArgs a) {
   double phi_new = 0.0, K_new = 0.0, rho_new = 0.0;  // stage output and source seen by the reduction
   if (valid) {
     // ---------------------------------------------------------- algebra
-    const TC phiF = (TC)a.phi_F, KF = (TC)a.K_F, rhoF = (TC)a.rho_F, SF = (TC)a.S_F;
+    const TC phiF = (TC)fv.phi, KF = (TC)fv.K, rhoF = (TC)fv.rho, SF = (TC)fv.S;
     const TC PI = TC(3.14159265358979323846);
     const TC alpha = v[ALPHA] + TC(1);
-    const TC phi = (TC)(phi_c + a.phi_F);
+    const TC phi = (TC)(phi_c + fv.phi);
     const TC K = v[KK] + KF;
 
     // sources (components/static/static.cc:28-35, components/Lambda/lambda.cc:32-41)
     TC DIFFr = TC(0), DIFFS = TC(0);
     if (a.matter == MATTER_STATIC) {
       const TC D = (TC)ldo((const TS*)a.DIFFD, idx);
-      DIFFr = exp((TC)(-6.0 * (phi_c + a.phi_F))) * D + rhoF * expm1(TC(-6) * v[PHI]);
+      DIFFr = exp((TC)(-6.0 * (phi_c + fv.phi))) * D + rhoF * expm1(TC(-6) * v[PHI]);
     }
     DIFFr += (TC)a.lambda;
     DIFFS += TC(-3) * (TC)a.lambda;
@@ -459,7 +460,7 @@ stage_kernel(const StageArgs a) {
       } else {
         // split precision: (2/3)K^2 - 16 pi rho in difference form around FRW
         const TC dK2 = TC(2.0 / 3.0) * (TC(2) * KF + v[KK]) * v[KK];
-        Hc = -exp(TC(5) * phi) / TC(8) * (ricci + ((TC)a.H_frw + dK2) - AijAij - TC(16) * PI * DIFFr);
+        Hc = -exp(TC(5) * phi) / TC(8) * (ricci + ((TC)fv.H + dK2) - AijAij - TC(16) * PI * DIFFr);
       }
     }
 
@@ -468,7 +469,7 @@ stage_kernel(const StageArgs a) {
       double* __restrict__ co = a.cons;
       const long cs = a.cons_stride;
       auto put = [&](int q, double val) { co[(long)q * cs + idx] = val; };
-      const double rr = (double)DIFFr + a.rho_F;
+      const double rr = (double)DIFFr + fv.rho;
       const double e5 = exp(5.0 * (double)phi), e6 = exp(6.0 * (double)phi);
       // hamiltonianConstraintScale (bssn.cc:1086-1100)
       const double Hs = (e5 / 8.0) * sqrt((double)ricci * ricci + (double)AijAij * AijAij +
@@ -589,13 +590,13 @@ stage_kernel(const StageArgs a) {
       if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
       const double cc = (double)rhs[f];
       if (metric_field(f)) {
-        double* __restrict__ o = (double*)a.out[f];
+        double* __restrict__ o = RM(f, a.s_out);
         if (st == 0) {
           sto(o, idx, cc);
           continue;
         }
-        const double p = ldo((const double*)a.p[f], idx);
-        double* __restrict__ fr = (double*)a.f[f];
+        const double p = ldo((const double*)RM(f, a.s_p), idx);
+        double* __restrict__ fr = RM(f, a.s_f);
         const double fo = st >= 2 ? ldo((const double*)fr, idx) : 0.0;
         const double inc = a.cf * cc;
         const double nv = st == 4 ? fo + (inc + p) : p + a.ca * cc;
@@ -607,13 +608,13 @@ stage_kernel(const StageArgs a) {
           sto(fr, idx, fo + inc);
         }
       } else {
-        TS* __restrict__ o = (TS*)a.out[f];
+        TS* __restrict__ o = RS(f, a.s_out);
         if (st == 0) {
           sto(o, idx, (TS)cc);
           continue;
         }
-        const double p = (double)ldo((const TS*)a.p[f], idx);
-        TS* __restrict__ fr = (TS*)a.f[f];
+        const double p = (double)ldo((const TS*)RS(f, a.s_p), idx);
+        TS* __restrict__ fr = RS(f, a.s_f);
         const double fo = st >= 2 ? (double)ldo((const TS*)fr, idx) : 0.0;
         const double inc = a.cf * cc;
         const TS nv = st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc);
@@ -630,14 +631,52 @@ stage_kernel(const StageArgs a) {
   // live-lapse variants carry the reduction (the host computes the values
   // of the last stage on demand otherwise).
   if constexpr (LIVE_ALPHA) {
-    if (a.ext_part && a.stage >= 1) block_ext_put(a.ext_part, t, valid, phi_new, K_new, rho_new, a.phi_F_next);
+    if (a.ext_part && a.stage >= 1) block_ext_put(a.ext_part, t, valid, phi_new, K_new, rho_new, fv.phi_next);
   }
+}
+
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LIVE_ALPHA, int MODE = 0>
+__global__ void __launch_bounds__(TI* TJ* TK) __attribute__((amdgpu_waves_per_eu(WPE)))
+stage_kernel(const StageArgs a) {
+  const int b = blockIdx.x;
+  const int t = (b & 7) * a.per_xcd + (b >> 3);  // consecutive tiles stay on one XCD
+  if (t >= a.ntx * a.nty * a.ntz) return;
+  stage_body<TS, TC, R, H, TI, TJ, TK, LIVE_ALPHA, MODE>(
+      a, t, FrwV{a.phi_F, a.K_F, a.rho_F, a.S_F, a.H_frw, a.phi_F_next});
+}
+
+// Batched ensemble: one launch over the tiles of many grids.  Tiles are
+// numbered grid after grid (ba.tstart[r] = first tile of grid r) and mapped
+// to blocks with the same XCD-aware remap as a single grid, so each XCD's L2
+// sees neighbouring tiles of one grid.  (A persistent loop over tiles would
+// let the compiler hoist per-thread tile geometry out of the loop and spill.)
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LIVE_ALPHA>
+__global__ void __launch_bounds__(TI* TJ* TK) __attribute__((amdgpu_waves_per_eu(WPE)))
+stage_kernel_batch(const BatchArgs ba) {
+  const int b = blockIdx.x;
+  const int g = (b & 7) * ba.per_xcd + (b >> 3);
+  if (g >= ba.total) return;
+  int r = 0, n = ba.nreal;  // last grid whose first tile is <= g
+  while (n > 1) {
+    const int h = n >> 1;
+    if (ba.tstart[r + h] <= g) { r += h; n -= h; } else { n = h; }
+  }
+  const double* f = ba.frw + (long)r * FRW_N;
+  stage_body<TS, TC, R, H, TI, TJ, TK, LIVE_ALPHA, 0>(ba.tab[r], g - ba.tstart[r],
+                                                      FrwV{f[0], f[1], f[2], f[3], f[4], f[5]});
 }
 
 // ------------------------------------------------------------ variants
 template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA>
 int stage_launch_t(const StageArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((stage_kernel<TS, TC, R, H, TI, TJ, TK, WPE, LA>), dim3(8 * a.per_xcd), dim3(TI * TJ * TK), 0, s, a);
+  return (int)hipGetLastError();
+}
+
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA>
+int stage_launch_batch_t(const BatchArgs& ba, hipStream_t s) {
+  hipLaunchKernelGGL((stage_kernel_batch<TS, TC, R, H, TI, TJ, TK, WPE, LA>), dim3(8 * ba.per_xcd), dim3(TI * TJ * TK), 0,
+                     s, ba);
   return (int)hipGetLastError();
 }
 
@@ -658,10 +697,18 @@ int cons_launch_t(const StageArgs& a, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
-#define CG_REGISTER_STAGE(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA)                                            \
+#define CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, BATCH)                                   \
   static StageVariantReg CG_CAT(reg_, __COUNTER__)(StageVariant{PREC, 2 * R, H, TI, TJ, TK, WPE, LA,          \
                                                                 &stage_launch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>, \
-                                                                &stage_info_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>});
+                                                                &stage_info_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>,   \
+                                                                BATCH});
+// single-grid kernel only
+#define CG_REGISTER_STAGE(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA) \
+  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, nullptr)
+// single-grid and batched-ensemble kernels
+#define CG_REGISTER_STAGE_B(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA) \
+  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA,     \
+                      (&stage_launch_batch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>))
 #define CG_CAT2(a, b) a##b
 #define CG_CAT(a, b) CG_CAT2(a, b)
 

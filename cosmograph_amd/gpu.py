@@ -112,6 +112,17 @@ def load_library():
         "cg_constraints": ([vp, dp, dp, dp, dp, dp], C.c_int),
         "cg_stage_time_ms": ([vp, dp, ip], C.c_int),
         "cg_extra_field_data": ([vp, dp], C.c_int),
+        "cg_batch_create": ([C.POINTER(_Params), C.c_int, C.POINTER(vp)], C.c_int),
+        "cg_batch_destroy": ([vp], C.c_int),
+        "cg_batch_last_error": ([vp], C.c_char_p),
+        "cg_batch_size": ([vp], C.c_int),
+        "cg_batch_member": ([vp, C.c_int, C.POINTER(vp)], C.c_int),
+        "cg_batch_step": ([vp, C.c_int], C.c_int),
+        "cg_batch_set_graph": ([vp, C.c_int], C.c_int),
+        "cg_batch_step_time_ms": ([vp, dp], C.c_int),
+        "cg_batch_constraints": ([vp, dp, dp, dp, dp, dp], C.c_int),
+        "cg_batch_autotune": ([vp, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
+        "cg_batch_kernel_info": ([vp, ip, ip, ip, ip, ip], C.c_int),
         "cg_num_nans": ([vp, C.c_char_p, C.POINTER(C.c_longlong)], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -128,11 +139,15 @@ def _dptr(a: np.ndarray):
 class BSSN:
     """GPU-resident BSSN state with the reference's method names."""
 
-    def __init__(self, params: GpuParams):
+    def __init__(self, params: GpuParams, _borrowed=None):
         self.L = load_library()
         self.params = params
         self.shape = params.shape()
         self.npts = int(np.prod(self.shape))
+        self._owner = None
+        if _borrowed is not None:  # a member of a Batch: the batch owns the handle
+            self._owner, self.h = _borrowed
+            return
         h = C.c_void_p()
         rc = self.L.cg_create(C.byref(params.to_c()), C.byref(h))
         self.h = h
@@ -144,9 +159,9 @@ class BSSN:
             raise CosmoGpuError(rc, msg)
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "_owner", None) is None:
             self.L.cg_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def _chk(self, rc):
         if rc:
@@ -244,3 +259,74 @@ class BSSN:
         avg, n = C.c_double(), C.c_int()
         self._chk(self.L.cg_stage_time_ms(self.h, C.byref(avg), C.byref(n)))
         return avg.value, n.value
+
+
+class Batch:
+    """Batched ensemble (cg_batch_*): independent grids advanced by one
+    stage-kernel launch per RK4 stage, replayed from a HIP graph.
+    ``members[r]`` is a ``BSSN`` view of realisation r (uploads, downloads,
+    FRW set-up, constraints...)."""
+
+    def __init__(self, params_list):
+        self.L = load_library()
+        n = len(params_list)
+        arr = (_Params * n)(*[p.to_c() for p in params_list])
+        h = C.c_void_p()
+        rc = self.L.cg_batch_create(arr, n, C.byref(h))
+        self.h = h
+        if rc:
+            msg = self.L.cg_batch_last_error(h).decode() if h.value else "invalid parameters"
+            if h.value:
+                self.L.cg_batch_destroy(h)
+            self.h = None
+            raise CosmoGpuError(rc, msg)
+        self.members = []
+        for r, p in enumerate(params_list):
+            mh = C.c_void_p()
+            self._chk(self.L.cg_batch_member(self.h, r, C.byref(mh)))
+            self.members.append(BSSN(p, _borrowed=(self, mh)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            for m in getattr(self, "members", []):
+                m.h = None
+            self.L.cg_batch_destroy(self.h)
+            self.h = None
+
+    def __len__(self):
+        return len(self.members)
+
+    def _chk(self, rc):
+        if rc:
+            raise CosmoGpuError(rc, self.L.cg_batch_last_error(self.h).decode())
+
+    def step(self, nsteps: int = 1):
+        self._chk(self.L.cg_batch_step(self.h, nsteps))
+
+    def set_graph(self, on: bool):
+        self._chk(self.L.cg_batch_set_graph(self.h, int(on)))
+
+    def step_time_ms(self) -> float:
+        ms = C.c_double()
+        self._chk(self.L.cg_batch_step_time_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def setConstraintCalcs(self):
+        """setConstraintCalcs of every realisation: dict of [n, 8|7] arrays."""
+        n = len(self.members)
+        H, M = np.zeros((n, 8)), np.zeros((n, 8))
+        G, A, S = np.zeros((n, 7)), np.zeros((n, 7)), np.zeros((n, 7))
+        self._chk(self.L.cg_batch_constraints(self.h, _dptr(H), _dptr(M), _dptr(G), _dptr(A), _dptr(S)))
+        return {"H": H, "M": M, "G": G, "A": A, "S": S}
+
+    def autotune(self, reps: int = 3):
+        import json
+        buf = C.create_string_buffer(16384)
+        self._chk(self.L.cg_batch_autotune(self.h, reps, buf, len(buf)))
+        return json.loads(buf.value.decode() or "[]")
+
+    def kernel_info(self):
+        v = [C.c_int() for _ in range(5)]
+        self._chk(self.L.cg_batch_kernel_info(self.h, *[C.byref(x) for x in v]))
+        vg, sc, ti, tj, tk = (x.value for x in v)
+        return {"vgprs": vg, "scratch_bytes": sc, "tile": (ti, tj, tk)}

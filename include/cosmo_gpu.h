@@ -129,6 +129,47 @@ int cg_autotune(cg_ctx* ctx, int reps, char* report, size_t report_len);
 int cg_set_timing(cg_ctx* ctx, int on);
 int cg_stage_time_ms(cg_ctx* ctx, double* avg_ms, int* count);
 
+/* ------------------------------------------------------------------------
+ * Batched ensemble.  The reference runs one realisation per process
+ * (sims/sim.cc:101-131); a batch advances many independent grids together:
+ * one stage-kernel launch per RK4 stage covers the tiles of every grid, and
+ * the launches of a call are captured in a HIP graph and replayed.  Every
+ * realisation is an ordinary context (cg_batch_member) on which the
+ * single-grid entry points above work: field uploads/downloads, FRW set-up,
+ * constraints, extra field data, numNaNs.  Member handles are owned by the
+ * batch (never cg_destroy them).  All realisations share precision, stencil
+ * order, device and whether KO damping is on; grid sizes, dx, dt, lambda,
+ * FRW backgrounds and damping amplitudes may differ.  The ConformalFLRW
+ * lapse (a per-grid reduction every stage) and KO at orders 2/8 with metric
+ * normalisation are single-grid only (CG_ERR_ARG here).
+ * ---------------------------------------------------------------------- */
+typedef struct cg_batch cg_batch;
+
+/* n realisations with params[0..n-1]; on failure *out may still hold a batch
+ * for cg_batch_last_error, to be released with cg_batch_destroy. */
+int cg_batch_create(const cg_params* params, int n, cg_batch** out);
+int cg_batch_destroy(cg_batch* batch);
+const char* cg_batch_last_error(const cg_batch* batch);
+int cg_batch_size(const cg_batch* batch);
+int cg_batch_member(cg_batch* batch, int r, cg_ctx** out);
+
+/* nsteps whole RK4 steps of every realisation (each as cg_rk4_steps).
+ * Returns CG_ERR_NAN if any realisation's DIFFphi turned NaN; the message
+ * lists them.  Work queued on member streams is waited for first. */
+int cg_batch_step(cg_batch* batch, int nsteps);
+/* Capture the launches of a call in a HIP graph and replay it (default 1),
+ * or launch them one by one (0). */
+int cg_batch_set_graph(cg_batch* batch, int on);
+/* GPU time of the last cg_batch_step call (HIP events around the launches). */
+int cg_batch_step_time_ms(const cg_batch* batch, double* ms);
+/* cg_constraints of every realisation: H, M [n][8]; G, A, S [n][7]. */
+int cg_batch_constraints(cg_batch* batch, double* H, double* M, double* G, double* A, double* S);
+/* Times every compiled batched variant that fits all grids and keeps the
+ * fastest (state untouched; between whole steps). */
+int cg_batch_autotune(cg_batch* batch, int reps, char* report, size_t report_len);
+int cg_batch_kernel_info(const cg_batch* batch, int* vgprs, int* scratch_bytes, int* tile_i, int* tile_j,
+                         int* tile_k);
+
 #ifdef __cplusplus
 }
 #endif
