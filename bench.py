@@ -106,6 +106,47 @@ def copy_probe(gib):
     print(json.dumps({"copy_probe_bytes": n}))
 
 
+ENSEMBLE = [(48, 48), (24, 96), (8, 128)]  # (realisations, grid side)
+
+
+def ensemble_leg(cg, steps, warmup, log):
+    """Batched ensemble: 48 x 48^3 + 24 x 96^3 + 8 x 128^3 = 80 realisations
+    (43,319,296 points per step), BSSN + static dust, order 4, FP32 split,
+    P(k)~k^-2 with rms drawn uniformly in [1e-4, 1e-2], seeds 1..80; one
+    batched launch per RK4 stage, replayed from a HIP graph."""
+    rng = np.random.default_rng(80)
+    params, spec = [], []
+    seed = 0
+    for count, n in ENSEMBLE:
+        for _ in range(count):
+            seed += 1
+            spec.append((n, float(rng.uniform(1e-4, 1e-2)), seed))
+            params.append(cg.GpuParams(n=n, order=4, precision="fp32_split", matter="static"))
+    log(f"[bench] ensemble: {len(params)} realisations, ICs")
+    b = cg.Batch(params)
+    for m, (n, rms, sd) in zip(b.members, spec):
+        fields, frw = cg.static_dust_ics(n, -2.0, rms, sd)
+        for name, arr in fields.items():
+            m.upload(name, arr)
+        m.frw_set(frw["phi"], frw["K"])
+        for rho, w in frw["fluids"]:
+            m.frw_add_fluid(rho, w)
+    tune = b.autotune(3)
+    b.step(warmup)
+    t0 = time.perf_counter()
+    b.step(steps)
+    wall = time.perf_counter() - t0
+    gpu_ms = b.step_time_ms()
+    pts = sum(n ** 3 for n, _, _ in spec)
+    nan = sum(m.numNaNs("DIFFphi_p") for m in b.members)
+    out = {"workload": "batched ensemble: 48x48^3 + 24x96^3 + 8x128^3, order 4, FP32 split",
+           "realisations": len(spec), "points_per_step": pts, "steps": steps,
+           "value": pts * steps / wall, "unit": "point-updates/s", "ms_per_step": 1e3 * wall / steps,
+           "gpu_ms_per_step": gpu_ms / steps, "kernel": b.kernel_info(), "autotune": tune, "nan_points": nan}
+    del b
+    return out
+
+
 def load_traffic(workload):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(path):
@@ -127,6 +168,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=128, help="grid side of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=100, help="batched-ensemble leg (0 = skip)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -179,6 +221,12 @@ def main():
     achieved = bytes_per_launch / (k32_ms * 1e-3) / 1e9
     traffic = load_traffic(f"bssn_static_dust_{n}_o{order}_fp32")
 
+    del g32
+    ens = None
+    if args.ensemble_steps > 0:
+        ens = ensemble_leg(cg, args.ensemble_steps, args.warmup, log)
+        log(f"[bench] ensemble: {ens['value'] / 1e9:.3f} G point-updates/s")
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"[bench] CPU baseline sample {args.cpu_sample}^3")
@@ -210,6 +258,7 @@ def main():
         "fp32_autotune": tune32,
     }
     line.update(res)
+    line["ensemble"] = ens
     print(json.dumps(line))
 
 
