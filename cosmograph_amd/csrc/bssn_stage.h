@@ -13,6 +13,10 @@ enum Field {
   G11 = 0, G12, G13, G22, G23, G33, PHI, A11, A12, A13, A22, A23, A33, KK, GAM1, GAM2, GAM3, ALPHA
 };
 enum Matter { MATTER_NONE = 0, MATTER_STATIC = 1, MATTER_DUST = 2 };
+constexpr int NDUST = 4;     // D, S1, S2, S3
+constexpr int NFLUX = 12;    // alpha D v^i (3), alpha S_j v^i (9)
+// stage-kernel modes: bit 0 constraint diagnostics, bit 1 dust fluid
+enum { MODE_STAGE = 0, MODE_CONS = 1, MODE_DUST = 2, MODE_DUST_CONS = 3 };
 enum Lapse { LAPSE_STATIC = 0, LAPSE_HARMONIC = 1, LAPSE_CONFORMAL_FLRW = 2 };
 
 // One fused RK4 stage over the whole grid: stencils of the `in` register,
@@ -30,6 +34,17 @@ struct StageArgs {
   long rstride;
   int s_in, s_p, s_f, s_out;
   const void* DIFFD;      // static dust: conserved density perturbation
+  // dust fluid (components/dust_fluid/dust.h:32-46): registers of D, S1..S3
+  // at dreg + (q*4 + slot)*rstride (storage type); the 12 fluxes
+  // alpha D v^i, alpha S_j v^i of the stencil input at flux + q*rstride;
+  // the matter sources use W, detg, g_ij of the previous stage's input
+  // (the reference's lag, dust.cc:226-264) read at slot s_lag
+  void* dreg;
+  const void* flux;
+  int s_lag;
+  int lag_mode;           // 0: lag = stencil input; 1: slot s_lag; 2: slot s_lag, metric normalised
+  double phi_F_lag;       // FRW phi the lagged state was seen with
+  double* rho_out;        // dust stage: DIFFr per point (null: not kept)
   int nx, ny, nz;
   int ntx, nty, ntz;      // tile counts per axis
   int per_xcd;            // tiles handled per XCD (XCD-aware block remap)
@@ -76,7 +91,7 @@ struct BatchArgs {
 // ricci_a and AijAij_a arrays of prepBSSNOutput (sims/sim.cc:214-231)
 enum { CONS_H = 0, CONS_M = 3, CONS_G = 6, CONS_A = 9, CONS_S = 12, CONS_RICCI = 15, CONS_AIJAIJ = 16,
        CONS_N = 17 };
-int launch_constraints(int prec, int order, bool live_alpha, const StageArgs& a, hipStream_t s);
+int launch_constraints(int prec, int order, bool live_alpha, bool dust, const StageArgs& a, hipStream_t s);
 
 // BSSN::setExtraFieldData (bssn.cc:165-170) partial sums, one row per
 // workgroup: sum w*DIFFK, sum w, sum w*DIFFphi, sum w*DIFFr, min DIFFK with
@@ -127,6 +142,7 @@ struct StageVariant {
   int ti, tj, tk;         // tile extents (i slowest, k fastest)
   int wpe;                // waves per SIMD the register budget allows
   bool live_alpha;        // false: lapse frozen at 1 (synchronous gauge)
+  bool dust;              // evolves the dust fluid with the metric
   int (*launch)(const StageArgs&, hipStream_t);
   int (*info)(int* vgprs, int* scratch, int* lds);
   int (*launch_batch)(const BatchArgs&, hipStream_t);  // null: no batched kernel compiled

@@ -66,9 +66,13 @@ __device__ __forceinline__ bool nan_as_float(T v) {
 // so FP32 storage would put its rounding noise straight into A_ij.
 __device__ __host__ constexpr bool metric_field(int f) { return f <= G33 || f == PHI; }
 
+// codes of the dust fluxes in the staging sequence (beyond the field codes)
+constexpr int FLUX0 = 32;
+
 // order in which fields are staged through LDS:
-// PHI, G11..G33, [ALPHA], KK, GAM1..GAM3, A11..A33 (the A_ij only for KO)
-template <bool LA>
+// PHI, G11..G33, [ALPHA], KK, GAM1..GAM3, [12 dust fluxes], A11..A33 (the
+// A_ij only for KO and the constraints)
+template <bool LA, bool DU>
 __device__ __host__ constexpr int stencil_seq(int s) {
   if (s == 0) return PHI;
   if (s <= 6) return G11 + (s - 1);
@@ -78,6 +82,10 @@ __device__ __host__ constexpr int stencil_seq(int s) {
   }
   if (s == 7) return KK;
   if (s <= 10) return GAM1 + (s - 8);
+  if (DU) {
+    if (s <= 10 + NFLUX) return FLUX0 + (s - 11);
+    s -= NFLUX;
+  }
   return A11 + (s - 11 < 5 ? s - 11 : 5);
 }
 
@@ -86,9 +94,10 @@ __device__ __host__ constexpr int stencil_seq(int s) {
 // lapse, zero initial lapse perturbation).  Every lapse derivative is then an
 // exact zero, so the lapse stencil, D_iD_j alpha and the lapse registers are
 // skipped; the result is bitwise the same as differentiating zeros.
-// MODE 0: RK4 stage.  MODE 1: constraint diagnostics (TC = double), which
-// also differentiates A_ij for the momentum constraint and writes per-point
-// values instead of the RK4 epilogue.
+// MODE bit 0: constraint diagnostics (TC = double), which also
+// differentiates A_ij for the momentum constraint and writes per-point
+// values instead of the RK4 epilogue.  MODE bit 1: the dust fluid of
+// components/dust_fluid/dust.cc is evolved with the metric and sources it.
 //
 // stage_body is one tile t (< a.ntx*a.nty*a.ntz) of one grid; fv holds the
 // FRW values of the stage.  Every thread of the block must call it (it holds
@@ -132,7 +141,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   auto RM = [&](int f, int slot) -> double* { return a.met + ((long)f * 4 + slot) * a.rstride; };
   auto RS = [&](int f, int slot) -> TS* { return (TS*)a.regs + ((long)(f - 7) * 4 + slot) * a.rstride; };
   auto INM = [&](int f) -> const double* { return RM(f, a.s_in); };
-  auto INS = [&](int f) -> const TS* { return RS(f, a.s_in); };
+  auto INS = [&](int f) -> const TS* {
+    return f >= FLUX0 ? (const TS*)a.flux + (long)(f - FLUX0) * a.rstride : RS(f, a.s_in);
+  };
+  auto RD = [&](int q, int slot) -> TS* { return (TS*)a.dreg + ((long)q * 4 + slot) * a.rstride; };
+  constexpr bool CONS = (MODE & MODE_CONS) != 0;
+  constexpr bool DUST = (MODE & MODE_DUST) != 0;
   const bool norm = a.normalize != 0;
 
   // stage-1 unit-determinant factor at the halo points (bssn.cc:75-99):
@@ -243,15 +257,20 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   TC T1[6];         // gamma^{kl} d_k d_l gamma_ij
   TC dphi[3], ddphi[6], dal[3], ddal[6], dK[3], dG[3][3];
   TC dA[3][6];      // d_j A_kl, constraint mode only
+  TC ddust[NDUST];  // sum_i d_i of the dust fluxes
+#pragma unroll
+  for (int q = 0; q < NDUST; ++q) ddust[q] = TC(0);
   TC ko[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) ko[f] = TC(0);
   const bool use_ko = a.use_ko != 0;
   constexpr int KO_ORDER = (H > R) ? 2 * R : 0;
 
-  // stencil fields in order; A_ij only when KO is active
-  constexpr int NSEQ = LIVE_ALPHA ? 18 : 17;
-  const int nseq = (use_ko || MODE == 1) ? NSEQ : NSEQ - 6;
+  // stencil fields in order; A_ij only when KO is active or for the
+  // constraints; the dust fluxes only in the dust stage
+  constexpr bool FLUXES = DUST && !CONS;
+  constexpr int NSEQ = (LIVE_ALPHA ? 18 : 17) + (FLUXES ? NFLUX : 0);
+  const int nseq = (use_ko || CONS) ? NSEQ : NSEQ - 6;
   if constexpr (!LIVE_ALPHA) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) dal[d] = TC(0);
@@ -267,9 +286,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   for (int s = 0; s < NSEQ; ++s) {
     if (s < nseq) {
       const bool more = (s + 1) < nseq;
-      const int fn = stencil_seq<LIVE_ALPHA>((s + 1) < NSEQ ? s + 1 : 0);
+      const int fn = stencil_seq<LIVE_ALPHA, FLUXES>((s + 1) < NSEQ ? s + 1 : 0);
       if (more) fetch(fn);
-      const int f = stencil_seq<LIVE_ALPHA>(s);
+      const int f = stencil_seq<LIVE_ALPHA, FLUXES>(s);
       if (valid) {
         if (metric_field(f)) {
           const double* __restrict__ L = ldsraw[s & 1];
@@ -289,7 +308,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
           }
         } else {
           const TC* __restrict__ L = reinterpret_cast<const TC*>(ldsraw[s & 1]);
-          if (KO_ORDER && use_ko) ko[f] = (TC)a.ko_pref * ko_sum<KO_ORDER, TC>(L, c, SI, SJ, SK);
+          if (KO_ORDER && use_ko && f < NF) ko[f] = (TC)a.ko_pref * ko_sum<KO_ORDER, TC>(L, c, SI, SJ, SK);
           if (f == ALPHA) {
             const TC idx2 = (TC)a.inv_dx2;
             dal[0] = fd1<R>(L, c, SI) * idx1; dal[1] = fd1<R>(L, c, SJ) * idx1; dal[2] = fd1<R>(L, c, SK) * idx1;
@@ -302,9 +321,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
             const int kk = f - GAM1;
             dG[0][kk] = fd1<R>(L, c, SI) * idx1; dG[1][kk] = fd1<R>(L, c, SJ) * idx1;
             dG[2][kk] = fd1<R>(L, c, SK) * idx1;
-          } else if (MODE == 1 && f >= A11 && f <= A33) {
+          } else if (CONS && f >= A11 && f <= A33) {
             const int q = f - A11;
             dA[0][q] = fd1<R>(L, c, SI) * idx1; dA[1][q] = fd1<R>(L, c, SJ) * idx1; dA[2][q] = fd1<R>(L, c, SK) * idx1;
+          } else if (FLUXES && f >= FLUX0) {
+            // dust.cc:203-224: flux q*3 + i is differentiated along i only
+            const int q = (f - FLUX0) / 3, dir = (f - FLUX0) % 3;
+            ddust[q] += fd1<R>(L, c, dir == 0 ? SI : dir == 1 ? SJ : SK) * idx1;
           }
         }
       }
@@ -325,12 +348,72 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
 
     // sources (components/static/static.cc:28-35, components/Lambda/lambda.cc:32-41)
     TC DIFFr = TC(0), DIFFS = TC(0);
+    TC Si[3] = {TC(0), TC(0), TC(0)}, STF[6] = {TC(0), TC(0), TC(0), TC(0), TC(0), TC(0)};
+    TC Dc = TC(0), Sc[3] = {TC(0), TC(0), TC(0)}, Wc = TC(1), vc[3] = {TC(0), TC(0), TC(0)};
     if (a.matter == MATTER_STATIC) {
       const TC D = (TC)ldo((const TS*)a.DIFFD, idx);
       DIFFr = exp((TC)(-6.0 * (phi_c + fv.phi))) * D + rhoF * expm1(TC(-6) * v[PHI]);
     }
+    if constexpr (DUST) {
+      // Dust::getDustData at the point (dust.cc:177-200)
+      Dc = (TC)ldo((const TS*)RD(0, a.s_in), idx);
+  #pragma unroll
+      for (int j = 0; j < 3; ++j) Sc[j] = (TC)ldo((const TS*)RD(1 + j, a.s_in), idx);
+      {
+        const TC em4 = exp(TC(-4) * phi), e6 = exp(TC(6) * phi);
+        const TC giSS = em4 * (gi[0] * Sc[0] * Sc[0] + gi[3] * Sc[1] * Sc[1] + gi[5] * Sc[2] * Sc[2] +
+                               TC(2) * (gi[1] * Sc[0] * Sc[1] + gi[2] * Sc[0] * Sc[2] + gi[4] * Sc[1] * Sc[2]));
+        Wc = sqrt(TC(1) + giSS / Dc / Dc);
+  #pragma unroll
+        for (int i = 0; i < 3; ++i) vc[i] = em4 / e6 * (GI(i, 0) * Sc[0] + GI(i, 1) * Sc[1] + GI(i, 2) * Sc[2]);
+      }
+      // Dust::addBSSNSrc (dust.cc:226-264): D, S_i of this stage with W,
+      // detg, g_ij of the state the previous populateDerivedFields saw
+      TC Wl = Wc, detg = exp(TC(6) * phi), e4l = exp(TC(4) * phi);
+      TC gl[6];
+  #pragma unroll
+      for (int q = 0; q < 6; ++q) gl[q] = v[G11 + q] + (diag6(q) ? TC(1) : TC(0));
+      if (a.lag_mode != 0) {
+        double m[6];
+  #pragma unroll
+        for (int q = 0; q < 6; ++q) m[q] = ldo((const double*)RM(q, a.s_lag), idx);
+        if (a.lag_mode == 2) {
+          const double tn = norm_factor<double>(m[0], m[1], m[2], m[3], m[4], m[5]);
+  #pragma unroll
+          for (int q = 0; q < 6; ++q) m[q] = (m[q] + (diag6(q) ? tn : 0.0)) / (1.0 - tn);
+        }
+        const TC phl = (TC)(ldo((const double*)RM(PHI, a.s_lag), idx) + a.phi_F_lag);
+        const TC Dl = (TC)ldo((const TS*)RD(0, a.s_lag), idx);
+        TC Sl[3];
+  #pragma unroll
+        for (int j = 0; j < 3; ++j) Sl[j] = (TC)ldo((const TS*)RD(1 + j, a.s_lag), idx);
+        const TC g11 = (TC)m[0], g12 = (TC)m[1], g13 = (TC)m[2], g22 = (TC)m[3], g23 = (TC)m[4], g33 = (TC)m[5];
+        const TC i11 = TC(1) + g22 + g33 - g23 * g23 + g22 * g33, i22 = TC(1) + g11 + g33 - g13 * g13 + g11 * g33;
+        const TC i33 = TC(1) + g11 + g22 - g12 * g12 + g11 * g22, i12 = g13 * g23 - g12 * (TC(1) + g33);
+        const TC i13 = g12 * g23 - g13 * (TC(1) + g22), i23 = g12 * g13 - g23 * (TC(1) + g11);
+        const TC giSS = exp(TC(-4) * phl) * (i11 * Sl[0] * Sl[0] + i22 * Sl[1] * Sl[1] + i33 * Sl[2] * Sl[2] +
+                                             TC(2) * (i12 * Sl[0] * Sl[1] + i13 * Sl[0] * Sl[2] + i23 * Sl[1] * Sl[2]));
+        Wl = sqrt(TC(1) + giSS / Dl / Dl);
+        detg = exp(TC(6) * phl);
+        e4l = exp(TC(4) * phl);
+  #pragma unroll
+        for (int q = 0; q < 6; ++q) gl[q] = (TC)m[q] + (diag6(q) ? TC(1) : TC(0));
+      }
+      DIFFr += Wl * Dc / detg;
+      DIFFS += Dc / detg * (Wl * Wl - TC(1)) / Wl;
+  #pragma unroll
+      for (int j = 0; j < 3; ++j) Si[j] = Sc[j] / detg;
+  #pragma unroll
+      for (int I = 0; I < 3; ++I)
+  #pragma unroll
+        for (int J = I; J < 3; ++J)
+          STF[sy(I, J)] = Sc[I] * Sc[J] / Wl / Dc / detg - TC(1.0 / 3.0) * (e4l * gl[sy(I, J)]) * DIFFS;
+    }
     DIFFr += (TC)a.lambda;
     DIFFS += TC(-3) * (TC)a.lambda;
+    if constexpr (DUST && !CONS) {
+      if (a.rho_out) a.rho_out[idx] = (double)DIFFr;
+    }
 
     // A^ij and A_ij A^ij (bssn.cc:463-472)
     TC Ac[6];
@@ -453,7 +536,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
 
     // Hamiltonian constraint (bssn.cc:1073-1084), only when a damping term uses it
     TC Hc = TC(0);
-    if (a.use_H || MODE == 1) {
+    if (a.use_H || CONS) {
       const TC rr = DIFFr + rhoF;
       if constexpr (sizeof(TC) == 8) {
         Hc = -exp(TC(5) * phi) / TC(8) * (ricci + TC(2.0 / 3.0) * K * K - AijAij - TC(16) * PI * rr);
@@ -464,7 +547,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       }
     }
 
-    if constexpr (MODE == 1) {
+    if constexpr (CONS) {
       // setConstraintCalcs (bssn.cc:965-1071) per-point quantities, FP64
       double* __restrict__ co = a.cons;
       const long cs = a.cons_stride;
@@ -475,7 +558,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       const double Hs = (e5 / 8.0) * sqrt((double)ricci * ricci + (double)AijAij * AijAij +
                                           pow(2.0 / 3.0 * (double)K * K, 2) + pow(16.0 * 3.14159265358979323846 * rr, 2));
       put(CONS_H, Hc); put(CONS_H + 1, Hs); put(CONS_H + 2, Hc / Hs);
-      // BSSN_MI / BSSN_MI_SCALE (bssn_macros.h:487-527), S_i = 0 for static dust
+      // BSSN_MI / BSSN_MI_SCALE (bssn_macros.h:487-527)
       double M2 = 0, Ms2 = 0;
   #pragma unroll
       for (int I = 0; I < 3; ++I) {
@@ -493,8 +576,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         for (int n = 0; n < 3; ++n)
   #pragma unroll
           for (int m = 0; m < 3; ++m) td -= GL[m][sy(n, I)] * Ac[sy(m, n)];
-        const double MI = e6 * (-2.0 / 3.0 * dK[I] + 6.0 * t6 + td);
-        const double MS = e6 * (fabs(2.0 / 3.0 * dK[I]) + 6.0 * fabs(t6) + fabs(td));
+        const double PI8S = 8.0 * 3.14159265358979323846 * (double)Si[I];
+        const double MI = e6 * (-2.0 / 3.0 * dK[I] - PI8S + 6.0 * t6 + td);
+        const double MS = e6 * (fabs(2.0 / 3.0 * dK[I]) + fabs(PI8S) + 6.0 * fabs(t6) + fabs(td));
         M2 += MI * MI;
         Ms2 += MS * MS;
       }
@@ -554,18 +638,19 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         for (int m = 0; m < 3; ++m)
           AA += AM(m, I) * (GI(m, 0) * AM(0, J) + GI(m, 1) * AM(1, J) + GI(m, 2) * AM(2, J));
         const TC ricTF = ric[s] - TC(1.0 / 3.0) * e4phi * GM(I, J) * ricci;
-        const TC STF = TC(0);  // static dust and Lambda have no trace-free stress
-        rhs[A11 + s] = em4phi * (alpha * (ricTF - TC(8) * PI * STF) - DDa[s]) +
+        rhs[A11 + s] = em4phi * (alpha * (ricTF - TC(8) * PI * STF[s]) - DDa[s]) +
                        alpha * (K * v[A11 + s] - TC(2) * AA) - adj * dt * v[A11 + s] * Hc - ko[A11 + s];
       }
-    // bssn_macros.h:340-361 (S_i = 0 for static dust / Lambda)
+    // bssn_macros.h:340-361
   #pragma unroll
     for (int I = 0; I < 3; ++I) {
       const TC second = G[I][0] * Ac[0] + G[I][3] * Ac[3] + G[I][5] * Ac[5] +
                         TC(2) * (G[I][1] * Ac[1] + G[I][2] * Ac[2] + G[I][4] * Ac[4]) +
                         TC(6) * (Ac[sy(I, 0)] * dphi[0] + Ac[sy(I, 1)] * dphi[1] + Ac[sy(I, 2)] * dphi[2]);
       rhs[GAM1 + I] = TC(-2) * (Ac[sy(I, 0)] * dal[0] + Ac[sy(I, 1)] * dal[1] + Ac[sy(I, 2)] * dal[2]) +
-                      TC(2) * alpha * (second - TC(2.0 / 3.0) * (GI(I, 0) * dK[0] + GI(I, 1) * dK[1] + GI(I, 2) * dK[2])) -
+                      TC(2) * alpha *
+                          (second - TC(2.0 / 3.0) * (GI(I, 0) * dK[0] + GI(I, 1) * dK[1] + GI(I, 2) * dK[2]) -
+                           TC(8) * PI * (GI(I, 0) * Si[0] + GI(I, 1) * Si[1] + GI(I, 2) * Si[2])) -
                       ko[GAM1 + I];
     }
     // bssn.cc:766-795
@@ -623,6 +708,37 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         if (st != 4) sto(fr, idx, (TS)(fo + inc));
       }
     }
+    if constexpr (DUST) {
+      // dust right-hand sides (dust.cc:203-224): flux divergence plus the
+      // momentum source of populateDerivedFields (dust.cc:139-150)
+      TC drhs[NDUST];
+      drhs[0] = ddust[0];
+      const TC pre = TC(0.5) * alpha * Wc * Dc * exp(TC(4) * phi);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const TC dp4 = TC(4) * dphi[k];
+        const TC acc = vc[0] * vc[0] * (dp4 * GM(0, 0) + dg[k][0]) + vc[1] * vc[1] * (dp4 * GM(1, 1) + dg[k][3]) +
+                       vc[2] * vc[2] * (dp4 * GM(2, 2) + dg[k][5]) +
+                       TC(2) * (vc[0] * vc[1] * (dp4 * GM(0, 1) + dg[k][1]) + vc[0] * vc[2] * (dp4 * GM(0, 2) + dg[k][2]) +
+                                vc[1] * vc[2] * (dp4 * GM(1, 2) + dg[k][4]));
+        drhs[1 + k] = ddust[1 + k] + (pre * acc - Wc * Dc * dal[k]);
+      }
+#pragma unroll
+      for (int q = 0; q < NDUST; ++q) {
+        const double cc = (double)drhs[q];
+        TS* __restrict__ o = RD(q, a.s_out);
+        if (st == 0) {
+          sto(o, idx, (TS)cc);
+          continue;
+        }
+        const double p = (double)ldo((const TS*)RD(q, a.s_p), idx);
+        TS* __restrict__ fr = RD(q, a.s_f);
+        const double fo = st >= 2 ? (double)ldo((const TS*)fr, idx) : 0.0;
+        const double inc = a.cf * cc;
+        sto(o, idx, st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc));
+        if (st != 4) sto(fr, idx, (TS)(fo + inc));
+      }
+    }
   }  // valid
 
   // setExtraFieldData after K{n}Finalize (bssn.cc:267-305): the averages of
@@ -667,9 +783,10 @@ stage_kernel_batch(const BatchArgs ba) {
 }
 
 // ------------------------------------------------------------ variants
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA>
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA, int MODE = MODE_STAGE>
 int stage_launch_t(const StageArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((stage_kernel<TS, TC, R, H, TI, TJ, TK, WPE, LA>), dim3(8 * a.per_xcd), dim3(TI * TJ * TK), 0, s, a);
+  hipLaunchKernelGGL((stage_kernel<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE>), dim3(8 * a.per_xcd), dim3(TI * TJ * TK), 0,
+                     s, a);
   return (int)hipGetLastError();
 }
 
@@ -680,10 +797,10 @@ int stage_launch_batch_t(const BatchArgs& ba, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA>
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA, int MODE = MODE_STAGE>
 int stage_info_t(int* vgprs, int* spills, int* lds) {
   hipFuncAttributes attr;
-  hipError_t e = hipFuncGetAttributes(&attr, (const void*)stage_kernel<TS, TC, R, H, TI, TJ, TK, WPE, LA>);
+  hipError_t e = hipFuncGetAttributes(&attr, (const void*)stage_kernel<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE>);
   if (e != hipSuccess) return (int)e;
   *vgprs = attr.numRegs;
   *spills = (int)attr.localSizeBytes;
@@ -691,24 +808,27 @@ int stage_info_t(int* vgprs, int* spills, int* lds) {
   return 0;
 }
 
-template <typename TS, int R, bool LA>
+template <typename TS, int R, bool LA, int MODE = MODE_CONS>
 int cons_launch_t(const StageArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((stage_kernel<TS, double, R, R, 4, 8, 8, 1, LA, 1>), dim3(8 * a.per_xcd), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((stage_kernel<TS, double, R, R, 4, 8, 8, 1, LA, MODE>), dim3(8 * a.per_xcd), dim3(256), 0, s, a);
   return (int)hipGetLastError();
 }
 
-#define CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, BATCH)                                   \
-  static StageVariantReg CG_CAT(reg_, __COUNTER__)(StageVariant{PREC, 2 * R, H, TI, TJ, TK, WPE, LA,          \
-                                                                &stage_launch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>, \
-                                                                &stage_info_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>,   \
-                                                                BATCH});
+#define CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE, BATCH)                              \
+  static StageVariantReg CG_CAT(reg_, __COUNTER__)(                                                           \
+      StageVariant{PREC, 2 * R, H, TI, TJ, TK, WPE, LA, (MODE & MODE_DUST) != 0,                            \
+                   &stage_launch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE>,                                 \
+                   &stage_info_t<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE>, BATCH});
 // single-grid kernel only
 #define CG_REGISTER_STAGE(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA) \
-  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, nullptr)
+  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE, nullptr)
 // single-grid and batched-ensemble kernels
 #define CG_REGISTER_STAGE_B(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA) \
-  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA,     \
+  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE, \
                       (&stage_launch_batch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>))
+// metric + dust fluid (single grid)
+#define CG_REGISTER_DUST(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA) \
+  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_DUST, nullptr)
 #define CG_CAT2(a, b) a##b
 #define CG_CAT(a, b) CG_CAT2(a, b)
 

@@ -90,6 +90,7 @@ __global__ void k_step_init(RegTable t, int rp, int ra, int rf, long n, int norm
 
 template <typename TS>
 __device__ __forceinline__ double diff_r(const SrcArgs& s, double dphi, long x) {
+  if (s.rho) return s.rho[x];
   double r = 0.0;
   if (s.matter == MATTER_STATIC)
     r = exp(-6.0 * (dphi + s.phi_F)) * (double)((const TS*)s.DIFFD)[x] + s.rho_F * expm1(-6.0 * dphi);
@@ -182,6 +183,77 @@ __global__ void k_ext_reduce(const double* __restrict__ in, long rows, double* _
       ext[EXTV_AVGVOL] = exp(6.0 * (sh[EXT_SWPHI][0] / npts / vol));
       if (!keep_rho) ext[EXTV_RHOAVG] = sh[EXT_SWR][0] / npts / vol;
     }
+  }
+}
+
+// ------------------------------------------------------------ dust fluid --
+// Dust::populateDerivedFields, the flux part (dust.cc:109-131, 177-200): at
+// every point of the stage input, v^i = e^{-10 phi} gamma~^{ij} S_j and the
+// fluxes alpha D v^i, alpha S_j v^i (flux q*3 + i; q = 0 D, 1..3 S_j).  At
+// stage 1 of a fused step the metric is normalised on the fly as the stage
+// kernel does (bssn.cc:61-119).
+template <typename TS>
+__global__ void k_dust_flux(const double* __restrict__ met, const TS* __restrict__ alpha, const TS* __restrict__ dreg,
+                            long rstride, int slot, int normalize, double phiF, TS* __restrict__ flux, long n) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  auto M = [&](int f) { return met[((long)f * 4 + slot) * rstride + x]; };
+  double g11 = M(G11), g12 = M(G12), g13 = M(G13), g22 = M(G22), g23 = M(G23), g33 = M(G33);
+  if (normalize) {
+    const double omd = -1.0 * (g11 + g22 + g33 - g12 * g12 - g13 * g13 - g23 * g23 + g11 * g22 + g11 * g33 +
+                               g22 * g33 - g23 * g23 * g11 - g13 * g13 * g22 - g12 * g12 * g33 +
+                               2.0 * g12 * g13 * g23 + g11 * g22 * g33);
+    const double tn = -1.0 * expm1(log1p(-1.0 * omd) / 3.0);
+    g11 = (g11 + tn) / (1.0 - tn); g22 = (g22 + tn) / (1.0 - tn); g33 = (g33 + tn) / (1.0 - tn);
+    g12 = g12 / (1.0 - tn); g13 = g13 / (1.0 - tn); g23 = g23 / (1.0 - tn);
+  }
+  // set_gammai_values (bssn.cc:447-455)
+  const double i11 = 1.0 + g22 + g33 - g23 * g23 + g22 * g33, i22 = 1.0 + g11 + g33 - g13 * g13 + g11 * g33;
+  const double i33 = 1.0 + g11 + g22 - g12 * g12 + g11 * g22, i12 = g13 * g23 - g12 * (1.0 + g33);
+  const double i13 = g12 * g23 - g13 * (1.0 + g22), i23 = g12 * g13 - g23 * (1.0 + g11);
+  const double phi = M(PHI) + phiF;
+  const double al = (double)alpha[(long)slot * rstride + x] + 1.0;
+  double q[4];
+  for (int f = 0; f < NDUST; ++f) q[f] = (double)dreg[((long)f * 4 + slot) * rstride + x];
+  const double S1 = q[1], S2 = q[2], S3 = q[3];
+  const double em4phi = exp(-4.0 * phi), e6phi = exp(6.0 * phi);
+  const double v[3] = {em4phi / e6phi * (i11 * S1 + i12 * S2 + i13 * S3),
+                       em4phi / e6phi * (i12 * S1 + i22 * S2 + i23 * S3),
+                       em4phi / e6phi * (i13 * S1 + i23 * S2 + i33 * S3)};
+  for (int f = 0; f < NDUST; ++f)
+    for (int d = 0; d < 3; ++d) flux[(long)(f * 3 + d) * rstride + x] = (TS)(al * q[f] * v[d]);
+}
+
+// RK4Register::stepInit of the dust registers (dust.cc:65-73)
+template <typename TS>
+__global__ void k_dust_reg_init(TS* __restrict__ dreg, long rstride, int rp, int ra, int rf, long n) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  for (int f = 0; f < NDUST; ++f) {
+    TS* r = dreg + (long)f * 4 * rstride;
+    r[(long)ra * rstride + x] = r[(long)rp * rstride + x];
+    r[(long)rf * rstride + x] = TS(0);
+  }
+}
+
+// Dust::K{n}Finalize (dust.cc:78-106): RK4Register::K{n}Finalize of D, S_j
+template <typename TS>
+__global__ void k_dust_finalize(TS* __restrict__ dreg, long rstride, int rp, int rc, int rf, long n, int stage,
+                                double dt) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  for (int f = 0; f < NDUST; ++f) {
+    TS* r = dreg + (long)f * 4 * rstride;
+    const double c = (double)r[(long)rc * rstride + x], p = (double)r[(long)rp * rstride + x],
+                 fo = (double)r[(long)rf * rstride + x];
+    double fn, cn;
+    if (stage == 1) { fn = fo + dt * c / 6.0; cn = p + dt * c / 2.0; }
+    else if (stage == 2) { fn = fo + dt * c / 3.0; cn = p + dt * c / 2.0; }
+    else if (stage == 3) { fn = fo + dt * c / 3.0; cn = p + dt * c; }
+    else { fn = fo + (dt * c / 6.0 + p); cn = fn; }
+    r[(long)rf * rstride + x] = (TS)fn;
+    r[(long)rc * rstride + x] = (TS)cn;
+    if (stage == 4) r[(long)rp * rstride + x] = (TS)fn;
   }
 }
 
@@ -312,7 +384,7 @@ static RegTable reg_table(cg_ctx* c) {
 
 static bool variant_matches(const cg_ctx* c, const StageVariant& v) {
   return v.prec == c->prec && v.order == c->p.stencil_order && v.halo == c->H && v.live_alpha == alpha_live(c) &&
-         variant_fits(c, v);
+         v.dust == (c->p.matter == CG_MATTER_DUST) && variant_fits(c, v);
 }
 
 // default: the requested tile, else the 4x8x8 tile with the register budget
@@ -346,7 +418,10 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return CG_ERR_ARG;
   if (p.stencil_order != 2 && p.stencil_order != 4 && p.stencil_order != 6 && p.stencil_order != 8) return CG_ERR_ARG;
   if (p.precision != CG_FP64 && p.precision != CG_FP32_SPLIT) return CG_ERR_ARG;
-  if (p.matter != CG_MATTER_NONE && p.matter != CG_MATTER_STATIC) return CG_ERR_ARG;
+  if (p.matter != CG_MATTER_NONE && p.matter != CG_MATTER_STATIC && p.matter != CG_MATTER_DUST) return CG_ERR_ARG;
+  // the dust fluid evolves with an empty or full FRW reference (dust_ic.cc),
+  // so its fields are not small perturbations: FP64 storage only
+  if (p.matter == CG_MATTER_DUST && p.precision != CG_FP64) return CG_ERR_ARG;
   if (p.lapse != CG_LAPSE_STATIC && p.lapse != CG_LAPSE_HARMONIC && p.lapse != CG_LAPSE_CONFORMAL_FLRW)
     return CG_ERR_ARG;
   if (!(p.dx > 0) || !std::isfinite(p.dt)) return CG_ERR_ARG;
@@ -386,6 +461,13 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   HIPCHK(c, hipMalloc((void**)&c->ext, 4 * sizeof(double)));
   HIPCHK(c, hipMemsetAsync(c->ext, 0, 4 * sizeof(double), c->stream));
   HIPCHK(c, hipMalloc((void**)&c->nan_cnt, sizeof(unsigned long long)));
+  if (p.matter == CG_MATTER_DUST) {
+    HIPCHK(c, hipMalloc(&c->dreg, (size_t)NDUST * 4 * pad * c->ts));
+    HIPCHK(c, hipMemsetAsync(c->dreg, 0, (size_t)NDUST * 4 * pad * c->ts, c->stream));
+    HIPCHK(c, hipMalloc(&c->flux, (size_t)NFLUX * pad * c->ts));
+    HIPCHK(c, hipMalloc((void**)&c->rho_src, (size_t)pad * sizeof(double)));
+    HIPCHK(c, hipMemsetAsync(c->rho_src, 0, (size_t)pad * sizeof(double), c->stream));
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CG_OK;
 }
@@ -404,6 +486,9 @@ int cg_destroy(cg_ctx* c) {
   if (c->ext_part) (void)hipFree(c->ext_part);
   if (c->ext_mid) (void)hipFree(c->ext_mid);
   if (c->nan_cnt) (void)hipFree(c->nan_cnt);
+  if (c->dreg) (void)hipFree(c->dreg);
+  if (c->flux) (void)hipFree(c->flux);
+  if (c->rho_src) (void)hipFree(c->rho_src);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CG_OK;
@@ -415,6 +500,11 @@ static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err, size
   *esz = c->ts;
   std::string s(name ? name : "");
   if (s == "DIFFD_a") return c->DIFFD;
+  if (s == "DIFFr_a" && c->rho_src) {  // dust: the last stage's source (read only)
+    if (for_write) { *err = 1; return nullptr; }
+    *esz = 8;
+    return c->rho_src;
+  }
   if (s == "ricci_a" || s == "AijAij_a") {
     if (for_write || !c->cons_valid) { *err = 3; return nullptr; }
     *esz = 8;
@@ -423,6 +513,18 @@ static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err, size
   if (s.size() < 3 || s[s.size() - 2] != '_') { *err = 1; return nullptr; }
   const std::string base = s.substr(0, s.size() - 2);
   const char r = s.back();
+  auto slot_of = [&](char rr) {
+    return rr == 'p' ? c->sP : rr == 'f' ? (c->a_is_p ? c->sP : c->sF) : rr == 'a' ? (c->a_is_p ? c->sP : c->sA)
+           : rr == 'c' ? c->sC : -1;
+  };
+  static const char* kDust[NDUST] = {"dust_D", "dust_S1", "dust_S2", "dust_S3"};
+  for (int q = 0; q < NDUST && c->dreg; ++q) {
+    if (base != kDust[q]) continue;
+    const int slot = slot_of(r);
+    if (slot < 0) { *err = 1; return nullptr; }
+    if (for_write && c->a_is_p && (r == 'a' || r == 'f')) { *err = 2; return nullptr; }
+    return (char*)c->dreg + ((long)q * 4 + slot) * c->rstride * (long)c->ts;
+  }
   for (int f = 0; f < NF; ++f) {
     if (base != kFieldNames[f]) continue;
     int slot;
@@ -534,6 +636,12 @@ StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int
   a.ext_part = c->p.lapse == CG_LAPSE_CONFORMAL_FLRW ? c->ext_part : nullptr;
   a.phi_F_next = c->frw.g_phi;
   a.lambda = c->lambda;
+  a.dreg = c->dreg;
+  a.flux = c->flux;
+  a.s_lag = rin;
+  a.lag_mode = 0;
+  a.phi_F_lag = c->frw.g_phi;
+  a.rho_out = c->rho_src;
   const double dx = c->p.dx, ko = c->p.ko_coeff;
   // KO_dissipation_Q (math.h:36-72): ko*dx^3/64 * sum/dx^4 ; -ko*dx^9/1024 * sum/dx^10
   a.ko_pref = c->p.stencil_order == 2 ? ko / (64.0 * dx) : c->p.stencil_order == 8 ? -ko / (1024.0 * dx) : 0.0;
@@ -634,10 +742,35 @@ static int step_init_t(cg_ctx* c) {
   return rc;
 }
 
+// Dust::stepInit's register part (dust.cc:65-73) into slot ra
+static int dust_reg_init(cg_ctx* c, int rp, int ra, int rf) {
+  const int bs = 256;
+  const long nb = (c->npts + bs - 1) / bs;
+  hipLaunchKernelGGL(k_dust_reg_init<double>, dim3(nb), dim3(bs), 0, c->stream, (double*)c->dreg, c->rstride, rp, ra,
+                     rf, c->npts);
+  HIPCHK(c, hipGetLastError());
+  return CG_OK;
+}
+
+// fluxes of the dust fluid at every point of register slot `slot`
+static int dust_flux(cg_ctx* c, int slot, int normalize) {
+  const int bs = 256;
+  const long nb = (c->npts + bs - 1) / bs;
+  hipLaunchKernelGGL(k_dust_flux<double>, dim3(nb), dim3(bs), 0, c->stream, (const double*)c->met,
+                     (const double*)reg_ptr(c, ALPHA, 0), (const double*)c->dreg, c->rstride, slot, normalize,
+                     c->frw.g_phi, (double*)c->flux, c->npts);
+  HIPCHK(c, hipGetLastError());
+  return CG_OK;
+}
+
 int cg_step_init(cg_ctx* c) {
   if (!c) return CG_ERR_ARG;
   c->ext_lazy = false;  // superseded by this stepInit's averages
   if (c->a_is_p) { c->a_is_p = false; c->sA = 2; c->sC = 3; }
+  if (c->dreg) {
+    c->dust_rk = 0;
+    if (int rc = dust_reg_init(c, c->sP, c->sA, c->sF)) return rc;
+  }
   return c->prec == CG_FP64 ? step_init_t<double>(c) : step_init_t<float>(c);
 }
 
@@ -645,6 +778,19 @@ int cg_rk_evolve(cg_ctx* c) {
   if (!c) return CG_ERR_ARG;
   if (c->a_is_p) return fail(c, CG_ERR_STATE, "RKEvolve before stepInit");
   StageArgs a = make_args(c, 0, c->sA, c->sP, c->sF, c->sC, 0);
+  if (c->dreg) {
+    // Dust::RKEvolve (dust.cc:160-174) with the sources addBSSNSrc formed
+    // from the previous RKEvolve's derived fields: after swap_a_c that
+    // stage's input sits in _c, read at each point before it is overwritten
+    if (int rc = dust_flux(c, c->sA, 0)) return rc;
+    if (c->dust_rk > 0) {
+      a.lag_mode = 1;
+      a.s_lag = c->sC;
+      a.phi_F_lag = c->dust_lag_phiF;
+    }
+    c->dust_lag_phiF = c->frw.g_phi;
+    ++c->dust_rk;
+  }
   return launch(c, a);
 }
 
@@ -663,8 +809,15 @@ int cg_k_finalize(cg_ctx* c, int n) {
   if (c->a_is_p) return fail(c, CG_ERR_STATE, "K-finalize before stepInit");
   c->ext_lazy = false;
   // the finished stage's sources were formed with the FRW values before this step
-  const SrcArgs src{c->DIFFD, c->p.matter, c->frw.g_phi, c->frw.g_rho, c->lambda};
+  const SrcArgs src{c->DIFFD, c->p.matter, c->frw.g_phi, c->frw.g_rho, c->lambda, c->rho_src};
   c->frw.stage(n, c->p.dt);  // bssn.cc:269: FRW stage first
+  if (c->dreg) {
+    const int bs = 256;
+    const long nb = (c->npts + bs - 1) / bs;
+    hipLaunchKernelGGL(k_dust_finalize<double>, dim3(nb), dim3(bs), 0, c->stream, (double*)c->dreg, c->rstride,
+                       c->sP, c->sC, c->sF, c->npts, n, c->p.dt);
+    HIPCHK(c, hipGetLastError());
+  }
   int rc = c->prec == CG_FP64 ? finalize_t<double>(c, n, src) : finalize_t<float>(c, n, src);
   if (rc) return rc;
   std::swap(c->sA, c->sC);  // swap_a_c
@@ -683,9 +836,23 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
   // FRW stage step (bssn.cc:269), whose phi weights the averages of the output
   const bool flrw = c->p.lapse == CG_LAPSE_CONFORMAL_FLRW;
   SrcArgs src4{};
+  // dust: the sources of stage n use W, detg, g_ij of stage n-1's input
+  int prev_in = 0, prev_norm = 0;
+  double prev_phiF = 0.0;
   auto stage = [&](int n, int rin, int rout, int normalize) -> int {
     StageArgs a = make_args(c, n, rin, 0, 1, rout, normalize);
-    src4 = SrcArgs{c->DIFFD, c->p.matter, c->frw.g_phi, c->frw.g_rho, c->lambda};
+    if (c->dreg) {
+      if (int rc = dust_flux(c, rin, normalize)) return rc;
+      if (n > 1) {
+        a.lag_mode = prev_norm ? 2 : 1;
+        a.s_lag = prev_in;
+        a.phi_F_lag = prev_phiF;
+      }
+      prev_in = rin;
+      prev_norm = normalize;
+      prev_phiF = c->frw.g_phi;
+    }
+    src4 = SrcArgs{c->DIFFD, c->p.matter, c->frw.g_phi, c->frw.g_rho, c->lambda, c->rho_src};
     c->frw.stage(n, c->p.dt);
     a.phi_F_next = c->frw.g_phi;
     int rc = launch(c, a);
@@ -800,7 +967,9 @@ int cg_constraints(cg_ctx* c, double H[8], double M[8], double G[7], double A[7]
       hipLaunchKernelGGL(k_step_init<float>, dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin, c->sF,
                          c->npts, c->p.normalize_metric, (double*)nullptr, 0.0);
     HIPCHK(c, hipGetLastError());
+    if (c->dreg && dust_reg_init(c, c->sP, rin, c->sF)) return CG_ERR_HIP;
   }
+  // dust: sources from this state itself (initDustStep's populate, sims/dust.cc:53-63)
   StageArgs a = make_args(c, 0, rin, c->sP, c->sF, c->sC, 0);
   a.cons = c->cons;
   a.cons_stride = pad;
@@ -810,7 +979,7 @@ int cg_constraints(cg_ctx* c, double H[8], double M[8], double G[7], double A[7]
   a.per_xcd = (a.ntx * a.nty * a.ntz + 7) / 8;
   if (c->p.nx <= 4 + c->R - 2 || c->p.ny <= 8 + c->R - 2 || c->p.nz <= 8 + c->R - 2)
     return fail(c, CG_ERR_ARG, "grid too small for the constraint kernel tile");
-  int rc = launch_constraints(c->prec, c->p.stencil_order, alpha_live(c), a, c->stream);
+  int rc = launch_constraints(c->prec, c->p.stencil_order, alpha_live(c), c->dreg != nullptr, a, c->stream);
   if (rc) return fail(c, CG_ERR_HIP, std::string("constraint launch: ") + hipGetErrorString((hipError_t)rc));
   c->cons_valid = true;
   double* part = c->cons_part;

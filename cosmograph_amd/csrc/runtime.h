@@ -84,6 +84,7 @@ struct SrcArgs {
   const void* DIFFD;
   int matter;
   double phi_F, rho_F, lambda;
+  const double* rho;  // DIFFr per point as the stage kernel formed it (dust fluid), else null
 };
 
 }  // namespace cg
@@ -108,6 +109,15 @@ struct cg_ctx {
   void* regs = nullptr;   // [11][4][npts] A_ij, DIFFK, Gamma^i, DIFFalpha in the storage type
   double* met = nullptr;  // [7][4][npts] DIFFgamma11..33, DIFFphi registers, always FP64
   void* DIFFD = nullptr;
+  // dust fluid (components/dust_fluid/dust.h:32-46): registers of D, S1..S3
+  // [4][4][pad] in the storage type, the fluxes of the current stage input
+  // [12][pad], and DIFFr of the last stage [pad] (what the reference keeps
+  // in DIFFr_a; the averages of setExtraFieldData read it)
+  void* dreg = nullptr;
+  void* flux = nullptr;
+  double* rho_src = nullptr;
+  int dust_rk = 0;            // method-level RKEvolve calls since stepInit (source lag)
+  double dust_lag_phiF = 0;   // FRW phi the previous RKEvolve's sources were formed with
   unsigned* nan_dev = nullptr;
   unsigned nan_seen = 0;
   // setExtraFieldData (bssn.cc:165-170): [K_min, K_avg, avg_vol, rho_avg] on

@@ -1,0 +1,113 @@
+"""GPU parity of the dust fluid (components/dust_fluid/dust.cc, the DustSim
+sequence sims/dust.cc:53-169) against the CPU oracle: the metric and the
+four fluid registers after fused steps and after every method-level call,
+the constraints with the fluid's sources, and the averages.  FP64 bar:
+<= 1e-12 relative L2 on every register."""
+import sys
+
+import numpy as np
+import pytest
+
+from common import FIELDS, rel_l2, setup_pair, smooth_random_state
+
+pytestmark = pytest.mark.gpu
+
+DUST = ["dust_D", "dust_S1", "dust_S2", "dust_S3"]
+EMPTY_FRW = {"phi": 0.0, "K": 0.0, "fluids": []}
+FULL_FRW = {"phi": 0.0, "K": -3.0, "fluids": [(3.0 / (8.0 * np.pi), 0.0)]}
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import cosmograph_amd as cg
+    cg.load_library()
+    return cg
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def dust_pair(oracle_mod, gpu, shape, order=4, frw=EMPTY_FRW, lam=0.0, seed=61, lapse="Static"):
+    st = smooth_random_state(shape, seed=seed, with_alpha=(lapse != "Static"))
+    if not frw["fluids"]:
+        st["DIFFK"] = st["DIFFK"] - 0.5  # an expanding box without a reference background
+    o, g = setup_pair(oracle_mod, gpu, {"n": shape, "order": order, "matter": "dust", "lam": lam,
+                                        "lapse": lapse, "frw": frw}, state=st)
+    rng = np.random.default_rng(seed + 1)
+    x = [np.arange(n) / n for n in shape]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    D = 0.05 * (1.0 + 0.02 * np.cos(2 * np.pi * (X + Y)) + 0.01 * np.sin(2 * np.pi * (Z - 2 * X)))
+    S = [1e-3 * np.sin(2 * np.pi * (X + rng.uniform()) + 2 * np.pi * k * Z) for k in range(3)]
+    for name, arr in zip(DUST, [D] + S):
+        o.field(name + "_p")[:] = arr
+        g.upload(name + "_p", arr)
+    return o, g
+
+
+def check(g, o, regs="p", tol=1e-12, names=None):
+    for f in (names or FIELDS + DUST):
+        for r in regs:
+            ref = o.field(f"{f}_{r}")
+            err = rel_l2(g.download(f"{f}_{r}"), ref)
+            assert err <= (tol if np.any(ref != 0) else 1e-30), (f, r, err)
+
+
+@pytest.mark.parametrize("order,frw,lam", [(4, "empty", 0.0), (6, "empty", 0.02), (4, "full", 0.0)])
+def test_dust_fused_steps(oracle_mod, gpu, order, frw, lam):
+    shape = (18, 16, 20)
+    _say(f"case dust fused order={order} frw={frw} lam={lam}")
+    o, g = dust_pair(oracle_mod, gpu, shape, order, EMPTY_FRW if frw == "empty" else FULL_FRW, lam)
+    o.run_steps(3)
+    g.step(3)
+    check(g, o)
+    assert g.frw_get() == o.frw_get()
+
+
+def test_dust_method_sequence(oracle_mod, gpu):
+    """stepInit / RKEvolve / K{n}Finalize of BSSN and the fluid together, the
+    sources rebuilt (with the lag) before every RKEvolve, as runDustStep does."""
+    shape = (16, 16, 16)
+    _say("case dust method-level")
+    o, g = dust_pair(oracle_mod, gpu, shape, 4, EMPTY_FRW, 0.01)
+    L = oracle_mod.lib()
+    for step in range(2):
+        L.or_bssn_step_init(o.h); L.or_dust_step_init(o.h)
+        g.stepInit()
+        for n in range(1, 5):
+            L.or_bssn_clear_src(o.h); L.or_dust_add_src(o.h); L.or_lambda_add_src(o.h)
+            L.or_bssn_rk_evolve(o.h); L.or_dust_rk_evolve(o.h)
+            g.RKEvolve()
+            check(g, o, "c", 1e-10)  # right-hand sides
+            L.or_bssn_k_finalize(o.h, n); L.or_dust_k_finalize(o.h, n)
+            getattr(g, f"K{n}Finalize")()
+            check(g, o, "pacf")
+        ex, want = g.extra_field_data(), o.extra()
+        for k in ("K_min", "K_avg", "avg_vol", "rho_avg"):
+            assert abs(ex[k] - want[k]) <= 1e-12 * abs(want[k]), (step, k)
+
+
+def test_dust_constraints_and_averages(oracle_mod, gpu):
+    shape = (20, 18, 24)
+    _say("case dust constraints")
+    o, g = dust_pair(oracle_mod, gpu, shape, 4, FULL_FRW, 0.0)
+    o.run_steps(2)
+    g.step(2)
+    # averages of the last K4Finalize (DIFFr of stage 4, lagged)
+    ex, want = g.extra_field_data(), o.extra()
+    for k in ("K_min", "K_avg", "avg_vol"):
+        assert abs(ex[k] - want[k]) <= 1e-12 * abs(want[k]), k
+    scale = np.abs(o.field("DIFFr_a")).max()
+    assert abs(ex["rho_avg"] - want["rho_avg"]) <= 1e-12 * max(abs(want["rho_avg"]), scale)
+    np.testing.assert_allclose(g.download("DIFFr_a"), o.field("DIFFr_a"), rtol=1e-11, atol=1e-12 * scale)
+    # the output step sees initDustStep's sources (sims/dust.cc:53-63)
+    L = oracle_mod.lib()
+    L.or_bssn_step_init(o.h); L.or_dust_step_init(o.h)
+    L.or_bssn_clear_src(o.h); L.or_dust_add_src(o.h); L.or_lambda_add_src(o.h)
+    want = o.constraints()
+    got = g.setConstraintCalcs()
+    for c in "HM":
+        for q in (0, 2, 6, 7):
+            w = want[c][q]
+            assert abs(got[c][q] - w) <= 1e-10 * max(abs(w), abs(want[c][2])), (c, q, got[c][q], w)
