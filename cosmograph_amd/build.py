@@ -45,7 +45,26 @@ def build(verbose: bool = True) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, OUT)
+    build_driver(verbose)
     return OUT
+
+
+DRIVER = os.path.join(HERE, "cosmo_gpu")
+
+
+def build_driver(verbose: bool = True) -> str:
+    """The C++ host classes and the `cosmo_gpu` driver (plain host code:
+    g++, linked against libcosmo_gpu.so next to it)."""
+    host = os.path.join(HERE, "host")
+    tmp = DRIVER + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           os.path.join(host, "cosmo_host.cc"), os.path.join(host, "cosmo_main.cc"),
+           "-L", HERE, "-lcosmo_gpu", "-lz", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -1,0 +1,394 @@
+// cosmo_host.cc -- the C++ host classes of cosmo_host.h.
+#include "cosmo_host.h"
+
+#include <sys/stat.h>
+#include <zlib.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+namespace cosmo {
+
+// ------------------------------------------------------------ ConfigParser
+// utils/ConfigParser.cc:17-40: read "param = value" triples until the
+// stream ends; a missing required parameter throws -1
+void ConfigParser::parse(const std::string& fname) {
+  fileName = fname;
+  std::ifstream fin(fname);
+  if (!fin) {
+    std::cerr << "error opening " << fname << "! Make sure you have specified a valid config file.\n";
+    throw -1;
+  }
+  std::string param, eq, val;
+  while (fin >> param >> eq >> val) config[param] = val;
+}
+
+std::string ConfigParser::operator[](const std::string& param) const {
+  auto it = config.find(param);
+  if (it == config.end()) {
+    std::cerr << "Error: param `" << param << "` is required, but was not set! Please set this in the "
+              << "configuration file, " << fileName << ".\n";
+    throw -1;
+  }
+  return it->second;
+}
+
+std::string ConfigParser::operator()(const std::string& param, const std::string& dflt) const {
+  auto it = config.find(param);
+  return it == config.end() ? dflt : it->second;
+}
+
+// ------------------------------------------------------------------ IOData
+IOData::IOData(const std::string& dir_in, bool q) : quiet(q) {
+  std::string d = dir_in.empty() ? "output" : dir_in;
+  if (d.size() > 1 && d.back() == '/') d.pop_back();
+  // IO/IOData.h:48-60: never write into a directory that holds a run
+  if (std::ifstream(d + "/log.txt")) {
+    int s = 1;
+    while (std::ifstream(d + "." + std::to_string(s) + "/log.txt")) ++s;
+    d = d + "." + std::to_string(s);
+  }
+  mkdir(d.c_str(), 0755);
+  output_dir = d + "/";
+  logfile = std::fopen((output_dir + "log.txt").c_str(), "w");
+}
+
+IOData::~IOData() {
+  if (logfile) std::fclose(logfile);
+}
+
+void IOData::log(const std::string& msg) {
+  if (!quiet) std::cout << msg << "\n";
+  if (logfile) {
+    std::fputs((msg + "\n").c_str(), logfile);
+    std::fflush(logfile);
+  }
+}
+
+// one value and a delimiter appended to <dir><file> (gzip text, "%.15g" as
+// io_dump_value writes it, IO/io.cc:813-832)
+static void dump_value(IOData* io, double value, const std::string& file, const char* delim) {
+  const std::string fn = io->dir() + file;
+  gzFile f = gzopen(fn.c_str(), "ab");
+  if (!f) {
+    io->log("Error opening file: " + fn);
+    return;
+  }
+  char data[35];
+  std::snprintf(data, sizeof data, "%.15g", value);
+  gzwrite(f, data, (unsigned)std::strlen(data));
+  gzwrite(f, delim, (unsigned)std::strlen(delim));
+  gzclose(f);
+}
+
+// ------------------------------------------------------------------ params
+cg_params params_from_config(const ConfigParser& cfg) {
+  cg_params p{};
+  if (cfg.has("N")) {
+    p.nx = p.ny = p.nz = std::stoi(cfg["N"]);
+  } else {
+    p.nx = std::stoi(cfg["NX"]);
+    p.ny = std::stoi(cfg["NY"]);
+    p.nz = std::stoi(cfg["NZ"]);
+  }
+  p.stencil_order = std::stoi(cfg("stencil_order", "8"));
+  const double hlen = std::stod(cfg("H_LEN_FRAC", "0.5"));
+  p.dx = cfg.has("dx") ? std::stod(cfg["dx"]) : hlen / (1.0 * p.nx);
+  p.dt = std::stod(cfg("dt_frac", "0.1")) * p.dx;
+  p.ko_coeff = std::stod(cfg("KO_damping_coefficient", "0.0"));
+  p.a_adj_amp = std::stod(cfg("a_adj_amp", "0.0"));
+  p.k_damping_amp = std::stod(cfg("k_damping_amp", "0.0"));
+  p.normalize_metric = std::stoi(cfg("normalize_metric", "1"));
+  const std::string lapse = cfg("lapse", "");
+  if (lapse == "" || lapse == "Static") p.lapse = CG_LAPSE_STATIC;
+  else if (lapse == "Harmonic") p.lapse = CG_LAPSE_HARMONIC;
+  else if (lapse == "ConformalFLRW") p.lapse = CG_LAPSE_CONFORMAL_FLRW;
+  else {
+    std::cerr << "Error - lapse `" << lapse << "` is not available on the GPU path.\n";
+    throw -1;
+  }
+  const std::string sim = cfg("simulation_type", "static");
+  p.matter = sim == "dust" ? CG_MATTER_DUST : sim == "static" ? CG_MATTER_STATIC : CG_MATTER_NONE;
+  const std::string prec = cfg("gpu_precision", "fp64");
+  if (prec == "fp64") p.precision = CG_FP64;
+  else if (prec == "fp32_split") p.precision = CG_FP32_SPLIT;
+  else {
+    std::cerr << "Error - gpu_precision must be fp64 or fp32_split.\n";
+    throw -1;
+  }
+  p.device = std::stoi(cfg("gpu_device", "0"));
+  // features of the reference that this path does not carry
+  if (std::stod(cfg("rescale_metric", "1.0")) != 1.0 || std::stod(cfg("gd_eta", "0.0")) != 0.0 ||
+      cfg("shift", "") != "") {
+    std::cerr << "Error - rescale_metric, gd_eta and shift are not available on the GPU path.\n";
+    throw -1;
+  }
+  return p;
+}
+
+// -------------------------------------------------------------------- BSSN
+BSSN::BSSN(const ConfigParser& cfg) : BSSN(params_from_config(cfg)) {}
+
+BSSN::BSSN(const cg_params& pin) : p(pin) {
+  int rc = cg_create(&p, &ctx);
+  if (rc) {
+    std::cerr << "Error creating the GPU context: " << (ctx ? cg_last_error(ctx) : "invalid parameters") << "\n";
+    if (ctx) cg_destroy(ctx);
+    ctx = nullptr;
+    throw -1;
+  }
+}
+
+BSSN::~BSSN() {
+  if (ctx) cg_destroy(ctx);
+}
+
+void BSSN::check(int rc) const {
+  if (!rc) return;
+  std::cerr << "GPU error: " << cg_last_error(ctx) << "\n";
+  throw rc == CG_ERR_NAN ? 10 : -1;
+}
+
+void BSSN::stepInit() { check(cg_step_init(ctx)); }
+void BSSN::RKEvolve() { check(cg_rk_evolve(ctx)); }
+void BSSN::K1Finalize() { check(cg_k_finalize(ctx, 1)); }
+void BSSN::K2Finalize() { check(cg_k_finalize(ctx, 2)); }
+void BSSN::K3Finalize() { check(cg_k_finalize(ctx, 3)); }
+void BSSN::K4Finalize() { check(cg_k_finalize(ctx, 4)); }
+void BSSN::step(int nsteps) { check(cg_rk4_steps(ctx, nsteps)); }
+
+void BSSN::setConstraintCalcs(double H[8], double M[8], double G[7], double A[7], double S[7]) {
+  check(cg_constraints(ctx, H, M, G, A, S));
+}
+
+void BSSN::setExtraFieldData() {
+  double e[4];
+  check(cg_extra_field_data(ctx, e));
+  K_min = e[0]; K_avg = e[1]; avg_vol = e[2]; rho_avg = e[3];
+}
+
+std::vector<double> BSSN::field(const std::string& name) const {
+  std::vector<double> v((size_t)points());
+  check(cg_field_download(ctx, name.c_str(), v.data(), v.size()));
+  return v;
+}
+
+void BSSN::setField(const std::string& name, const std::vector<double>& v) {
+  if ((long)v.size() != points()) {
+    std::cerr << "Error: field " << name << " has " << v.size() << " values, expected " << points() << "\n";
+    throw -1;
+  }
+  check(cg_field_upload(ctx, name.c_str(), v.data(), v.size()));
+}
+
+long BSSN::numNaNs(const std::string& name) const {
+  long long n = 0;
+  check(cg_num_nans(ctx, name.c_str(), &n));
+  return (long)n;
+}
+
+void BSSN::frwSet(double phi, double K) { check(cg_frw_set(ctx, phi, K)); }
+void BSSN::frwAddFluid(double rho, double w) { check(cg_frw_add_fluid(ctx, rho, w)); }
+void BSSN::frwGet(double out[4]) const { check(cg_frw_get(ctx, out)); }
+
+void Lambda::addBSSNSource(BSSN* bssn) const {
+  // lambda.cc:26-43 runs inside the stage kernel; the value is set once
+  if (cg_set_lambda(bssn->context(), lambda)) throw -1;
+}
+
+// ---------------------------------------------------------------- CosmoSim
+CosmoSim::CosmoSim(const ConfigParser& c) : cfg(c) {
+  iodata = new IOData(cfg("output_dir", "output"), std::stoi(cfg("verbosity", "0")) < 0);
+  num_steps = std::stol(cfg["steps"]);
+  // back up the configuration next to the output (sims/sim.cc:10-14)
+  std::ifstream src(cfg.getFileName());
+  std::ofstream dst(iodata->dir() + "config.txt");
+  dst << src.rdbuf();
+}
+
+CosmoSim::~CosmoSim() {
+  delete bssnSim;
+  delete lambda;
+  delete iodata;
+}
+
+void CosmoSim::init() {
+  bssnSim = new BSSN(cfg);
+  lambda = new Lambda();
+  dt = bssnSim->params().dt;
+  iodata->log("Initialized " + cfg("simulation_type", "static") + " simulation on the GPU: " +
+              std::to_string(bssnSim->params().nx) + "x" + std::to_string(bssnSim->params().ny) + "x" +
+              std::to_string(bssnSim->params().nz) + ", stencil order " +
+              std::to_string(bssnSim->params().stencil_order));
+}
+
+// Initial data written by cosmograph_amd.ics (`save_ics`): <ic_dir>/<field>.f64
+// raw little-endian doubles for every listed field, and <ic_dir>/frw.txt with
+// "phi K", then "fluid rho w" lines and an optional "lambda L" line.  The
+// reference's IC generators (FFTW, boost) are outside this path.
+void CosmoSim::setICs() {
+  const std::string dir = cfg["ic_dir"];
+  std::ifstream frw(dir + "/frw.txt");
+  if (!frw) {
+    iodata->log("Error - missing " + dir + "/frw.txt");
+    throw -1;
+  }
+  double phi = 0, K = 0;
+  if (!(frw >> phi >> K)) {
+    iodata->log("Error - " + dir + "/frw.txt must start with \"phi K\"");
+    throw -1;
+  }
+  bssnSim->frwSet(phi, K);
+  std::string tag;
+  while (frw >> tag) {
+    double a = 0, b = 0;
+    if (tag == "fluid" && (frw >> a >> b)) {
+      bssnSim->frwAddFluid(a, b);
+    } else if (tag == "lambda" && (frw >> a)) {
+      lambda->setLambda(a);
+    } else {
+      iodata->log("Error - unreadable line in " + dir + "/frw.txt at `" + tag + "`");
+      throw -1;
+    }
+  }
+  if (cfg.has("lambda")) lambda->setLambda(std::stod(cfg["lambda"]));
+  lambda->addBSSNSource(bssnSim);
+  std::ifstream list(dir + "/fields.txt");
+  std::string name;
+  const long n = bssnSim->points();
+  while (list >> name) {
+    std::ifstream f(dir + "/" + name + ".f64", std::ios::binary);
+    std::vector<double> v((size_t)n);
+    if (!f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(n * sizeof(double)))) {
+      iodata->log("Error - " + dir + "/" + name + ".f64 does not hold " + std::to_string(n) + " doubles");
+      throw -1;
+    }
+    bssnSim->setField(name, v);
+  }
+  iodata->log("Initial conditions read from " + dir);
+}
+
+// sims/sim.cc:176-183: NaN check every step
+void CosmoSim::runCommonStepTasks() {
+  if (bssnSim->numNaNs("DIFFphi_a") > 0) {
+    iodata->log("\nNAN detected!");
+    throw 10;
+  }
+}
+
+// output of the step before it is taken (sims/static.cc:111-129): the
+// constraint series of IO/io.cc:240-275 and the FRW statistics of
+// IO/io.cc:341-376, byte-compatible gzip text
+void CosmoSim::outputStep() {
+  const long ci = std::stol(cfg("IO_constraint_interval", "0"));
+  if (ci > 0 && step % ci == 0) {
+    double H[8] = {0}, M[8] = {0}, G[7] = {0}, A[7] = {0}, S[7] = {0};
+    bssnSim->setConstraintCalcs(H, M, G, A, S);
+    // columns of IO/io.cc:249-275: mean, stdev, max of C/[C], max C (and L2)
+    const double* rows[5] = {H, M, G, A, S};
+    const char* names[5] = {"H_violations.dat.gz", "M_violations.dat.gz", "G_violations.dat.gz",
+                            "A_violations.dat.gz", "S_violations.dat.gz"};
+    for (int c = 0; c < 5; ++c) {
+      const double* v = rows[c];
+      const bool l2 = c < 2;
+      dump_value(iodata, v[4], names[c], "\t");
+      dump_value(iodata, v[5], names[c], "\t");
+      dump_value(iodata, v[6], names[c], "\t");
+      dump_value(iodata, v[2], names[c], l2 ? "\t" : "\n");
+      if (l2) dump_value(iodata, v[7], names[c], "\n");
+    }
+  }
+  const long si = std::stol(cfg("IO_bssnstats_interval", "0"));
+  if (si > 0 && step % si == 0) {
+    double f[4];
+    bssnSim->frwGet(f);
+    // phi, K, rho, S of the reference FRW, each followed by a tab
+    const std::string file = cfg("dump_file", "calculated") + ".frwdat.gz";
+    for (int q = 0; q < 4; ++q) dump_value(iodata, f[q], file, q < 3 ? "\t" : "\t\n");
+  }
+}
+
+// StaticSim/DustSim::runStep (sims/static.cc:172-178, sims/dust.cc:163-169):
+// initStep, NaN check, output, RK step
+void CosmoSim::runStep() {
+  runCommonStepTasks();
+  auto t0 = std::chrono::steady_clock::now();
+  outputStep();
+  auto t1 = std::chrono::steady_clock::now();
+  bssnSim->step(1);
+  auto t2 = std::chrono::steady_clock::now();
+  t_out += std::chrono::duration<double>(t1 - t0).count();
+  t_rk += std::chrono::duration<double>(t2 - t1).count();
+  ++rk_steps;
+}
+
+// sims/sim.cc:101-131: `steps = N` runs N+1 steps; the expansion goal is
+// checked on avg_vol after every step
+void CosmoSim::run() {
+  iodata->log("Running simulation...");
+  const bool stop_goal = std::stod(cfg("stop_at_expansion_goal", "0")) != 0.0;
+  const double goal = std::stod(cfg("expansion_goal", "100.0"));
+  double avg_vol_i = 1.0;
+  while (step <= num_steps) {
+    runStep();
+    t += dt;
+    if (step == 0 || stop_goal) bssnSim->setExtraFieldData();
+    if (step == 0) {
+      avg_vol_i = bssnSim->avg_vol;
+    } else if (stop_goal && std::pow(bssnSim->avg_vol / avg_vol_i, 1.0 / 3.0) >= goal) {
+      iodata->log("Target expasion reached, run ending.");
+      break;
+    }
+    ++step;
+  }
+  if (std::stoi(cfg("IO_final_state", "0")) > 0) {
+    std::ofstream list(iodata->dir() + "final_fields.txt");
+    static const char* kNames[] = {"DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIFFgamma22", "DIFFgamma23",
+                                   "DIFFgamma33", "DIFFphi", "A11", "A12", "A13", "A22", "A23", "A33", "DIFFK",
+                                   "Gamma1", "Gamma2", "Gamma3", "DIFFalpha", "dust_D", "dust_S1", "dust_S2",
+                                   "dust_S3"};
+    const int nf = bssnSim->params().matter == CG_MATTER_DUST ? 22 : 18;
+    for (int f = 0; f < nf; ++f) {
+      const std::string name = std::string(kNames[f]) + "_p";
+      std::vector<double> v = bssnSim->field(name);
+      std::ofstream out(iodata->dir() + name + ".f64", std::ios::binary);
+      out.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(v.size() * sizeof(double)));
+      list << name << "\n";
+    }
+  }
+  std::ostringstream s;
+  s << "\nEnding simulation.\nRK steps: " << rk_steps << "\nRK_steps time: " << t_rk << " s ("
+    << (t_rk > 0 ? (double)bssnSim->points() * rk_steps / t_rk : 0.0) << " point-updates/s)\noutput time: " << t_out
+    << " s";
+  iodata->log(s.str());
+}
+
+StaticSim::StaticSim(const ConfigParser& c) : CosmoSim(c) {
+  const std::string lapse = cfg("lapse", "");
+  if (lapse != "" && lapse != "Static" && lapse != "ConformalFLRW") {
+    iodata->log("Error - not using a constant lapse gauge! You must use it for 'static' sims.");
+    throw -1;
+  }
+}
+
+DustSim::DustSim(const ConfigParser& c) : CosmoSim(c) {
+  if (cfg("shift", "") != "") {
+    iodata->log("Error - non-zero shift not ok for this class!");
+    throw -1;
+  }
+}
+
+CosmoSim* make_sim(const ConfigParser& cfg) {
+  const std::string type = cfg["simulation_type"];
+  if (type == "static") return new StaticSim(cfg);
+  if (type == "dust") return new DustSim(cfg);
+  if (type == "vacuum") return new VacuumSim(cfg);
+  std::cerr << "Invalid simulation type specified. ";
+  throw 2;
+}
+
+}  // namespace cosmo

@@ -80,3 +80,43 @@ def static_dust_ics(n: int, spectral_index: float = -2.0, rms: float = 1e-3, see
     K_frw = -np.sqrt(24.0 * np.pi * RHO_BAR)
     frw = {"phi": 0.0, "K": K_frw, "fluids": [(RHO_BAR, 0.0)]}
     return fields, frw
+
+
+def save_ics(directory: str, fields: dict, frw: dict, lam: float = 0.0) -> None:
+    """Write initial data in the layout the `cosmo_gpu` driver reads
+    (`ic_dir` config key): `fields.txt` lists the field-map names, each
+    `<name>.f64` holds its values as raw little-endian doubles in the grid's
+    index order, and `frw.txt` holds "phi K", one "fluid rho w" line per
+    fluid and "lambda L"."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    with open(os.path.join(directory, "fields.txt"), "w") as fh:
+        for name, arr in fields.items():
+            np.ascontiguousarray(arr, dtype="<f8").tofile(os.path.join(directory, name + ".f64"))
+            fh.write(name + "\n")
+    with open(os.path.join(directory, "frw.txt"), "w") as fh:
+        # repr of a Python float round-trips exactly through strtod
+        fh.write(f"{float(frw['phi'])!r} {float(frw['K'])!r}\n")
+        for rho, w in frw.get("fluids", []):
+            fh.write(f"fluid {float(rho)!r} {float(w)!r}\n")
+        fh.write(f"lambda {float(lam)!r}\n")
+
+
+def main(argv=None):
+    """python -m cosmograph_amd.ics OUT_DIR --n 64 [--spectral-index -2] [--rms 1e-3] [--seed 9]"""
+    import argparse
+    ap = argparse.ArgumentParser(description="static-dust initial data for the cosmo_gpu driver")
+    ap.add_argument("out")
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--spectral-index", type=float, default=-2.0)
+    ap.add_argument("--rms", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=9)
+    ap.add_argument("--box", type=float, default=1.0)
+    ap.add_argument("--lambda", dest="lam", type=float, default=0.0)
+    a = ap.parse_args(argv)
+    fields, frw = static_dust_ics(a.n, a.spectral_index, a.rms, a.seed, a.box)
+    save_ics(a.out, fields, frw, a.lam)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,111 @@
+"""The `cosmo_gpu` driver end to end on the GPU: a config file and initial
+data in, the StaticSim / DustSim step sequence of the reference
+(`steps = N` runs N+1 RK4 steps, output before each step, sims/sim.cc:101-131),
+the constraint and FRW series in the reference's gzip text format, and the
+final state, which must match the oracle run for the same number of steps."""
+import gzip
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from common import FIELDS, rel_l2, smooth_random_state
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "cosmograph_amd", "cosmo_gpu")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_driver(cfg_path):
+    r = subprocess.run([DRIVER, str(cfg_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r
+
+
+def rows(path):
+    with gzip.open(path, "rt") as fh:
+        return [[float(x) for x in line.split("\t") if x] for line in fh.read().splitlines()]
+
+
+def oracle_output_constraints(oracle_mod, o, dust):
+    L = oracle_mod.lib()
+    L.or_bssn_step_init(o.h)
+    if dust:
+        L.or_dust_step_init(o.h)
+    L.or_bssn_clear_src(o.h)
+    (L.or_dust_add_src if dust else L.or_static_add_src)(o.h)
+    L.or_lambda_add_src(o.h)
+    return o.constraints()
+
+
+@pytest.mark.parametrize("kind", ["static", "dust"])
+def test_driver_matches_oracle(oracle_mod, tmp_path, kind):
+    from cosmograph_amd.ics import save_ics, static_dust_ics
+    n, steps, lam = 20, 2, 0.01
+    _say(f"case driver {kind} n={n} steps={steps}")
+    if kind == "static":
+        fields, frw = static_dust_ics(n, -2.0, 1e-3, 5)
+    else:
+        st = smooth_random_state((n, n, n), seed=71, with_alpha=False)
+        fields = {f + "_p": st[f] for f in FIELDS}
+        fields["dust_D_p"] = 0.05 + 0.1 * st["DIFFD"]
+        fields["dust_S1_p"] = 0.5 * st["DIFFK"]
+        frw = {"phi": 0.0, "K": 0.0, "fluids": []}
+    save_ics(str(tmp_path / "ic"), fields, frw, lam)
+    out = tmp_path / "out"
+    cfg = tmp_path / "config.txt"
+    cfg.write_text(f"""N = {n}
+stencil_order = 4
+H_LEN_FRAC = 1
+dt_frac = 0.1
+simulation_type = {kind}
+lapse = Static
+steps = {steps}
+ic_dir = {tmp_path / 'ic'}
+output_dir = {out}
+IO_constraint_interval = 1
+IO_bssnstats_interval = 1
+dump_file = calculated
+IO_final_state = 1
+""")
+    r = run_driver(cfg)
+    assert "RK steps: 3" in r.stdout                        # steps = 2 runs 3 RK4 steps
+
+    # the oracle, same data and sequence
+    p = oracle_mod.make_params(n, order=4, matter=kind, lam=lam)
+    o = oracle_mod.Sim(p)
+    for name, arr in fields.items():
+        o.field(name)[:] = arr
+    o.frw_set(frw["phi"], frw["K"])
+    for rho, w in frw["fluids"]:
+        o.frw_add_fluid(rho, w)
+    want_H = []
+    for s in range(steps + 1):
+        want_H.append(oracle_output_constraints(oracle_mod, o, kind == "dust")["H"])
+        o.run_steps(1)
+    names = (out / "final_fields.txt").read_text().split()
+    assert len(names) == (22 if kind == "dust" else 18)
+    for name in names:
+        got = np.fromfile(out / (name + ".f64"), dtype="<f8").reshape(n, n, n)
+        ref = o.field(name)
+        assert rel_l2(got, ref) <= (1e-12 if np.any(ref != 0) else 1e-30), name
+
+    H = rows(out / "H_violations.dat.gz")
+    assert len(H) == steps + 1 and all(len(r_) == 5 for r_ in H)
+    for s in range(steps + 1):
+        # columns: mean, stdev, max of H/[H], max H, L2 (IO/io.cc:249-253)
+        for col, q in zip(range(5), (4, 5, 6, 2, 7)):
+            w = want_H[s][q]
+            assert abs(H[s][col] - w) <= 1e-9 * max(abs(w), abs(want_H[s][2]) * 1e-3), (s, col, H[s][col], w)
+    for f in ("M_violations", "G_violations", "A_violations", "S_violations"):
+        assert len(rows(out / (f + ".dat.gz"))) == steps + 1
+    frw_rows = rows(out / "calculated.frwdat.gz")
+    assert len(frw_rows) == steps + 1 and all(len(r_) == 4 for r_ in frw_rows)
