@@ -52,6 +52,21 @@ __device__ __forceinline__ void sto(T* __restrict__ base, unsigned eoff, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)(eoff * (unsigned)sizeof(T))) = v;
 }
 
+// two consecutive elements from an element-aligned address (one 16-byte /
+// 8-byte load; global memory allows element alignment on gfx950)
+typedef double d2u __attribute__((ext_vector_type(2))) __attribute__((aligned(8)));
+typedef float f2u __attribute__((ext_vector_type(2))) __attribute__((aligned(4)));
+__device__ __forceinline__ void ld2(const double* __restrict__ base, unsigned eoff, double& v0, double& v1) {
+  const d2u v = *reinterpret_cast<const d2u*>(reinterpret_cast<const char*>(base) + (size_t)(eoff * 8u));
+  v0 = v.x;
+  v1 = v.y;
+}
+__device__ __forceinline__ void ld2(const float* __restrict__ base, unsigned eoff, float& v0, float& v1) {
+  const f2u v = *reinterpret_cast<const f2u*>(reinterpret_cast<const char*>(base) + (size_t)(eoff * 4u));
+  v0 = v.x;
+  v1 = v.y;
+}
+
 template <typename T>
 __device__ __forceinline__ bool nan_as_float(T v) {
   // numNaNs tests the float-rounded value (utils/math.h:1524-1541)
@@ -108,9 +123,15 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   constexpr int HJ = TJ + 2 * H, HK = TK + 2 * H, HI = TI + 2 * H;
   constexpr int HV = HI * HJ * HK;
   constexpr int SI = HJ * HK, SJ = HK, SK = 1;
-  constexpr int NPF = (HV + NT - 1) / NT;
-  __shared__ double ldsraw[2][HV];  // double-buffered field tile (FP32 fields use the first half)
-  __shared__ double ldn[HV];        // stage-1 unit-determinant factor t at every halo point
+  // The halo tile is loaded in pairs of k-neighbours (HK is even, so a pair
+  // never straddles a row): one 16-byte (FP64) or 8-byte (FP32) load per
+  // pair, or two element loads where the row wraps around the periodic k
+  // boundary between the two.
+  static_assert(HK % 2 == 0, "halo rows hold whole pairs");
+  constexpr int HP = HV / 2;
+  constexpr int NPP = (HP + NT - 1) / NT;
+  __shared__ __align__(16) double ldsraw[2][HV];  // double-buffered field tile (FP32 fields use the first half)
+  __shared__ double ldn[HV];                      // stage-1 unit-determinant factor t at every halo point
 
   const int tz = t % a.ntz, ty = (t / a.ntz) % a.nty, tx = t / (a.ntz * a.nty);
   const int i0 = tx * TI, j0 = ty * TJ, k0 = tz * TK;
@@ -123,20 +144,35 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   const unsigned idx = valid ? (unsigned)((px * a.ny + py) * a.nz + pz) : 0u;
   const int c = ((li + H) * HJ + (lj + H)) * HK + (lk + H);
 
-  // wrapped global offsets of this thread's share of the halo tile; the same
-  // share is used for every field, so per-element data can stay in registers
-  unsigned goff[NPF];
+  // wrapped global offset of the first element of each of this thread's
+  // halo pairs (pair p = tid + q*NT, elements 2p and 2p+1); the same share is
+  // used for every field, so per-element data can stay in registers.  Bit q
+  // of `split` marks a pair whose first element is the last of its row
+  // (z = nz-1) and whose second is therefore at z = 0.
+  unsigned goff[NPP];
+  unsigned split = 0u;
 #pragma unroll
-  for (int q = 0; q < NPF; ++q) {
-    const int e = tid + q * NT;
-    const int ee = e < HV ? e : 0;
-    const int hk = ee % HK, hj = (ee / HK) % HJ, hi = ee / (HK * HJ);
+  for (int q = 0; q < NPP; ++q) {
+    const int pp = tid + q * NT;
+    const int e = 2 * (pp < HP ? pp : 0);
+    const int hk = e % HK, hj = (e / HK) % HJ, hi = e / (HK * HJ);
     int x = i0 + hi - H, y = j0 + hj - H, z = k0 + hk - H;
+    if (z == -1 || z == a.nz - 1) split |= 1u << q;
     x += x < 0 ? a.nx : 0; x -= x >= a.nx ? a.nx : 0;
     y += y < 0 ? a.ny : 0; y -= y >= a.ny ? a.ny : 0;
     z += z < 0 ? a.nz : 0; z -= z >= a.nz ? a.nz : 0;
     goff[q] = (unsigned)((x * a.ny + y) * a.nz + z);
   }
+  const unsigned wrapk = (unsigned)(a.nz - 1);
+  // both elements of pair q of a field
+  auto pair = [&](const auto* __restrict__ src, int q, auto& v0, auto& v1) {
+    if (!((split >> q) & 1u)) {
+      ld2(src, goff[q], v0, v1);
+    } else {
+      v0 = ldo(src, goff[q]);
+      v1 = ldo(src, goff[q] - wrapk);
+    }
+  };
 
   auto RM = [&](int f, int slot) -> double* { return a.met + ((long)f * 4 + slot) * a.rstride; };
   auto RS = [&](int f, int slot) -> TS* { return (TS*)a.regs + ((long)(f - 7) * 4 + slot) * a.rstride; };
@@ -154,50 +190,59 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   // thread writes and later reads only its own elements, so no barrier.
   if (norm) {
 #pragma unroll
-    for (int q = 0; q < NPF; ++q) {
-      const int e = tid + q * NT;
-      if (e < HV) {
-        const unsigned o = goff[q];
-        ldn[e] = norm_factor<double>(ldo(INM(G11), o), ldo(INM(G12), o), ldo(INM(G13), o), ldo(INM(G22), o),
-                                     ldo(INM(G23), o), ldo(INM(G33), o));
+    for (int q = 0; q < NPP; ++q) {
+      const int pp = tid + q * NT;
+      if (pp < HP) {
+        double g[6][2];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) pair(INM(G11 + m), q, g[m][0], g[m][1]);
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          ldn[2 * pp + h] = norm_factor<double>(g[0][h], g[1][h], g[2][h], g[3][h], g[4][h], g[5][h]);
       }
     }
   }
 
-  double pd[NPF];
-  TC pf[NPF];
+  double pd[NPP][2];
+  TS pf[NPP][2];
   auto fetch = [&](int f) {
     if (metric_field(f)) {
       const double* __restrict__ src = INM(f);
 #pragma unroll
-      for (int q = 0; q < NPF; ++q) pd[q] = ldo(src, goff[q]);
+      for (int q = 0; q < NPP; ++q)
+        if (tid + q * NT < HP) pair(src, q, pd[q][0], pd[q][1]);
     } else {
       const TS* __restrict__ src = INS(f);
 #pragma unroll
-      for (int q = 0; q < NPF; ++q) pf[q] = (TC)ldo(src, goff[q]);
+      for (int q = 0; q < NPP; ++q)
+        if (tid + q * NT < HP) pair(src, q, pf[q][0], pf[q][1]);
     }
   };
   auto store = [&](int buf, int f) {
     if (metric_field(f)) {
       double* __restrict__ dst = ldsraw[buf];
 #pragma unroll
-      for (int q = 0; q < NPF; ++q) {
-        const int e = tid + q * NT;
-        if (e < HV) {
-          double v = pd[q];
+      for (int q = 0; q < NPP; ++q) {
+        const int pp = tid + q * NT;
+        if (pp < HP) {
+          double v0 = pd[q][0], v1 = pd[q][1];
           if (norm && f <= G33) {
-            const double tn = ldn[e];
-            v = (v + (diag6(f) ? tn : 0.0)) / (1.0 - tn);
+            const double t0 = ldn[2 * pp], t1 = ldn[2 * pp + 1];
+            v0 = (v0 + (diag6(f) ? t0 : 0.0)) / (1.0 - t0);
+            v1 = (v1 + (diag6(f) ? t1 : 0.0)) / (1.0 - t1);
           }
-          dst[e] = v;
+          *reinterpret_cast<d2u*>(dst + 2 * pp) = d2u{v0, v1};
         }
       }
     } else {
       TC* __restrict__ dst = reinterpret_cast<TC*>(ldsraw[buf]);
 #pragma unroll
-      for (int q = 0; q < NPF; ++q) {
-        const int e = tid + q * NT;
-        if (e < HV) dst[e] = pf[q];
+      for (int q = 0; q < NPP; ++q) {
+        const int pp = tid + q * NT;
+        if (pp < HP) {
+          dst[2 * pp] = (TC)pf[q][0];
+          dst[2 * pp + 1] = (TC)pf[q][1];
+        }
       }
     }
   };
