@@ -130,7 +130,7 @@ int cg_batch_create(const cg_params* params, int n, cg_batch** out) {
     if (p.precision != p0.precision || p.stencil_order != p0.stencil_order || p.device != p0.device ||
         (p.ko_coeff != 0.0) != (p0.ko_coeff != 0.0))
       return CG_ERR_ARG;  // one kernel variant serves every grid
-    if (p.lapse == CG_LAPSE_CONFORMAL_FLRW) return CG_ERR_ARG;  // per-grid K_avg every stage: single grids only
+    if (lapse_needs_kavg(p.lapse)) return CG_ERR_ARG;  // per-grid K_avg every stage: single grids only
     if (p.matter == CG_MATTER_DUST) return CG_ERR_ARG;           // the dust fluid runs on single grids
     if (p.ko_coeff != 0.0 && p.normalize_metric && (p.stencil_order == 2 || p.stencil_order == 8))
       return CG_ERR_ARG;  // needs the materialised stepInit of the single-grid path
@@ -353,40 +353,46 @@ int cg_batch_autotune(cg_batch* b, int reps, char* report, size_t report_len) {
   std::string rep = "[";
   const StageVariant* best = nullptr;
   double best_ms = 1e30;
+  int best_order = 0;
   for (const auto& v : stage_variants()) {
     if (!batch_fits(b, v)) continue;
-    if (int rc = build_tables(b, &v)) return rc;
-    for (int r = 0; r < n; ++r) {  // retarget stage 2 to slots 0 -> 3
-      StageArgs& a = b->tab_h[1 * n + r];
-      a.s_in = 0; a.s_out = 3;
+    for (int order = 0; order < 3; ++order) {
+      for (auto* c : b->m) c->tile_order = order;
+      if (int rc = build_tables(b, &v)) return rc;
+      for (int r = 0; r < n; ++r) {  // retarget stage 2 to slots 0 -> 3
+        StageArgs& a = b->tab_h[1 * n + r];
+        a.s_in = 0; a.s_out = 3;
+      }
+      BCHK(b, hipMemcpyAsync(b->tab + n, b->tab_h + n, sizeof(StageArgs) * n, hipMemcpyHostToDevice, b->stream));
+      BatchArgs ba{};
+      ba.tab = b->tab + n;
+      ba.tstart = b->tstart;
+      ba.frw = b->frw;
+      ba.nreal = n;
+      ba.total = b->total;
+      ba.per_xcd = (b->total + 7) / 8;
+      int rc = v.launch_batch(ba, b->stream);  // warm-up
+      if (rc) return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
+      BCHK(b, hipEventRecord(b->e0, b->stream));
+      for (int q = 0; q < reps; ++q)
+        if ((rc = v.launch_batch(ba, b->stream)))
+          return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
+      BCHK(b, hipEventRecord(b->e1, b->stream));
+      BCHK(b, hipEventSynchronize(b->e1));
+      float ms = 0.f;
+      BCHK(b, hipEventElapsedTime(&ms, b->e0, b->e1));
+      const double avg = ms / reps;
+      int vg = 0, sc = 0, ld = 0;
+      v.info(&vg, &sc, &ld);
+      char buf[256];
+      snprintf(buf, sizeof buf,
+               "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"order\":%d,\"ms\":%.4f,\"vgprs\":%d,\"scratch\":%d}",
+               rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, order, avg, vg, sc);
+      rep += buf;
+      if (avg < best_ms) { best_ms = avg; best = &v; best_order = order; }
     }
-    BCHK(b, hipMemcpyAsync(b->tab + n, b->tab_h + n, sizeof(StageArgs) * n, hipMemcpyHostToDevice, b->stream));
-    BatchArgs ba{};
-    ba.tab = b->tab + n;
-    ba.tstart = b->tstart;
-    ba.frw = b->frw;
-    ba.nreal = n;
-    ba.total = b->total;
-    ba.per_xcd = (b->total + 7) / 8;
-    int rc = v.launch_batch(ba, b->stream);  // warm-up
-    if (rc) return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
-    BCHK(b, hipEventRecord(b->e0, b->stream));
-    for (int q = 0; q < reps; ++q)
-      if ((rc = v.launch_batch(ba, b->stream)))
-        return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
-    BCHK(b, hipEventRecord(b->e1, b->stream));
-    BCHK(b, hipEventSynchronize(b->e1));
-    float ms = 0.f;
-    BCHK(b, hipEventElapsedTime(&ms, b->e0, b->e1));
-    const double avg = ms / reps;
-    int vg = 0, sc = 0, ld = 0;
-    v.info(&vg, &sc, &ld);
-    char buf[256];
-    snprintf(buf, sizeof buf, "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"ms\":%.4f,\"vgprs\":%d,\"scratch\":%d}",
-             rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, avg, vg, sc);
-    rep += buf;
-    if (avg < best_ms) { best_ms = avg; best = &v; }
   }
+  for (auto* c : b->m) c->tile_order = best_order;
   rep += "]";
   if (report && report_len) {
     strncpy(report, rep.c_str(), report_len - 1);

@@ -17,7 +17,15 @@ constexpr int NDUST = 4;     // D, S1, S2, S3
 constexpr int NFLUX = 12;    // alpha D v^i (3), alpha S_j v^i (9)
 // stage-kernel modes: bit 0 constraint diagnostics, bit 1 dust fluid
 enum { MODE_STAGE = 0, MODE_CONS = 1, MODE_DUST = 2, MODE_DUST_CONS = 3 };
-enum Lapse { LAPSE_STATIC = 0, LAPSE_HARMONIC = 1, LAPSE_CONFORMAL_FLRW = 2 };
+enum Lapse {
+  LAPSE_STATIC = 0, LAPSE_HARMONIC = 1, LAPSE_CONFORMAL_FLRW = 2, LAPSE_ANHARMONIC = 3, LAPSE_ONE_PLUS_LOG = 4,
+  LAPSE_CONFORMAL_SYNC = 5, LAPSE_AWA_GAUGE_WAVE = 6, LAPSE_AWA_SHIFTED_WAVE = 7, LAPSE_DAMPED_WAVE = 8,
+  LAPSE_TEST_AIJ_DRIVER = 9, LAPSE_N = 10
+};
+// gauges that read K_avg of setExtraFieldData every stage
+__host__ __device__ constexpr bool lapse_needs_kavg(int l) {
+  return l == LAPSE_CONFORMAL_FLRW || l == LAPSE_ANHARMONIC || l == LAPSE_ONE_PLUS_LOG;
+}
 
 // One fused RK4 stage over the whole grid: stencils of the `in` register,
 // BSSN right-hand side, matter sources, and the RK4Register update
@@ -48,6 +56,7 @@ struct StageArgs {
   int nx, ny, nz;
   int ntx, nty, ntz;      // tile counts per axis
   int per_xcd;            // tiles handled per XCD (XCD-aware block remap)
+  int tile_order;         // traversal order of the tiles (0, 1, 2: j, k, i slowest)
   int stage;              // 0: RHS only into out; 1..4: fused K{n}Finalize
   int normalize;          // stage 1: det(gamma)=1 rescale, trace-free A on the fly
   int matter;
@@ -62,6 +71,7 @@ struct StageArgs {
   double phi_F_next;      // FRW phi after this stage's FRW step (weights of the averages)
   double lambda;
   double ko_pref;         // KO prefactor (sign included), per unit stencil sum
+  double gd_c, dw_mu_l, dw_p;  // gauge parameters
   double a_adj, k_damp;
   double H_frw;           // 2/3 K_F^2 - 16 pi rho_F (for the split-precision H)
   unsigned int* nan_count;

@@ -133,7 +133,20 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   __shared__ __align__(16) double ldsraw[2][HV];  // double-buffered field tile (FP32 fields use the first half)
   __shared__ double ldn[HV];                      // stage-1 unit-determinant factor t at every halo point
 
-  const int tz = t % a.ntz, ty = (t / a.ntz) % a.nty, tx = t / (a.ntz * a.nty);
+  // Tile order: the ~100 tiles one XCD has in flight hold each tile's
+  // neighbours along the two faster axes, so their halos are re-read from L2
+  // and only the halo along the slowest axis comes back from memory.  The
+  // slowest axis should be the one with the smallest halo share (the
+  // autotuner picks): 0 = j slowest (k, i, j), 1 = k slowest (i, j, k),
+  // 2 = i slowest (k, j, i)
+  int tx, ty, tz;
+  if (a.tile_order == 0) {
+    tz = t % a.ntz; tx = (t / a.ntz) % a.ntx; ty = t / (a.ntz * a.ntx);
+  } else if (a.tile_order == 1) {
+    tx = t % a.ntx; ty = (t / a.ntx) % a.nty; tz = t / (a.ntx * a.nty);
+  } else {
+    tz = t % a.ntz; ty = (t / a.ntz) % a.nty; tx = t / (a.ntz * a.nty);
+  }
   const int i0 = tx * TI, j0 = ty * TJ, k0 = tz * TK;
 
   const int tid = threadIdx.x;
@@ -704,10 +717,23 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
               (TC)a.k_damp * Hc * exp(TC(-5) * phi) - ko[KK];
     // bssn.cc:797-820
     rhs[PHI] = TC(0.1) * adj * dt * Hc - TC(1.0 / 6.0) * (alpha * v[KK] + v[ALPHA] * KF) - ko[PHI];
-    // BSSNGaugeHandler.cc:16-35
+    // BSSNGaugeHandler.cc (the shift-free gauges; beta = 0)
     TC lapse = TC(0);
-    if (a.lapse == LAPSE_HARMONIC) lapse = -alpha * K;
-    else if (a.lapse == LAPSE_CONFORMAL_FLRW) lapse = TC(-1.0 / 3.0) * alpha * (TC)a.ext_in[EXTV_KAVG];
+    switch (a.lapse) {
+      case LAPSE_HARMONIC: lapse = -alpha * K; break;                                          // :32-35
+      case LAPSE_CONFORMAL_FLRW: lapse = TC(-1.0 / 3.0) * alpha * (TC)a.ext_in[EXTV_KAVG]; break;  // :24-27
+      case LAPSE_ANHARMONIC: lapse = alpha * alpha * (K - (TC)a.ext_in[EXTV_KAVG]); break;     // :50-58
+      case LAPSE_ONE_PLUS_LOG:                                                                 // :64-68
+        lapse = TC(-2) * alpha * (K - (TC)a.ext_in[EXTV_KAVG]) * (TC)a.gd_c;
+        break;
+      case LAPSE_CONFORMAL_SYNC: lapse = TC(-1.0 / 3.0) * alpha * KF; break;                   // :84-87
+      case LAPSE_AWA_GAUGE_WAVE: lapse = -alpha * alpha * v[KK]; break;                        // :209-212
+      case LAPSE_AWA_SHIFTED_WAVE: lapse = -v[KK]; break;                                      // :217-220
+      case LAPSE_DAMPED_WAVE:                                                                  // :151-155
+        lapse = alpha * alpha * ((TC)a.dw_mu_l * (TC(12) * phi * (TC)a.dw_p - log(alpha)) - K);
+        break;
+      default: break;  // Static, TestAijDriverLapse
+    }
     rhs[ALPHA] = lapse - ko[ALPHA];
 
     // ------------------------------------------------ RK4Register epilogue

@@ -422,8 +422,7 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   // the dust fluid evolves with an empty or full FRW reference (dust_ic.cc),
   // so its fields are not small perturbations: FP64 storage only
   if (p.matter == CG_MATTER_DUST && p.precision != CG_FP64) return CG_ERR_ARG;
-  if (p.lapse != CG_LAPSE_STATIC && p.lapse != CG_LAPSE_HARMONIC && p.lapse != CG_LAPSE_CONFORMAL_FLRW)
-    return CG_ERR_ARG;
+  if (p.lapse < 0 || p.lapse >= LAPSE_N) return CG_ERR_ARG;
   if (!(p.dx > 0) || !std::isfinite(p.dt)) return CG_ERR_ARG;
   cg_ctx* c = new cg_ctx();
   c->p = p;
@@ -608,6 +607,12 @@ int cg_set_lambda(cg_ctx* c, double lambda) {
   return CG_OK;
 }
 
+int cg_set_gauge_params(cg_ctx* c, double gd_c, double dw_mu_l, double dw_p) {
+  if (!c) return CG_ERR_ARG;
+  c->gd_c = gd_c; c->dw_mu_l = dw_mu_l; c->dw_p = dw_p;
+  return CG_OK;
+}
+
 namespace cg {
 StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int normalize) {
   StageArgs a{};
@@ -617,6 +622,7 @@ StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int
   a.s_in = rin; a.s_p = rp; a.s_f = rf; a.s_out = rout;
   a.DIFFD = c->DIFFD;
   a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
+  a.tile_order = c->tile_order;
 
   a.stage = stage;
   a.normalize = normalize;
@@ -633,7 +639,8 @@ StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int
   a.ca = (stage == 3) ? dt : dt / 2.0;
   a.phi_F = c->frw.g_phi; a.K_F = c->frw.g_K; a.rho_F = c->frw.g_rho; a.S_F = c->frw.g_S;
   a.ext_in = c->ext;
-  a.ext_part = c->p.lapse == CG_LAPSE_CONFORMAL_FLRW ? c->ext_part : nullptr;
+  a.ext_part = lapse_needs_kavg(c->p.lapse) ? c->ext_part : nullptr;
+  a.gd_c = c->gd_c; a.dw_mu_l = c->dw_mu_l; a.dw_p = c->dw_p;
   a.phi_F_next = c->frw.g_phi;
   a.lambda = c->lambda;
   a.dreg = c->dreg;
@@ -834,7 +841,7 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
                        (c->p.stencil_order == 2 || c->p.stencil_order == 8);
   // one fused stage: sources and RHS with the current FRW values, then the
   // FRW stage step (bssn.cc:269), whose phi weights the averages of the output
-  const bool flrw = c->p.lapse == CG_LAPSE_CONFORMAL_FLRW;
+  const bool flrw = lapse_needs_kavg(c->p.lapse);  // averages needed inside the step
   SrcArgs src4{};
   // dust: the sources of stage n use W, detg, g_ij of stage n-1's input
   int prev_in = 0, prev_norm = 0;
@@ -1034,28 +1041,34 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
   HIPCHK(c, hipEventCreate(&e1));
   const bool timing = c->timing;
   c->timing = false;
+  int best_order = 0;
   for (const auto& v : stage_variants()) {
     if (!variant_matches(c, v)) continue;
     c->var = &v;
-    StageArgs a = make_args(c, 2, 0, 0, 1, 3, 0);
-    int rc = launch(c, a);
-    if (rc) { c->timing = timing; return rc; }
-    HIPCHK(c, hipEventRecord(e0, c->stream));
-    for (int r = 0; r < reps; ++r)
-      if ((rc = launch(c, a))) { c->timing = timing; return rc; }
-    HIPCHK(c, hipEventRecord(e1, c->stream));
-    HIPCHK(c, hipEventSynchronize(e1));
-    float ms = 0;
-    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-    const double avg = ms / reps;
-    int vg = 0, sc = 0, ld = 0;
-    v.info(&vg, &sc, &ld);
-    char buf[256];
-    snprintf(buf, sizeof buf, "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"ms\":%.4f,\"vgprs\":%d,\"scratch\":%d}",
-             rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, avg, vg, sc);
-    rep += buf;
-    if (avg < best_ms) { best_ms = avg; best = &v; }
+    for (int order = 0; order < 3; ++order) {
+      c->tile_order = order;
+      StageArgs a = make_args(c, 2, 0, 0, 1, 3, 0);
+      int rc = launch(c, a);
+      if (rc) { c->timing = timing; return rc; }
+      HIPCHK(c, hipEventRecord(e0, c->stream));
+      for (int r = 0; r < reps; ++r)
+        if ((rc = launch(c, a))) { c->timing = timing; return rc; }
+      HIPCHK(c, hipEventRecord(e1, c->stream));
+      HIPCHK(c, hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+      const double avg = ms / reps;
+      int vg = 0, sc = 0, ld = 0;
+      v.info(&vg, &sc, &ld);
+      char buf[256];
+      snprintf(buf, sizeof buf,
+               "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"order\":%d,\"ms\":%.4f,\"vgprs\":%d,\"scratch\":%d}",
+               rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, order, avg, vg, sc);
+      rep += buf;
+      if (avg < best_ms) { best_ms = avg; best = &v; best_order = order; }
+    }
   }
+  c->tile_order = best_order;
   rep += "]";
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

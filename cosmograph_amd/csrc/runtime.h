@@ -101,6 +101,8 @@ struct cg_ctx {
   int R = 2, H = 2;
   int req_ti = 0, req_tj = 0, req_tk = 0;  // requested tile (0 = automatic)
   const StageVariant* var = nullptr;     // stage kernel in use
+  int tile_order = 0;                     // tile traversal order (autotuned)
+  double gd_c = 1.0, dw_mu_l = 0.0, dw_p = 0.0;  // gauge parameters (BSSNGaugeHandler.h:139-149)
   bool alpha_nonzero = false;             // a non-zero DIFFalpha was uploaded
   double* cons = nullptr;                 // [CONS_N][pad] per-point constraint values (lazy)
   double* cons_part = nullptr;            // reduction partials

@@ -22,7 +22,8 @@ FIELDS = ["DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIFFgamma22", "DIFFgamma
           "Gamma1", "Gamma2", "Gamma3", "DIFFalpha"]
 PRECISION = {"fp64": 0, "fp32_split": 1}
 MATTER = {"none": 0, "vacuum": 0, "static": 1, "dust": 2}
-LAPSE = {"": 0, "Static": 0, "Harmonic": 1, "ConformalFLRW": 2}
+LAPSE = {"": 0, "Static": 0, "Harmonic": 1, "ConformalFLRW": 2, "Anharmonic": 3, "OnePlusLog": 4,
+         "ConformalSync": 5, "AwAGaugeWave": 6, "AwAShiftedWave": 7, "DampedWave": 8, "TestAijDriverLapse": 9}
 ERR_NAN = -10
 
 
@@ -60,6 +61,9 @@ class GpuParams:
     lambda_: float = 0.0
     device: int = 0
     tile: tuple | None = None
+    gd_c: float = 1.0           # gauge parameters (BSSNGaugeHandler.h:139-149)
+    dw_mu_l: float = 0.0
+    dw_p: float = 0.0
 
     def shape(self):
         return (self.n, self.n, self.n) if np.isscalar(self.n) else tuple(self.n)
@@ -101,6 +105,7 @@ def load_library():
         "cg_frw_add_fluid": ([vp, C.c_double, C.c_double], C.c_int),
         "cg_frw_get": ([vp, dp], C.c_int),
         "cg_set_lambda": ([vp, C.c_double], C.c_int),
+        "cg_set_gauge_params": ([vp, C.c_double, C.c_double, C.c_double], C.c_int),
         "cg_step_init": ([vp], C.c_int),
         "cg_rk_evolve": ([vp], C.c_int),
         "cg_k_finalize": ([vp, C.c_int], C.c_int),
@@ -157,6 +162,7 @@ class BSSN:
                 self.L.cg_destroy(h)
             self.h = None
             raise CosmoGpuError(rc, msg)
+        self._chk(self.L.cg_set_gauge_params(self.h, params.gd_c, params.dw_mu_l, params.dw_p))
 
     def __del__(self):
         if getattr(self, "h", None) and getattr(self, "_owner", None) is None:

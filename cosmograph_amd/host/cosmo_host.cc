@@ -104,10 +104,15 @@ cg_params params_from_config(const ConfigParser& cfg) {
   p.k_damping_amp = std::stod(cfg("k_damping_amp", "0.0"));
   p.normalize_metric = std::stoi(cfg("normalize_metric", "1"));
   const std::string lapse = cfg("lapse", "");
-  if (lapse == "" || lapse == "Static") p.lapse = CG_LAPSE_STATIC;
-  else if (lapse == "Harmonic") p.lapse = CG_LAPSE_HARMONIC;
-  else if (lapse == "ConformalFLRW") p.lapse = CG_LAPSE_CONFORMAL_FLRW;
-  else {
+  static const std::map<std::string, int> kLapse = {
+      {"", CG_LAPSE_STATIC}, {"Static", CG_LAPSE_STATIC}, {"Harmonic", CG_LAPSE_HARMONIC},
+      {"ConformalFLRW", CG_LAPSE_CONFORMAL_FLRW}, {"Anharmonic", CG_LAPSE_ANHARMONIC},
+      {"OnePlusLog", CG_LAPSE_ONE_PLUS_LOG}, {"ConformalSync", CG_LAPSE_CONFORMAL_SYNC},
+      {"AwAGaugeWave", CG_LAPSE_AWA_GAUGE_WAVE}, {"AwAShiftedWave", CG_LAPSE_AWA_SHIFTED_WAVE},
+      {"DampedWave", CG_LAPSE_DAMPED_WAVE}, {"TestAijDriverLapse", CG_LAPSE_TEST_AIJ_DRIVER}};
+  if (kLapse.count(lapse)) {
+    p.lapse = kLapse.at(lapse);
+  } else {
     std::cerr << "Error - lapse `" << lapse << "` is not available on the GPU path.\n";
     throw -1;
   }
@@ -131,7 +136,11 @@ cg_params params_from_config(const ConfigParser& cfg) {
 }
 
 // -------------------------------------------------------------------- BSSN
-BSSN::BSSN(const ConfigParser& cfg) : BSSN(params_from_config(cfg)) {}
+BSSN::BSSN(const ConfigParser& cfg) : BSSN(params_from_config(cfg)) {
+  // BSSNGaugeHandler::_initDefaultParameters (BSSNGaugeHandler.h:139-149)
+  check(cg_set_gauge_params(ctx, std::stod(cfg("gd_c", "1.0")), std::stod(cfg("dw_mu_l", "0.0")),
+                            std::stod(cfg("dw_p", "0.0"))));
+}
 
 BSSN::BSSN(const cg_params& pin) : p(pin) {
   int rc = cg_create(&p, &ctx);

@@ -30,7 +30,12 @@ typedef struct cg_ctx cg_ctx;
 enum { CG_OK = 0, CG_ERR_ARG = -1, CG_ERR_HIP = -2, CG_ERR_STATE = -3, CG_ERR_NAN = -10 };
 enum { CG_FP64 = 0, CG_FP32_SPLIT = 1 };
 enum { CG_MATTER_NONE = 0, CG_MATTER_STATIC = 1, CG_MATTER_DUST = 2 };
-enum { CG_LAPSE_STATIC = 0, CG_LAPSE_HARMONIC = 1, CG_LAPSE_CONFORMAL_FLRW = 2 };
+/* the shift-free lapse gauges of BSSNGaugeHandler (BSSNGaugeHandler.h:96-111) */
+enum {
+  CG_LAPSE_STATIC = 0, CG_LAPSE_HARMONIC = 1, CG_LAPSE_CONFORMAL_FLRW = 2, CG_LAPSE_ANHARMONIC = 3,
+  CG_LAPSE_ONE_PLUS_LOG = 4, CG_LAPSE_CONFORMAL_SYNC = 5, CG_LAPSE_AWA_GAUGE_WAVE = 6,
+  CG_LAPSE_AWA_SHIFTED_WAVE = 7, CG_LAPSE_DAMPED_WAVE = 8, CG_LAPSE_TEST_AIJ_DRIVER = 9
+};
 
 /* Construction parameters.  Mirrors the compile-time macros and config keys
  * the reference reads: COSMO_N/NX/NY/NZ and STENCIL_ORDER
@@ -73,6 +78,10 @@ int cg_frw_add_fluid(cg_ctx* ctx, double rho, double w);
 int cg_frw_get(const cg_ctx* ctx, double out_phi_K_rho_S[4]);
 /* Lambda::setLambda (components/Lambda/lambda.cc:16-19) */
 int cg_set_lambda(cg_ctx* ctx, double lambda);
+/* Gauge parameters of BSSNGaugeHandler::_initDefaultParameters
+ * (BSSNGaugeHandler.h:139-149): gd_c (OnePlusLog, default 1), dw_mu_l and
+ * dw_p (DampedWave, default 0). */
+int cg_set_gauge_params(cg_ctx* ctx, double gd_c, double dw_mu_l, double dw_p);
 
 /* Method-level drop-ins (unfused; each streams the grid once):
  * BSSN::stepInit (bssn.cc:219-226), BSSN::RKEvolve with the matter source

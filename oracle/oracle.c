@@ -522,8 +522,19 @@ void or_point_compute(const or_params *p, or_point *bd) {
                     bd->ko[F_PHI]);
   /* gauge: BSSNGaugeHandler.cc:16-35 */
   double lapse = 0.0;
-  if (p->lapse == OR_LAPSE_HARMONIC) lapse = -1.0 * alpha * (bd->K);
-  else if (p->lapse == OR_LAPSE_CONFORMAL_FLRW) lapse = -1.0 / 3.0 * alpha * bd->K_avg;
+  switch (p->lapse) {
+  case OR_LAPSE_HARMONIC: lapse = -1.0 * alpha * (bd->K); break;                        /* :32-35 */
+  case OR_LAPSE_CONFORMAL_FLRW: lapse = -1.0 / 3.0 * alpha * bd->K_avg; break;          /* :24-27 */
+  case OR_LAPSE_ANHARMONIC: lapse = 1.0 * PW2(alpha) * (bd->K - bd->K_avg); break;      /* :50-58 */
+  case OR_LAPSE_ONE_PLUS_LOG: lapse = -2.0 * alpha * (bd->K - bd->K_avg) * p->gd_c; break; /* :64-68, beta = 0 */
+  case OR_LAPSE_CONFORMAL_SYNC: lapse = -1.0 / 3.0 * alpha * bd->K_F; break;            /* :84-87 */
+  case OR_LAPSE_AWA_GAUGE_WAVE: lapse = -1.0 * PW2(alpha) * v[F_K]; break;              /* :209-212 */
+  case OR_LAPSE_AWA_SHIFTED_WAVE: lapse = -1.0 * v[F_K]; break;                         /* :217-220 */
+  case OR_LAPSE_DAMPED_WAVE:                                                            /* :151-155, beta = 0 */
+    lapse = PW2(alpha) * (p->dw_mu_l * (12.0 * bd->phi * p->dw_p - log(alpha)) - bd->K);
+    break;
+  default: lapse = 0.0; break;                                  /* Static, TestAijDriverLapse */
+  }
   bd->rhs[F_ALPHA] = lapse - bd->ko[F_ALPHA];
 #undef GI
 #undef GG

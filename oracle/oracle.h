@@ -34,7 +34,13 @@ extern "C" {
 
 enum { OR_P = 0, OR_A = 1, OR_C = 2, OR_F = 3 };
 enum { OR_MATTER_NONE = 0, OR_MATTER_STATIC = 1, OR_MATTER_DUST = 2 };
-enum { OR_LAPSE_STATIC = 0, OR_LAPSE_HARMONIC = 1, OR_LAPSE_CONFORMAL_FLRW = 2 };
+/* lapse gauges of BSSNGaugeHandler (components/bssn/BSSNGaugeHandler.h:96-111)
+ * that need no shift */
+enum {
+  OR_LAPSE_STATIC = 0, OR_LAPSE_HARMONIC = 1, OR_LAPSE_CONFORMAL_FLRW = 2, OR_LAPSE_ANHARMONIC = 3,
+  OR_LAPSE_ONE_PLUS_LOG = 4, OR_LAPSE_CONFORMAL_SYNC = 5, OR_LAPSE_AWA_GAUGE_WAVE = 6,
+  OR_LAPSE_AWA_SHIFTED_WAVE = 7, OR_LAPSE_DAMPED_WAVE = 8, OR_LAPSE_TEST_AIJ_DRIVER = 9
+};
 
 typedef struct {
   int nx, ny, nz;
@@ -47,6 +53,8 @@ typedef struct {
   int lapse;
   int matter;
   double lambda;          /* Lambda::lambda (rho_Lambda) */
+  double gd_c;            /* gauge parameters (BSSNGaugeHandler.h:139-149) */
+  double dw_mu_l, dw_p;
 } or_params;
 
 /* Per-point data: the oracle's analogue of BSSNData.  Inputs first (filled

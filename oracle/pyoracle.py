@@ -26,7 +26,9 @@ FIELDS = ["DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIFFgamma22", "DIFFgamma
           "Gamma1", "Gamma2", "Gamma3", "DIFFalpha"]
 SOURCES = ["DIFFr", "DIFFS", "S1", "S2", "S3", "STF11", "STF12", "STF13", "STF22", "STF23", "STF33"]
 MATTER = {"none": 0, "static": 1, "dust": 2}
-LAPSE = {"Static": 0, "Harmonic": 1, "ConformalFLRW": 2}
+LAPSE = {"Static": 0, "Harmonic": 1, "ConformalFLRW": 2, "Anharmonic": 3, "OnePlusLog": 4,
+         "ConformalSync": 5, "AwAGaugeWave": 6, "AwAShiftedWave": 7, "DampedWave": 8,
+         "TestAijDriverLapse": 9}
 
 
 class Params(C.Structure):
@@ -34,7 +36,8 @@ class Params(C.Structure):
                 ("dx", C.c_double), ("dt", C.c_double), ("ko_coeff", C.c_double),
                 ("a_adj_amp", C.c_double), ("k_damping_amp", C.c_double),
                 ("normalize_metric", C.c_int), ("lapse", C.c_int), ("matter", C.c_int),
-                ("lambda_", C.c_double)]
+                ("lambda_", C.c_double), ("gd_c", C.c_double), ("dw_mu_l", C.c_double),
+                ("dw_p", C.c_double)]
 
 
 def _point_fields():
@@ -106,11 +109,12 @@ def lib():
 
 
 def make_params(n, order=4, dx=None, dt_frac=0.1, ko=0.0, a_adj=0.0, k_damp=0.0,
-                normalize=1, lapse="Static", matter="static", lam=0.0, box=1.0):
+                normalize=1, lapse="Static", matter="static", lam=0.0, box=1.0, gd_c=1.0,
+                dw_mu_l=0.0, dw_p=0.0):
     nx, ny, nz = (n, n, n) if np.isscalar(n) else n
     dx = box / nx if dx is None else dx
     return Params(nx, ny, nz, order, dx, dt_frac * dx, ko, a_adj, k_damp, normalize,
-                  LAPSE[lapse], MATTER[matter], lam)
+                  LAPSE[lapse], MATTER[matter], lam, gd_c, dw_mu_l, dw_p)
 
 
 class Sim:

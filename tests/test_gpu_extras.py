@@ -135,3 +135,25 @@ def test_num_nans_and_nan_guard(oracle_mod, gpu):
         g.step(1)
     assert ei.value.code == gpu.ERR_NAN
     assert g.numNaNs("DIFFphi_p") > 0
+
+
+@pytest.mark.parametrize("lapse", ["Anharmonic", "OnePlusLog", "ConformalSync", "AwAGaugeWave",
+                                   "AwAShiftedWave", "DampedWave", "TestAijDriverLapse"])
+def test_lapse_gauges_vs_oracle(oracle_mod, gpu, lapse):
+    """The shift-free lapse gauges of BSSNGaugeHandler on a generic state
+    (K_avg read from the device every stage for Anharmonic and OnePlusLog)."""
+    shape = (16, 14, 18)
+    _say(f"case lapse {lapse}")
+    st = smooth_random_state(shape, seed=47)
+    o, g = setup_pair(oracle_mod, gpu, {"n": shape, "order": 4, "lapse": lapse}, state=st)
+    if lapse in ("OnePlusLog", "DampedWave"):  # non-default gauge parameters
+        o = oracle_mod.Sim(oracle_mod.make_params(shape, order=4, lapse=lapse, gd_c=0.7, dw_mu_l=0.3, dw_p=0.5))
+        for f in FIELDS:
+            o.field(f + "_p")[:] = st[f]
+        o.field("DIFFD_a")[:] = st["DIFFD"]
+        o.frw_set(0.0, -3.0)
+        o.frw_add_fluid(3.0 / (8.0 * np.pi), 0.0)
+        g.L.cg_set_gauge_params(g.h, 0.7, 0.3, 0.5)
+    o.run_steps(3)
+    g.step(3)
+    _check_fields(g, o, "p", 1e-12)

@@ -59,7 +59,7 @@ steps = 3
 
 
 @pytest.mark.parametrize("text,code", [
-    ("N = 16\nlapse = OnePlusLog\nsteps = 1\n", 1),      # not on this path: throw -1
+    ("N = 16\nlapse = GeneralizedNewton\nsteps = 1\n", 1),  # not on this path: throw -1
     ("N = 16\nshift = GammaDriver\nsteps = 1\n", 1),
     ("steps = 1\n", 1),                                  # N is required: throw -1
 ])

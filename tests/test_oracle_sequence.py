@@ -100,3 +100,55 @@ def test_homogeneous_conformal_flrw_lapse_matches_rk4(oracle_mod):
         assert ex["K_min"] == s.field("DIFFK_p").min()
         assert abs(ex["K_avg"] - dK) <= 1e-13 * abs(dK)
         assert abs(ex["avg_vol"] - math.exp(6 * dphi)) <= 1e-13
+
+
+# d_t DIFFalpha of the shift-free lapse gauges (BSSNGaugeHandler.cc), as
+# functions of alpha, the full K, DIFFK, K_avg (= DIFFK on uniform data),
+# K_FRW and the full phi
+LAPSES = {
+    "Harmonic": lambda a, K, dK, Kav, KF, ph: -a * K,
+    "Anharmonic": lambda a, K, dK, Kav, KF, ph: a * a * (K - Kav),
+    "OnePlusLog": lambda a, K, dK, Kav, KF, ph: -2.0 * a * (K - Kav) * 0.7,        # gd_c = 0.7
+    "ConformalSync": lambda a, K, dK, Kav, KF, ph: -a * KF / 3.0,
+    "AwAGaugeWave": lambda a, K, dK, Kav, KF, ph: -a * a * dK,
+    "AwAShiftedWave": lambda a, K, dK, Kav, KF, ph: -dK,
+    "DampedWave": lambda a, K, dK, Kav, KF, ph: a * a * (0.3 * (12.0 * ph * 0.5 - math.log(a)) - K),
+    "TestAijDriverLapse": lambda a, K, dK, Kav, KF, ph: 0.0,
+}
+
+
+@pytest.mark.parametrize("lapse", sorted(LAPSES))
+def test_homogeneous_lapse_gauges_match_rk4(oracle_mod, lapse):
+    """Every lapse gauge on uniform data: classical RK4 of the background +
+    perturbation ODE with d_t DIFFalpha from the gauge.  Pins which K each
+    gauge reads (full K, DIFFK, the averaged perturbation, K_FRW), the
+    gauge parameters and that every stage sees its own lapse and K_avg."""
+    n = 8
+    rho0 = 3.0 / (8.0 * math.pi)
+    s = oracle_mod.Sim(oracle_mod.make_params(n, order=4, lapse=lapse, gd_c=0.7, dw_mu_l=0.3, dw_p=0.5))
+    dphi0, dK0, da0, D = 2e-3, -5e-3, 1e-3, 4e-3
+    s.field("DIFFphi_p")[:] = dphi0
+    s.field("DIFFK_p")[:] = dK0
+    s.field("DIFFalpha_p")[:] = da0
+    s.field("DIFFD_a")[:] = D
+    s.frw_set(0.0, -3.0)
+    s.frw_add_fluid(rho0, 0.0)
+    h = s.p.dt
+    fl = LAPSES[lapse]
+
+    def f(y):
+        phiF, KF, rhoF, dphi, dK, da = y
+        alpha = 1.0 + da
+        DIFFr = math.exp(-6 * (dphi + phiF)) * D + rhoF * math.expm1(-6 * dphi)
+        return np.array([-KF / 6, KF * KF / 3 + 4 * math.pi * rhoF, KF * rhoF,
+                         -(alpha * dK + da * KF) / 6,
+                         alpha * dK * (dK + 2 * KF) / 3 + 4 * math.pi * alpha * DIFFr + 4 * math.pi * da * rhoF,
+                         fl(alpha, dK + KF, dK, dK, KF, dphi + phiF)])
+
+    want = rk4_reference([0.0, -3.0, rho0, dphi0, dK0, da0], f, h, 6)
+    for step in range(6):
+        s.run_steps(1)
+        phiF, KF, rhoF, _ = s.frw_get()
+        got = [phiF, KF, rhoF, s.field("DIFFphi_p")[1, 2, 3], s.field("DIFFK_p")[4, 0, 7],
+               s.field("DIFFalpha_p")[5, 5, 1]]
+        np.testing.assert_allclose(got, want[step], rtol=1e-12, atol=1e-17)
