@@ -31,6 +31,7 @@ struct cg_batch {
   hipStream_t stream = nullptr;
   const StageVariant* var = nullptr;
   int req_ti = 0, req_tj = 0, req_tk = 0;
+  bool dma_allowed = true, dma_pick = true;  // LDS-DMA halo staging (as cg_ctx)
   // device tables: stage arguments [4][n], first tile of each grid [n],
   // FRW values [steps][4][n][FRW_N]
   StageArgs* tab = nullptr;
@@ -47,6 +48,7 @@ struct cg_batch {
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
   const StageVariant* graph_var = nullptr;
+  bool graph_dma = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double last_ms = 0.0;
   std::string err;
@@ -74,6 +76,21 @@ static bool batch_fits(const cg_batch* b, const StageVariant& v) {
   for (auto* c : b->m)
     if (!variant_fits(c, v)) return false;
   return true;
+}
+
+// the LDS-DMA twin of v applies to every grid of the batch
+static bool batch_dma_fits(const cg_batch* b, const StageVariant& v) {
+  if (!b->dma_allowed || !b->dma_pick || !v.launch_batch_dma) return false;
+  for (auto* c : b->m)
+    if (!dma_fits(c, v)) return false;
+  return true;
+}
+
+// stage 1 may use the DMA twin only when no grid normalises its metric there
+static bool batch_norm_stage1(const cg_batch* b) {
+  for (auto* c : b->m)
+    if (c->p.normalize_metric) return true;
+  return false;
 }
 
 static const StageVariant* batch_pick(const cg_batch* b) {
@@ -170,6 +187,13 @@ int cg_batch_member(cg_batch* b, int r, cg_ctx** out) {
   return CG_OK;
 }
 
+int cg_batch_set_dma(cg_batch* b, int on) {
+  if (!b) return CG_ERR_ARG;
+  b->dma_allowed = on != 0;
+  drop_graph(b);
+  return CG_OK;
+}
+
 int cg_batch_set_graph(cg_batch* b, int on) {
   if (!b) return CG_ERR_ARG;
   b->use_graph = on != 0;
@@ -220,6 +244,7 @@ static int ensure_frw(cg_batch* b, int nsteps) {
 
 static int enqueue_steps(cg_batch* b, const StageVariant* v, int nsteps) {
   const int n = (int)b->m.size();
+  const bool dma = batch_dma_fits(b, *v), norm1 = batch_norm_stage1(b);
   for (int s = 0; s < nsteps; ++s)
     for (int st = 1; st <= 4; ++st) {
       BatchArgs ba{};
@@ -229,7 +254,7 @@ static int enqueue_steps(cg_batch* b, const StageVariant* v, int nsteps) {
       ba.nreal = n;
       ba.total = b->total;
       ba.per_xcd = (b->total + 7) / 8;
-      int rc = v->launch_batch(ba, b->stream);
+      int rc = (dma && !(st == 1 && norm1) ? v->launch_batch_dma : v->launch_batch)(ba, b->stream);
       if (rc) return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
     }
   return CG_OK;
@@ -270,7 +295,8 @@ int cg_batch_step(cg_batch* b, int nsteps) {
 
   BCHK(b, hipEventRecord(b->e0, b->stream));
   if (b->use_graph) {
-    if (!b->graph || b->graph_steps != nsteps || b->graph_var != v) {
+    const bool dma = batch_dma_fits(b, *v);
+    if (!b->graph || b->graph_steps != nsteps || b->graph_var != v || b->graph_dma != dma) {
       drop_graph(b);
       hipGraph_t g = nullptr;
       BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -283,6 +309,7 @@ int cg_batch_step(cg_batch* b, int nsteps) {
       BCHK(b, ei);
       b->graph_steps = nsteps;
       b->graph_var = v;
+      b->graph_dma = dma;
     }
     BCHK(b, hipGraphLaunch(b->graph, b->stream));
   } else {
@@ -354,45 +381,53 @@ int cg_batch_autotune(cg_batch* b, int reps, char* report, size_t report_len) {
   const StageVariant* best = nullptr;
   double best_ms = 1e30;
   int best_order = 0;
+  bool best_dma = false;
   for (const auto& v : stage_variants()) {
     if (!batch_fits(b, v)) continue;
-    for (int order = 0; order < 3; ++order) {
-      for (auto* c : b->m) c->tile_order = order;
-      if (int rc = build_tables(b, &v)) return rc;
-      for (int r = 0; r < n; ++r) {  // retarget stage 2 to slots 0 -> 3
-        StageArgs& a = b->tab_h[1 * n + r];
-        a.s_in = 0; a.s_out = 3;
+    for (int d = 0; d < 2; ++d) {
+      b->dma_pick = d == 1;
+      if (d == 1 && !batch_dma_fits(b, v)) continue;
+      for (int order = 0; order < 3; ++order) {
+        for (auto* c : b->m) c->tile_order = order;
+        if (int rc = build_tables(b, &v)) return rc;
+        for (int r = 0; r < n; ++r) {  // retarget stage 2 to slots 0 -> 3
+          StageArgs& a = b->tab_h[1 * n + r];
+          a.s_in = 0; a.s_out = 3;
+        }
+        BCHK(b, hipMemcpyAsync(b->tab + n, b->tab_h + n, sizeof(StageArgs) * n, hipMemcpyHostToDevice, b->stream));
+        BatchArgs ba{};
+        ba.tab = b->tab + n;
+        ba.tstart = b->tstart;
+        ba.frw = b->frw;
+        ba.nreal = n;
+        ba.total = b->total;
+        ba.per_xcd = (b->total + 7) / 8;
+        auto go = d ? v.launch_batch_dma : v.launch_batch;
+        int rc = go(ba, b->stream);  // warm-up
+        if (rc) return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
+        BCHK(b, hipEventRecord(b->e0, b->stream));
+        for (int q = 0; q < reps; ++q)
+          if ((rc = go(ba, b->stream)))
+            return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
+        BCHK(b, hipEventRecord(b->e1, b->stream));
+        BCHK(b, hipEventSynchronize(b->e1));
+        float ms = 0.f;
+        BCHK(b, hipEventElapsedTime(&ms, b->e0, b->e1));
+        const double avg = ms / reps;
+        int vg = 0, sc = 0, ld = 0;
+        (d ? v.info_dma : v.info)(&vg, &sc, &ld);
+        char buf[256];
+        snprintf(buf, sizeof buf,
+                 "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"dma\":%d,\"order\":%d,\"ms\":%.4f,\"vgprs\":%d,"
+                 "\"scratch\":%d}",
+                 rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, d, order, avg, vg, sc);
+        rep += buf;
+        if (avg < best_ms) { best_ms = avg; best = &v; best_order = order; best_dma = d == 1; }
       }
-      BCHK(b, hipMemcpyAsync(b->tab + n, b->tab_h + n, sizeof(StageArgs) * n, hipMemcpyHostToDevice, b->stream));
-      BatchArgs ba{};
-      ba.tab = b->tab + n;
-      ba.tstart = b->tstart;
-      ba.frw = b->frw;
-      ba.nreal = n;
-      ba.total = b->total;
-      ba.per_xcd = (b->total + 7) / 8;
-      int rc = v.launch_batch(ba, b->stream);  // warm-up
-      if (rc) return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
-      BCHK(b, hipEventRecord(b->e0, b->stream));
-      for (int q = 0; q < reps; ++q)
-        if ((rc = v.launch_batch(ba, b->stream)))
-          return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
-      BCHK(b, hipEventRecord(b->e1, b->stream));
-      BCHK(b, hipEventSynchronize(b->e1));
-      float ms = 0.f;
-      BCHK(b, hipEventElapsedTime(&ms, b->e0, b->e1));
-      const double avg = ms / reps;
-      int vg = 0, sc = 0, ld = 0;
-      v.info(&vg, &sc, &ld);
-      char buf[256];
-      snprintf(buf, sizeof buf,
-               "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"order\":%d,\"ms\":%.4f,\"vgprs\":%d,\"scratch\":%d}",
-               rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, order, avg, vg, sc);
-      rep += buf;
-      if (avg < best_ms) { best_ms = avg; best = &v; best_order = order; }
     }
   }
   for (auto* c : b->m) c->tile_order = best_order;
+  b->dma_pick = best_dma || !best;
   rep += "]";
   if (report && report_len) {
     strncpy(report, rep.c_str(), report_len - 1);
@@ -411,7 +446,7 @@ int cg_batch_kernel_info(const cg_batch* bc, int* vgprs, int* scratch, int* ti, 
   if (!b->var || !batch_fits(b, *b->var)) b->var = batch_pick(b);
   if (!b->var) return bfail(b, CG_ERR_ARG, "no batched stage kernel variant fits every grid");
   int lds = 0;
-  if (b->var->info(vgprs, scratch, &lds)) return CG_ERR_HIP;
+  if ((batch_dma_fits(b, *b->var) ? b->var->info_dma : b->var->info)(vgprs, scratch, &lds)) return CG_ERR_HIP;
   *ti = b->var->ti; *tj = b->var->tj; *tk = b->var->tk;
   return CG_OK;
 }

@@ -156,6 +156,11 @@ struct StageVariant {
   int (*launch)(const StageArgs&, hipStream_t);
   int (*info)(int* vgprs, int* scratch, int* lds);
   int (*launch_batch)(const BatchArgs&, hipStream_t);  // null: no batched kernel compiled
+  // the same tile with LDS-DMA halo staging (null: not compiled); used for
+  // launches without stage-1 normalisation on grids with nz % 4 == 0
+  int (*launch_dma)(const StageArgs&, hipStream_t);
+  int (*info_dma)(int* vgprs, int* scratch, int* lds);
+  int (*launch_batch_dma)(const BatchArgs&, hipStream_t);
 };
 
 std::vector<StageVariant>& stage_variants();

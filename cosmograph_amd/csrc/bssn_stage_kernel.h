@@ -23,6 +23,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bssn_stage.h"
 #include "stencil.h"
 
@@ -104,6 +106,55 @@ __device__ __host__ constexpr int stencil_seq(int s) {
   return A11 + (s - 11 < 5 ? s - 11 : 5);
 }
 
+// Staging order of the DMA path in split precision.  Position s is staged in
+// LDS buffer s % 3, and buffer 1 is FP32-sized, so every FP64 (metric) field
+// sits at a position s % 3 != 1:
+// PHI, KK, G11, G12, GAM1, G13, G22, GAM2, G23, G33, GAM3, [ALPHA], A11..A33.
+template <bool LA>
+__device__ __host__ constexpr int stencil_seq_split(int s) {
+  constexpr int head[11] = {PHI, KK, G11, G12, GAM1, G13, G22, GAM2, G23, G33, GAM3};
+  if (s < 11) return head[s];
+  if (LA) {
+    if (s == 11) return ALPHA;
+    s -= 1;
+  }
+  return A11 + (s - 11 < 5 ? s - 11 : 5);
+}
+
+// LDS DMA of 16 bytes per lane: lane l copies 16 bytes at base + voff into
+// LDS byte lds + 16*l (global_load_lds_dwordx4; lds and base wave-uniform,
+// 16-byte aligned).  Written as asm so that the compiler does not track these
+// loads: its own LDS wait analysis would otherwise wait for every
+// outstanding DMA before the first read of any tile.  The caller waits for
+// them explicitly (dma_wait_barrier).
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"  // m0 is reserved: set and used inside the statement
+__device__ __forceinline__ void dma16(const void* base, unsigned voff, unsigned lds) {
+  // base and lds are wave-uniform; readfirstlane puts them in SGPRs where
+  // the compiler holds them in VGPRs (it folds away for SGPR values)
+  // (readfirstlane returns int: each half goes through unsigned before it
+  // is widened, or a low half with bit 31 set would sign-extend into the
+  // high half)
+  const unsigned long long bb = (unsigned long long)base;
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(bb >> 32));
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)bb);
+  const unsigned long long bs = ((unsigned long long)hi << 32) | (unsigned long long)lo;
+  const unsigned ls = (unsigned)__builtin_amdgcn_readfirstlane((int)lds);
+  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(bs), "s"(ls)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+// End of one step of the DMA staging: this wave's vector memory operations
+// older than the newest K have completed (they retire in order), then the
+// block barrier.  No fence: the DMA loads of the next fields stay in flight
+// across it.
+template <int K>
+__device__ __forceinline__ void dma_wait_barrier() {
+  static_assert(K >= 0 && K < 64, "vmcnt range");
+  asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(K) : "memory");
+}
+
 // WPE: minimum waves per SIMD the register allocation must allow.
 // LIVE_ALPHA = false: synchronous gauge with DIFFalpha identically zero (Static
 // lapse, zero initial lapse perturbation).  Every lapse derivative is then an
@@ -117,13 +168,24 @@ __device__ __host__ constexpr int stencil_seq(int s) {
 // stage_body is one tile t (< a.ntx*a.nty*a.ntz) of one grid; fv holds the
 // FRW values of the stage.  Every thread of the block must call it (it holds
 // barriers).
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, bool LIVE_ALPHA, int MODE>
+//
+// DMA = true: the halo tiles go from memory straight into LDS
+// (global_load_lds, 16 bytes per lane), two fields ahead of the one being
+// differentiated, in three LDS buffers.  No registers hold the loads in
+// flight.  Requires nz % 4 == 0 and no stage-1 normalisation (a.normalize is
+// ignored); the launcher falls back to DMA = false otherwise.
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, bool LIVE_ALPHA, int MODE, bool DMA = false>
 __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, const FrwV fv) {
   constexpr int NT = TI * TJ * TK;
-  constexpr int HJ = TJ + 2 * H, HK = TK + 2 * H, HI = TI + 2 * H;
+  // halo along k: H, or for DMA staging H rounded up to a multiple of 4, so
+  // that every 16-byte group of a halo row (2 FP64 or 4 FP32 elements)
+  // starts at z = 0 mod 4 and stays contiguous across the periodic wrap
+  constexpr int KH = DMA ? (H + 3) / 4 * 4 : H;
+  constexpr int HJ = TJ + 2 * H, HK = TK + 2 * KH, HI = TI + 2 * H;
   constexpr int HV = HI * HJ * HK;
   constexpr int SI = HJ * HK, SJ = HK, SK = 1;
-  // The halo tile is loaded in pairs of k-neighbours (HK is even, so a pair
+
+  // Register path: the halo tile is loaded in pairs of k-neighbours (HK is even, so a pair
   // never straddles a row): one 16-byte (FP64) or 8-byte (FP32) load per
   // pair, or two element loads where the row wraps around the periodic k
   // boundary between the two.
@@ -155,7 +217,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   const int px = i0 + li, py = j0 + lj, pz = k0 + lk;
   const bool valid = (px < a.nx) && (py < a.ny) && (pz < a.nz);
   const unsigned idx = valid ? (unsigned)((px * a.ny + py) * a.nz + pz) : 0u;
-  const int c = ((li + H) * HJ + (lj + H)) * HK + (lk + H);
+  const int c = ((li + H) * HJ + (lj + H)) * HK + (lk + KH);
 
   // wrapped global offset of the first element of each of this thread's
   // halo pairs (pair p = tid + q*NT, elements 2p and 2p+1); the same share is
@@ -164,6 +226,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   // (z = nz-1) and whose second is therefore at z = 0.
   unsigned goff[NPP];
   unsigned split = 0u;
+  if constexpr (!DMA) {
 #pragma unroll
   for (int q = 0; q < NPP; ++q) {
     const int pp = tid + q * NT;
@@ -175,6 +238,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     y += y < 0 ? a.ny : 0; y -= y >= a.ny ? a.ny : 0;
     z += z < 0 ? a.nz : 0; z -= z >= a.nz ? a.nz : 0;
     goff[q] = (unsigned)((x * a.ny + y) * a.nz + z);
+  }
   }
   const unsigned wrapk = (unsigned)(a.nz - 1);
   // both elements of pair q of a field
@@ -196,7 +260,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   auto RD = [&](int q, int slot) -> TS* { return (TS*)a.dreg + ((long)q * 4 + slot) * a.rstride; };
   constexpr bool CONS = (MODE & MODE_CONS) != 0;
   constexpr bool DUST = (MODE & MODE_DUST) != 0;
-  const bool norm = a.normalize != 0;
+  const bool norm = !DMA && a.normalize != 0;
 
   // stage-1 unit-determinant factor at the halo points (bssn.cc:75-99):
   // gamma_ii -> (gamma_ii + t)/(1 - t), gamma_ij -> gamma_ij/(1 - t).  Each
@@ -336,63 +400,154 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     for (int d = 0; d < 6; ++d) ddal[d] = TC(0);
   }
 
-  fetch(PHI);
-  store(0, PHI);
-  __syncthreads();
+  // derivatives of field f, staged in the LDS tile BUF, at this thread's
+  // point, folded into the per-point tensors (a macro: the same code as a
+  // lambda allocates registers differently and spills)
+#define CG_DERIVE(f, BUF) \
+    if (metric_field(f)) {                                                                                          \
+      const double* __restrict__ L = reinterpret_cast<const double*>(BUF);                                          \
+      if (KO_ORDER && use_ko) ko[f] = (TC)(a.ko_pref * ko_sum<KO_ORDER, double>(L, c, SI, SJ, SK));                 \
+      const double d0 = fd1<R>(L, c, SI) * i1, d1 = fd1<R>(L, c, SJ) * i1, d2 = fd1<R>(L, c, SK) * i1;              \
+      const double e00 = fd2<R>(L, c, SI) * i2, e11 = fd2<R>(L, c, SJ) * i2, e22 = fd2<R>(L, c, SK) * i2;           \
+      const double e01 = fdm<R>(L, c, SI, SJ) * i2, e02 = fdm<R>(L, c, SI, SK) * i2,                                \
+                   e12 = fdm<R>(L, c, SJ, SK) * i2;                                                                 \
+      if (f == PHI) {                                                                                               \
+        dphi[0] = (TC)d0; dphi[1] = (TC)d1; dphi[2] = (TC)d2;                                                       \
+        ddphi[0] = (TC)e00; ddphi[1] = (TC)e01; ddphi[2] = (TC)e02;                                                 \
+        ddphi[3] = (TC)e11; ddphi[4] = (TC)e12; ddphi[5] = (TC)e22;                                                 \
+      } else {                                                                                                      \
+        dg[0][f] = (TC)d0; dg[1][f] = (TC)d1; dg[2][f] = (TC)d2;                                                    \
+        T1[f] = gi[0] * (TC)e00 + gi[3] * (TC)e11 + gi[5] * (TC)e22 +                                               \
+                TC(2) * (gi[1] * (TC)e01 + gi[2] * (TC)e02 + gi[4] * (TC)e12);                                      \
+      }                                                                                                             \
+    } else {                                                                                                        \
+      const TC* __restrict__ L = reinterpret_cast<const TC*>(BUF);                                                  \
+      if (KO_ORDER && use_ko && f < NF) ko[f] = (TC)a.ko_pref * ko_sum<KO_ORDER, TC>(L, c, SI, SJ, SK);             \
+      if (f == ALPHA) {                                                                                             \
+        const TC idx2 = (TC)a.inv_dx2;                                                                              \
+        dal[0] = fd1<R>(L, c, SI) * idx1; dal[1] = fd1<R>(L, c, SJ) * idx1; dal[2] = fd1<R>(L, c, SK) * idx1;       \
+        ddal[0] = fd2<R>(L, c, SI) * idx2; ddal[3] = fd2<R>(L, c, SJ) * idx2; ddal[5] = fd2<R>(L, c, SK) * idx2;    \
+        ddal[1] = fdm<R>(L, c, SI, SJ) * idx2; ddal[2] = fdm<R>(L, c, SI, SK) * idx2;                               \
+        ddal[4] = fdm<R>(L, c, SJ, SK) * idx2;                                                                      \
+      } else if (f == KK) {                                                                                         \
+        dK[0] = fd1<R>(L, c, SI) * idx1; dK[1] = fd1<R>(L, c, SJ) * idx1; dK[2] = fd1<R>(L, c, SK) * idx1;          \
+      } else if (f >= GAM1 && f <= GAM3) {                                                                          \
+        const int kk = f - GAM1;                                                                                    \
+        dG[0][kk] = fd1<R>(L, c, SI) * idx1; dG[1][kk] = fd1<R>(L, c, SJ) * idx1;                                   \
+        dG[2][kk] = fd1<R>(L, c, SK) * idx1;                                                                        \
+      } else if (CONS && f >= A11 && f <= A33) {                                                                    \
+        const int q = f - A11;                                                                                      \
+        dA[0][q] = fd1<R>(L, c, SI) * idx1; dA[1][q] = fd1<R>(L, c, SJ) * idx1; dA[2][q] = fd1<R>(L, c, SK) * idx1; \
+      } else if (FLUXES && f >= FLUX0) {                                                                            \
+        /* dust.cc:203-224: flux q*3 + i is differentiated along i only */                                         \
+        const int q = (f - FLUX0) / 3, dir = (f - FLUX0) % 3;                                                       \
+        ddust[q] += fd1<R>(L, c, dir == 0 ? SI : dir == 1 ? SJ : SK) * idx1;                                        \
+      }                                                                                                             \
+    }
 
+  if constexpr (DMA) {
+    static_assert(std::is_same<TS, TC>::value, "DMA staging differentiates the stored type");
+    constexpr bool SPLIT = sizeof(TS) == 4;
+    static_assert(!(SPLIT && FLUXES), "the dust fluid is FP64 only");
+    static_assert(NT % 64 == 0 && HK % 4 == 0, "whole waves, whole 16-byte groups per halo row");
+    constexpr int NW = NT / 64;                                   // waves per block
+    constexpr int NPR = HV / 2, NQD = HV / 4;                     // 16-byte groups: FP64 pairs, FP32 quads
+    constexpr int CPR = (NPR + 63) / 64, CQD = (NQD + 63) / 64;   // 64-lane chunks per field
+    constexpr int MPR = (CPR + NW - 1) / NW, MQD = (CQD + NW - 1) / NW;  // chunks per wave
+    using TB1 = typename std::conditional<SPLIT, float, double>::type;
+    // Three field tiles, padded to whole chunks; in split precision buffer 1
+    // only ever holds FP32 fields.  Every wave issues the same number of
+    // chunks with all lanes active (no branches around the DMA loads, which
+    // keeps the compiler's per-buffer wait tracking exact): surplus chunks
+    // repeat the last chunk (the same bytes to the same place), surplus lanes
+    // of the last chunk load element 0 into the padding.
+    __shared__ __align__(16) double lb0[CPR * 128];
+    __shared__ __align__(16) TB1 lb1[SPLIT ? CQD * 256 : CPR * 128];
+    __shared__ __align__(16) double lb2[CPR * 128];
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), ln = tid & 63;
+    // wrapped global element offset of halo element e (e % 4 == 0 for quads,
+    // e % 2 == 0 for pairs: the group never leaves its row or wraps inside)
+    auto goff16 = [&](int e) -> unsigned {
+      const int hk = e % HK, hj = (e / HK) % HJ, hi = e / (HK * HJ);
+      int x = i0 + hi - H, y = j0 + hj - H, z = k0 + hk - KH;
+      x += x < 0 ? a.nx : 0; x -= x >= a.nx ? a.nx : 0;
+      y += y < 0 ? a.ny : 0; y -= y >= a.ny ? a.ny : 0;
+      z += z < 0 ? a.nz : 0; z -= z >= a.nz ? a.nz : 0;
+      return (unsigned)((x * a.ny + y) * a.nz + z);
+    };
+    // chunk m of this wave is chunk min(wv + m*NW, last) of the field: lane l
+    // takes group 64*chunk + l, which lands at LDS byte 16*(64*chunk + l)
+    auto pchunk = [&](int m) { const int ch = wv + m * NW; return ch < CPR ? ch : CPR - 1; };
+    auto qchunk = [&](int m) { const int ch = wv + m * NW; return ch < CQD ? ch : CQD - 1; };
+    unsigned poff[MPR];
 #pragma unroll
-  for (int s = 0; s < NSEQ; ++s) {
-    if (s < nseq) {
-      const bool more = (s + 1) < nseq;
-      const int fn = stencil_seq<LIVE_ALPHA, FLUXES>((s + 1) < NSEQ ? s + 1 : 0);
-      if (more) fetch(fn);
-      const int f = stencil_seq<LIVE_ALPHA, FLUXES>(s);
-      if (valid) {
-        if (metric_field(f)) {
-          const double* __restrict__ L = ldsraw[s & 1];
-          if (KO_ORDER && use_ko) ko[f] = (TC)(a.ko_pref * ko_sum<KO_ORDER, double>(L, c, SI, SJ, SK));
-          const double d0 = fd1<R>(L, c, SI) * i1, d1 = fd1<R>(L, c, SJ) * i1, d2 = fd1<R>(L, c, SK) * i1;
-          const double e00 = fd2<R>(L, c, SI) * i2, e11 = fd2<R>(L, c, SJ) * i2, e22 = fd2<R>(L, c, SK) * i2;
-          const double e01 = fdm<R>(L, c, SI, SJ) * i2, e02 = fdm<R>(L, c, SI, SK) * i2,
-                       e12 = fdm<R>(L, c, SJ, SK) * i2;
-          if (f == PHI) {
-            dphi[0] = (TC)d0; dphi[1] = (TC)d1; dphi[2] = (TC)d2;
-            ddphi[0] = (TC)e00; ddphi[1] = (TC)e01; ddphi[2] = (TC)e02;
-            ddphi[3] = (TC)e11; ddphi[4] = (TC)e12; ddphi[5] = (TC)e22;
-          } else {
-            dg[0][f] = (TC)d0; dg[1][f] = (TC)d1; dg[2][f] = (TC)d2;
-            T1[f] = gi[0] * (TC)e00 + gi[3] * (TC)e11 + gi[5] * (TC)e22 +
-                    TC(2) * (gi[1] * (TC)e01 + gi[2] * (TC)e02 + gi[4] * (TC)e12);
-          }
-        } else {
-          const TC* __restrict__ L = reinterpret_cast<const TC*>(ldsraw[s & 1]);
-          if (KO_ORDER && use_ko && f < NF) ko[f] = (TC)a.ko_pref * ko_sum<KO_ORDER, TC>(L, c, SI, SJ, SK);
-          if (f == ALPHA) {
-            const TC idx2 = (TC)a.inv_dx2;
-            dal[0] = fd1<R>(L, c, SI) * idx1; dal[1] = fd1<R>(L, c, SJ) * idx1; dal[2] = fd1<R>(L, c, SK) * idx1;
-            ddal[0] = fd2<R>(L, c, SI) * idx2; ddal[3] = fd2<R>(L, c, SJ) * idx2; ddal[5] = fd2<R>(L, c, SK) * idx2;
-            ddal[1] = fdm<R>(L, c, SI, SJ) * idx2; ddal[2] = fdm<R>(L, c, SI, SK) * idx2;
-            ddal[4] = fdm<R>(L, c, SJ, SK) * idx2;
-          } else if (f == KK) {
-            dK[0] = fd1<R>(L, c, SI) * idx1; dK[1] = fd1<R>(L, c, SJ) * idx1; dK[2] = fd1<R>(L, c, SK) * idx1;
-          } else if (f >= GAM1 && f <= GAM3) {
-            const int kk = f - GAM1;
-            dG[0][kk] = fd1<R>(L, c, SI) * idx1; dG[1][kk] = fd1<R>(L, c, SJ) * idx1;
-            dG[2][kk] = fd1<R>(L, c, SK) * idx1;
-          } else if (CONS && f >= A11 && f <= A33) {
-            const int q = f - A11;
-            dA[0][q] = fd1<R>(L, c, SI) * idx1; dA[1][q] = fd1<R>(L, c, SJ) * idx1; dA[2][q] = fd1<R>(L, c, SK) * idx1;
-          } else if (FLUXES && f >= FLUX0) {
-            // dust.cc:203-224: flux q*3 + i is differentiated along i only
-            const int q = (f - FLUX0) / 3, dir = (f - FLUX0) % 3;
-            ddust[q] += fd1<R>(L, c, dir == 0 ? SI : dir == 1 ? SJ : SK) * idx1;
-          }
-        }
+    for (int m = 0; m < MPR; ++m) {
+      const int g = pchunk(m) * 64 + ln;
+      poff[m] = goff16(2 * (g < NPR ? g : 0));
+    }
+    unsigned qoff[SPLIT ? MQD : 1];
+    if constexpr (SPLIT) {
+#pragma unroll
+      for (int m = 0; m < MQD; ++m) {
+        const int g = qchunk(m) * 64 + ln;
+        qoff[m] = goff16(4 * (g < NQD ? g : 0));
       }
-      if (more) store((s + 1) & 1, fn);
-      __syncthreads();
+    }
+    typedef __attribute__((address_space(3))) char lds_char;
+    const unsigned lds0 = (unsigned)(size_t)(lds_char*)lb0, lds1 = (unsigned)(size_t)(lds_char*)lb1,
+                   lds2 = (unsigned)(size_t)(lds_char*)lb2;
+    auto dma = [&](const int f, const int b) {
+      const unsigned dst = b == 0 ? lds0 : b == 1 ? lds1 : lds2;
+      if (!SPLIT || metric_field(f)) {
+        const void* src = metric_field(f) ? (const void*)INM(f) : (const void*)INS(f);
+#pragma unroll
+        for (int m = 0; m < MPR; ++m) dma16(src, poff[m] * 8u, dst + pchunk(m) * 1024);
+      } else {
+        const void* src = (const void*)INS(f);
+#pragma unroll
+        for (int m = 0; m < MQD; ++m) dma16(src, qoff[m] * 4u, dst + qchunk(m) * 1024);
+      }
+    };
+    auto seq = [&](int s) { return SPLIT ? stencil_seq_split<LIVE_ALPHA>(s) : stencil_seq<LIVE_ALPHA, FLUXES>(s); };
+    dma(seq(0), 0);
+    if (nseq > 1) dma(seq(1), 1);
+#pragma unroll
+    for (int s = 0; s < NSEQ; ++s) {
+      if (s < nseq) {
+        // position s has landed for this wave once at most the chunks of
+        // position s + 1 are outstanding; for every wave after the barrier
+        const int f1 = seq(s + 1 < NSEQ ? s + 1 : 0);
+        if (s + 1 < nseq) {
+          if (!SPLIT || metric_field(f1)) dma_wait_barrier<MPR>(); else dma_wait_barrier<MQD>();
+        } else {
+          dma_wait_barrier<0>();
+        }
+        // buffer (s + 2) % 3 was last read at position s - 1, before the barrier
+        if (s + 2 < nseq) dma(seq(s + 2 < NSEQ ? s + 2 : 0), (s + 2) % 3);
+        const int f = seq(s);
+        const void* const buf = s % 3 == 0 ? (const void*)lb0 : s % 3 == 1 ? (const void*)lb1 : (const void*)lb2;
+        if (valid) { CG_DERIVE(f, buf); }
+      }
+    }
+  } else {
+    fetch(PHI);
+    store(0, PHI);
+    __syncthreads();
+
+  #pragma unroll
+    for (int s = 0; s < NSEQ; ++s) {
+      if (s < nseq) {
+        const bool more = (s + 1) < nseq;
+        const int fn = stencil_seq<LIVE_ALPHA, FLUXES>((s + 1) < NSEQ ? s + 1 : 0);
+        if (more) fetch(fn);
+        const int f = stencil_seq<LIVE_ALPHA, FLUXES>(s);
+        if (valid) { CG_DERIVE(f, ldsraw[s & 1]); }
+        if (more) store((s + 1) & 1, fn);
+        __syncthreads();
+      }
     }
   }
+#undef CG_DERIVE
   // Threads outside the grid (ragged edge tiles) skip the algebra but stay
   // alive for the block-wide setExtraFieldData reduction at the end.
   double phi_new = 0.0, K_new = 0.0, rho_new = 0.0;  // stage output and source seen by the reduction
@@ -822,13 +977,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   }
 }
 
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LIVE_ALPHA, int MODE = 0>
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LIVE_ALPHA, int MODE = 0,
+          bool DMA = false>
 __global__ void __launch_bounds__(TI* TJ* TK) __attribute__((amdgpu_waves_per_eu(WPE)))
 stage_kernel(const StageArgs a) {
   const int b = blockIdx.x;
   const int t = (b & 7) * a.per_xcd + (b >> 3);  // consecutive tiles stay on one XCD
   if (t >= a.ntx * a.nty * a.ntz) return;
-  stage_body<TS, TC, R, H, TI, TJ, TK, LIVE_ALPHA, MODE>(
+  stage_body<TS, TC, R, H, TI, TJ, TK, LIVE_ALPHA, MODE, DMA>(
       a, t, FrwV{a.phi_F, a.K_F, a.rho_F, a.S_F, a.H_frw, a.phi_F_next});
 }
 
@@ -837,7 +993,7 @@ stage_kernel(const StageArgs a) {
 // to blocks with the same XCD-aware remap as a single grid, so each XCD's L2
 // sees neighbouring tiles of one grid.  (A persistent loop over tiles would
 // let the compiler hoist per-thread tile geometry out of the loop and spill.)
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LIVE_ALPHA>
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LIVE_ALPHA, bool DMA = false>
 __global__ void __launch_bounds__(TI* TJ* TK) __attribute__((amdgpu_waves_per_eu(WPE)))
 stage_kernel_batch(const BatchArgs ba) {
   const int b = blockIdx.x;
@@ -849,29 +1005,32 @@ stage_kernel_batch(const BatchArgs ba) {
     if (ba.tstart[r + h] <= g) { r += h; n -= h; } else { n = h; }
   }
   const double* f = ba.frw + (long)r * FRW_N;
-  stage_body<TS, TC, R, H, TI, TJ, TK, LIVE_ALPHA, 0>(ba.tab[r], g - ba.tstart[r],
+  stage_body<TS, TC, R, H, TI, TJ, TK, LIVE_ALPHA, 0, DMA>(ba.tab[r], g - ba.tstart[r],
                                                       FrwV{f[0], f[1], f[2], f[3], f[4], f[5]});
 }
 
 // ------------------------------------------------------------ variants
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA, int MODE = MODE_STAGE>
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA, int MODE = MODE_STAGE,
+          bool DMA = false>
 int stage_launch_t(const StageArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((stage_kernel<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE>), dim3(8 * a.per_xcd), dim3(TI * TJ * TK), 0,
-                     s, a);
+  hipLaunchKernelGGL((stage_kernel<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE, DMA>), dim3(8 * a.per_xcd),
+                     dim3(TI * TJ * TK), 0, s, a);
   return (int)hipGetLastError();
 }
 
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA>
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA, bool DMA = false>
 int stage_launch_batch_t(const BatchArgs& ba, hipStream_t s) {
-  hipLaunchKernelGGL((stage_kernel_batch<TS, TC, R, H, TI, TJ, TK, WPE, LA>), dim3(8 * ba.per_xcd), dim3(TI * TJ * TK), 0,
-                     s, ba);
+  hipLaunchKernelGGL((stage_kernel_batch<TS, TC, R, H, TI, TJ, TK, WPE, LA, DMA>), dim3(8 * ba.per_xcd),
+                     dim3(TI * TJ * TK), 0, s, ba);
   return (int)hipGetLastError();
 }
 
-template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA, int MODE = MODE_STAGE>
+template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, int WPE, bool LA, int MODE = MODE_STAGE,
+          bool DMA = false>
 int stage_info_t(int* vgprs, int* spills, int* lds) {
   hipFuncAttributes attr;
-  hipError_t e = hipFuncGetAttributes(&attr, (const void*)stage_kernel<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE>);
+  hipError_t e =
+      hipFuncGetAttributes(&attr, (const void*)stage_kernel<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE, DMA>);
   if (e != hipSuccess) return (int)e;
   *vgprs = attr.numRegs;
   *spills = (int)attr.localSizeBytes;
@@ -885,21 +1044,40 @@ int cons_launch_t(const StageArgs& a, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
-#define CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE, BATCH)                              \
-  static StageVariantReg CG_CAT(reg_, __COUNTER__)(                                                           \
-      StageVariant{PREC, 2 * R, H, TI, TJ, TK, WPE, LA, (MODE & MODE_DUST) != 0,                            \
-                   &stage_launch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE>,                                 \
-                   &stage_info_t<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE>, BATCH});
+#define CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE, BATCH, DMA_L, DMA_I, DMA_B) \
+  static StageVariantReg CG_CAT(reg_, __COUNTER__)(                                                    \
+      StageVariant{PREC, 2 * R, H, TI, TJ, TK, WPE, LA, (MODE & MODE_DUST) != 0,                     \
+                   &stage_launch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE>,                          \
+                   &stage_info_t<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE>, BATCH, DMA_L, DMA_I, DMA_B});
+#define CG_DMA_L(TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE) (&stage_launch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE, true>)
+#define CG_DMA_I(TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE) (&stage_info_t<TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE, true>)
 // single-grid kernel only
 #define CG_REGISTER_STAGE(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA) \
-  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE, nullptr)
+  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE, nullptr, nullptr, nullptr, nullptr)
 // single-grid and batched-ensemble kernels
-#define CG_REGISTER_STAGE_B(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA) \
-  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE, \
-                      (&stage_launch_batch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>))
+#define CG_REGISTER_STAGE_B(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA)                                   \
+  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE,                            \
+                      (&stage_launch_batch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>), nullptr, nullptr, nullptr)
+// single-grid kernel, with its LDS-DMA twin
+#define CG_REGISTER_STAGE_D(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA)                                   \
+  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE, nullptr,                   \
+                      CG_DMA_L(TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE),                        \
+                      CG_DMA_I(TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE), nullptr)
+// single-grid and batched-ensemble kernels, each with its LDS-DMA twin
+#define CG_REGISTER_STAGE_BD(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA)                                  \
+  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE,                            \
+                      (&stage_launch_batch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA>),                     \
+                      CG_DMA_L(TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE),                        \
+                      CG_DMA_I(TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE_STAGE),                        \
+                      (&stage_launch_batch_t<TS, TC, R, H, TI, TJ, TK, WPE, LA, true>))
 // metric + dust fluid (single grid)
 #define CG_REGISTER_DUST(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA) \
-  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_DUST, nullptr)
+  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_DUST, nullptr, nullptr, nullptr, nullptr)
+// metric + dust fluid (single grid), with its LDS-DMA twin
+#define CG_REGISTER_DUST_D(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA)                                    \
+  CG_REGISTER_STAGE_X(TS, TC, PREC, R, H, TI, TJ, TK, WPE, LA, MODE_DUST, nullptr,                    \
+                      CG_DMA_L(TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE_DUST),                         \
+                      CG_DMA_I(TS, TC, R, H, TI, TJ, TK, WPE, LA, MODE_DUST), nullptr)
 #define CG_CAT2(a, b) a##b
 #define CG_CAT(a, b) CG_CAT2(a, b)
 

@@ -371,6 +371,15 @@ bool variant_fits(const cg_ctx* c, const StageVariant& v) {
   return true;
 }
 
+// the LDS-DMA twin of v applies to this grid: compiled, nz % 4 == 0 (16-byte
+// groups never straddle the periodic wrap) and one wrap of the k halo
+// (rounded up to 4) suffices
+bool dma_fits(const cg_ctx* c, const StageVariant& v) {
+  if (!v.launch_dma) return false;
+  const int kh = (v.halo + 3) / 4 * 4;
+  return c->p.nz % 4 == 0 && c->p.nz >= kh && c->p.nz > v.tk + kh - 2;
+}
+
 bool alpha_live(const cg_ctx* c) { return c->p.lapse != CG_LAPSE_STATIC || c->alpha_nonzero; }
 
 }  // namespace cg
@@ -607,6 +616,12 @@ int cg_set_lambda(cg_ctx* c, double lambda) {
   return CG_OK;
 }
 
+int cg_set_dma(cg_ctx* c, int on) {
+  if (!c) return CG_ERR_ARG;
+  c->dma_allowed = on != 0;
+  return CG_OK;
+}
+
 int cg_set_gauge_params(cg_ctx* c, double gd_c, double dw_mu_l, double dw_p) {
   if (!c) return CG_ERR_ARG;
   c->gd_c = gd_c; c->dw_mu_l = dw_mu_l; c->dw_p = dw_p;
@@ -678,7 +693,9 @@ static int launch(cg_ctx* c, StageArgs a) {
   a.per_xcd = (a.ntx * a.nty * a.ntz + 7) / 8;
   c->last_rows = (long)a.ntx * a.nty * a.ntz;
   if (c->last_rows > c->ext_rows) return fail(c, CG_ERR_STATE, "extra-field partial buffer too small");
-  int rc = c->var->launch(a, c->stream);
+  // stage-1 normalisation needs the register path (it transforms the metric on its way into LDS)
+  const bool dma = c->dma_allowed && c->dma_pick && !a.normalize && dma_fits(c, *c->var);
+  int rc = (dma ? c->var->launch_dma : c->var->launch)(a, c->stream);
   if (rc != 0) return fail(c, CG_ERR_HIP, std::string("stage launch: ") + hipGetErrorString((hipError_t)rc));
   if (c->timing) {
     HIPCHK(c, hipEventRecord(e1, c->stream));
@@ -944,7 +961,8 @@ int cg_kernel_info(const cg_ctx* cc, int* vgprs, int* scratch, int* lds, int* ti
   if (!c) return CG_ERR_ARG;
   if (!c->var || c->var->live_alpha != alpha_live(c)) c->var = pick_variant(c);
   if (!c->var) return fail(c, CG_ERR_ARG, "no stage kernel variant fits this grid/order");
-  int rc = c->var->info(vgprs, scratch, lds);
+  const bool dma = c->dma_allowed && c->dma_pick && dma_fits(c, *c->var);
+  int rc = (dma ? c->var->info_dma : c->var->info)(vgprs, scratch, lds);
   *ti = c->var->ti; *tj = c->var->tj; *tk = c->var->tk;
   return rc ? CG_ERR_HIP : CG_OK;
 }
@@ -1042,33 +1060,40 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
   const bool timing = c->timing;
   c->timing = false;
   int best_order = 0;
+  bool best_dma = false;
   for (const auto& v : stage_variants()) {
     if (!variant_matches(c, v)) continue;
     c->var = &v;
-    for (int order = 0; order < 3; ++order) {
-      c->tile_order = order;
-      StageArgs a = make_args(c, 2, 0, 0, 1, 3, 0);
-      int rc = launch(c, a);
-      if (rc) { c->timing = timing; return rc; }
-      HIPCHK(c, hipEventRecord(e0, c->stream));
-      for (int r = 0; r < reps; ++r)
-        if ((rc = launch(c, a))) { c->timing = timing; return rc; }
-      HIPCHK(c, hipEventRecord(e1, c->stream));
-      HIPCHK(c, hipEventSynchronize(e1));
-      float ms = 0;
-      HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-      const double avg = ms / reps;
-      int vg = 0, sc = 0, ld = 0;
-      v.info(&vg, &sc, &ld);
-      char buf[256];
-      snprintf(buf, sizeof buf,
-               "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"order\":%d,\"ms\":%.4f,\"vgprs\":%d,\"scratch\":%d}",
-               rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, order, avg, vg, sc);
-      rep += buf;
-      if (avg < best_ms) { best_ms = avg; best = &v; best_order = order; }
+    for (int d = 0; d < 2; ++d) {
+      if (d == 1 && !(c->dma_allowed && dma_fits(c, v))) continue;
+      c->dma_pick = d == 1;
+      for (int order = 0; order < 3; ++order) {
+        c->tile_order = order;
+        StageArgs a = make_args(c, 2, 0, 0, 1, 3, 0);
+        int rc = launch(c, a);
+        if (rc) { c->timing = timing; return rc; }
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; ++r)
+          if ((rc = launch(c, a))) { c->timing = timing; return rc; }
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+        const double avg = ms / reps;
+        int vg = 0, sc = 0, ld = 0;
+        (d ? v.info_dma : v.info)(&vg, &sc, &ld);
+        char buf[256];
+        snprintf(buf, sizeof buf,
+                 "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"dma\":%d,\"order\":%d,\"ms\":%.4f,\"vgprs\":%d,"
+                 "\"scratch\":%d}",
+                 rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, d, order, avg, vg, sc);
+        rep += buf;
+        if (avg < best_ms) { best_ms = avg; best = &v; best_order = order; best_dma = d == 1; }
+      }
     }
   }
   c->tile_order = best_order;
+  c->dma_pick = best_dma || !best;
   rep += "]";
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

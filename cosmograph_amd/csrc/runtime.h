@@ -102,6 +102,9 @@ struct cg_ctx {
   int req_ti = 0, req_tj = 0, req_tk = 0;  // requested tile (0 = automatic)
   const StageVariant* var = nullptr;     // stage kernel in use
   int tile_order = 0;                     // tile traversal order (autotuned)
+  // LDS-DMA halo staging: allowed by the caller (cg_set_dma) and picked by
+  // the autotuner; used where the variant has it and the grid fits
+  bool dma_allowed = true, dma_pick = true;
   double gd_c = 1.0, dw_mu_l = 0.0, dw_p = 0.0;  // gauge parameters (BSSNGaugeHandler.h:139-149)
   bool alpha_nonzero = false;             // a non-zero DIFFalpha was uploaded
   double* cons = nullptr;                 // [CONS_N][pad] per-point constraint values (lazy)
@@ -163,6 +166,7 @@ inline int fail(cg_ctx* c, int code, const std::string& msg) {
 void* reg_ptr(cg_ctx* c, int f, int slot);
 int halo_for(int order, double ko);
 bool variant_fits(const cg_ctx* c, const StageVariant& v);
+bool dma_fits(const cg_ctx* c, const StageVariant& v);
 bool alpha_live(const cg_ctx* c);
 StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int normalize);
 int ext_resolve(cg_ctx* c);

@@ -40,7 +40,7 @@ __host__ __device__ constexpr double cm(int m) {
 }
 
 template <int R, typename T>
-__device__ __forceinline__ T fd1(const T* __restrict__ s, int c, int st) {
+__device__ __forceinline__ T fd1(const T* s, int c, int st) {
   T acc = T(c1<R>(1)) * (s[c + st] - s[c - st]);
 #pragma unroll
   for (int m = 2; m <= R; ++m) acc += T(c1<R>(m)) * (s[c + m * st] - s[c - m * st]);
@@ -48,7 +48,7 @@ __device__ __forceinline__ T fd1(const T* __restrict__ s, int c, int st) {
 }
 
 template <int R, typename T>
-__device__ __forceinline__ T fd2(const T* __restrict__ s, int c, int st) {
+__device__ __forceinline__ T fd2(const T* s, int c, int st) {
   T acc = T(c2<R>(0)) * s[c];
 #pragma unroll
   for (int m = 1; m <= R; ++m) acc += T(c2<R>(m)) * (s[c + m * st] + s[c - m * st]);
@@ -56,7 +56,7 @@ __device__ __forceinline__ T fd2(const T* __restrict__ s, int c, int st) {
 }
 
 template <int R, typename T>
-__device__ __forceinline__ T fdm(const T* __restrict__ s, int c, int sa, int sb) {
+__device__ __forceinline__ T fdm(const T* s, int c, int sa, int sb) {
   T acc = T(0);
 #pragma unroll
   for (int m = 1; m <= R; ++m) {
@@ -69,7 +69,7 @@ __device__ __forceinline__ T fdm(const T* __restrict__ s, int c, int sa, int sb)
 // Kreiss-Oliger stencil sum (without prefactor) over the three axes.
 // order 2: [1,-4,6,-4,1];  order 8: [1,-10,45,-120,210,-252,...]
 template <int ORDER, typename T>
-__device__ __forceinline__ T ko_sum(const T* __restrict__ s, int c, int si, int sj, int sk) {
+__device__ __forceinline__ T ko_sum(const T* s, int c, int si, int sj, int sk) {
   if constexpr (ORDER == 2) {
     constexpr double w[3] = {6.0, -4.0, 1.0};
     T acc = T(3.0 * w[0]) * s[c];

@@ -113,6 +113,7 @@ def load_library():
         "cg_sync": ([vp], C.c_int),
         "cg_kernel_info": ([vp, ip, ip, ip, ip, ip, ip], C.c_int),
         "cg_set_timing": ([vp, C.c_int], C.c_int),
+        "cg_set_dma": ([vp, C.c_int], C.c_int),
         "cg_autotune": ([vp, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
         "cg_constraints": ([vp, dp, dp, dp, dp, dp], C.c_int),
         "cg_stage_time_ms": ([vp, dp, ip], C.c_int),
@@ -124,6 +125,7 @@ def load_library():
         "cg_batch_member": ([vp, C.c_int, C.POINTER(vp)], C.c_int),
         "cg_batch_step": ([vp, C.c_int], C.c_int),
         "cg_batch_set_graph": ([vp, C.c_int], C.c_int),
+        "cg_batch_set_dma": ([vp, C.c_int], C.c_int),
         "cg_batch_step_time_ms": ([vp, dp], C.c_int),
         "cg_batch_constraints": ([vp, dp, dp, dp, dp, dp], C.c_int),
         "cg_batch_autotune": ([vp, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
@@ -261,6 +263,10 @@ class BSSN:
     def set_timing(self, on: bool):
         self._chk(self.L.cg_set_timing(self.h, int(on)))
 
+    def set_dma(self, on: bool):
+        """LDS-DMA halo staging where it applies (default on); off = register staging."""
+        self._chk(self.L.cg_set_dma(self.h, int(on)))
+
     def stage_time_ms(self):
         avg, n = C.c_double(), C.c_int()
         self._chk(self.L.cg_stage_time_ms(self.h, C.byref(avg), C.byref(n)))
@@ -311,6 +317,9 @@ class Batch:
 
     def set_graph(self, on: bool):
         self._chk(self.L.cg_batch_set_graph(self.h, int(on)))
+
+    def set_dma(self, on: bool):
+        self._chk(self.L.cg_batch_set_dma(self.h, int(on)))
 
     def step_time_ms(self) -> float:
         ms = C.c_double()

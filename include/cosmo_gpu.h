@@ -133,6 +133,16 @@ int cg_kernel_info(const cg_ctx* ctx, int* vgprs, int* scratch_bytes, int* lds_b
  * steps.  `report` (optional) receives a JSON list of the timings. */
 int cg_autotune(cg_ctx* ctx, int reps, char* report, size_t report_len);
 
+/* Halo staging of the stage kernel.  on = 1 (default): a launch without
+ * stage-1 metric normalisation, on a grid with nz % 4 == 0, uses the
+ * kernel's LDS-DMA twin where one is compiled: halo tiles go from memory
+ * straight into LDS (global_load_lds), two fields ahead of the one being
+ * differentiated; the autotuner times both paths and keeps the faster.
+ * on = 0: register staging everywhere.  The two paths agree to rounding
+ * (the kernels may contract a few multiply-adds differently).
+ * No counterpart in the reference (its stencils read memory directly). */
+int cg_set_dma(cg_ctx* ctx, int on);
+
 /* Average duration of the last `n` stage-kernel launches recorded with HIP
  * events on the context's stream (ms); enable with cg_set_timing(ctx, 1). */
 int cg_set_timing(cg_ctx* ctx, int on);
@@ -176,6 +186,8 @@ int cg_batch_constraints(cg_batch* batch, double* H, double* M, double* G, doubl
 /* Times every compiled batched variant that fits all grids and keeps the
  * fastest (state untouched; between whole steps). */
 int cg_batch_autotune(cg_batch* batch, int reps, char* report, size_t report_len);
+/* cg_set_dma for the batched kernel (drops a captured graph). */
+int cg_batch_set_dma(cg_batch* batch, int on);
 int cg_batch_kernel_info(const cg_batch* batch, int* vgprs, int* scratch_bytes, int* tile_i, int* tile_j,
                          int* tile_k);
 
