@@ -41,6 +41,12 @@ struct StageArgs {
   void* regs;
   long rstride;
   int s_in, s_p, s_f, s_out;
+  // Stage 1 with the metric normalisation precomputed: DIFFgamma_ij and A_ij
+  // are read from slot s_in_norm, the other fields from s_in (-1: off).
+  // Stage 4: also write set_DIFFgamma_Aij_norm of the new state
+  // (bssn.cc:61-119) into slot s_norm_out for the next step's stage 1
+  // (-1: off).
+  int s_in_norm, s_norm_out;
   const void* DIFFD;      // static dust: conserved density perturbation
   // dust fluid (components/dust_fluid/dust.h:32-46): registers of D, S1..S3
   // at dreg + (q*4 + slot)*rstride (storage type); the 12 fluxes

@@ -253,9 +253,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
 
   auto RM = [&](int f, int slot) -> double* { return a.met + ((long)f * 4 + slot) * a.rstride; };
   auto RS = [&](int f, int slot) -> TS* { return (TS*)a.regs + ((long)(f - 7) * 4 + slot) * a.rstride; };
-  auto INM = [&](int f) -> const double* { return RM(f, a.s_in); };
+  // stencil / point input of field f: DIFFgamma_ij and A_ij come from the
+  // precomputed normalised copy when there is one
+  auto INM = [&](int f) -> const double* { return RM(f, (f <= G33 && a.s_in_norm >= 0) ? a.s_in_norm : a.s_in); };
   auto INS = [&](int f) -> const TS* {
-    return f >= FLUX0 ? (const TS*)a.flux + (long)(f - FLUX0) * a.rstride : RS(f, a.s_in);
+    return f >= FLUX0 ? (const TS*)a.flux + (long)(f - FLUX0) * a.rstride
+                      : RS(f, (f >= A11 && f <= A33 && a.s_in_norm >= 0) ? a.s_in_norm : a.s_in);
   };
   auto RD = [&](int q, int slot) -> TS* { return (TS*)a.dreg + ((long)q * 4 + slot) * a.rstride; };
   constexpr bool CONS = (MODE & MODE_CONS) != 0;
@@ -896,6 +899,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     // stage 4 p' = f + (cf*c + p).  Accumulation in FP64 for every precision.
     const int st = a.stage;
     rho_new = (double)DIFFr;
+    double gn[6], An[6];  // stage 4: the new DIFFgamma_ij and A_ij as stored
   #pragma unroll
     for (int f = 0; f < NF; ++f) {
       if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
@@ -912,6 +916,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         const double inc = a.cf * cc;
         const double nv = st == 4 ? fo + (inc + p) : p + a.ca * cc;
         if (f == PHI) phi_new = nv;
+        if (f <= G33) gn[f] = nv;
         sto(o, idx, nv);
         if (st == 4) {
           if (f == PHI && nan_as_float(nv)) atomicAdd(a.nan_count, 1u);
@@ -930,8 +935,31 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         const double inc = a.cf * cc;
         const TS nv = st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc);
         if (f == KK) K_new = (double)nv;
+        if (f >= A11 && f <= A33) An[f - A11] = (double)nv;
         sto(o, idx, nv);
         if (st != 4) sto(fr, idx, (TS)(fo + inc));
+      }
+    }
+    if (DMA && st == 4 && a.s_norm_out >= 0) {
+      // set_DIFFgamma_Aij_norm (bssn.cc:61-119) of the new state, as the
+      // next step's stepInit applies it to _a (DMA kernels only: the
+      // launcher pairs the copy with DMA staging)
+      const double tn = norm_factor<double>(gn[0], gn[1], gn[2], gn[3], gn[4], gn[5]);
+      double g[6];
+  #pragma unroll
+      for (int s = 0; s < 6; ++s) g[s] = (gn[s] + (diag6(s) ? tn : 0.0)) / (1.0 - tn);
+      const double i11 = 1.0 + g[3] + g[5] - g[4] * g[4] + g[3] * g[5];
+      const double i22 = 1.0 + g[0] + g[5] - g[2] * g[2] + g[0] * g[5];
+      const double i33 = 1.0 + g[0] + g[3] - g[1] * g[1] + g[0] * g[3];
+      const double i12 = g[2] * g[4] - g[1] * (1.0 + g[5]);
+      const double i13 = g[1] * g[4] - g[2] * (1.0 + g[3]);
+      const double i23 = g[1] * g[2] - g[4] * (1.0 + g[0]);
+      const double trA = i11 * An[0] + i22 * An[3] + i33 * An[5] + 2.0 * (i12 * An[1] + i13 * An[2] + i23 * An[4]);
+  #pragma unroll
+      for (int s = 0; s < 6; ++s) {
+        sto(RM(G11 + s, a.s_norm_out), idx, g[s]);
+        sto(RS(A11 + s, a.s_norm_out), idx,
+            (TS)((An[s] - (1.0 / 3.0) * ((diag6(s) ? 1.0 : 0.0) + g[s]) * trA) / (1.0 - tn)));
       }
     }
     if constexpr (DUST) {

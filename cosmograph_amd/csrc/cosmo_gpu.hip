@@ -635,6 +635,7 @@ StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int
   a.regs = c->regs;
   a.rstride = c->rstride;
   a.s_in = rin; a.s_p = rp; a.s_f = rf; a.s_out = rout;
+  a.s_in_norm = -1; a.s_norm_out = -1;
   a.DIFFD = c->DIFFD;
   a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
   a.tile_order = c->tile_order;
@@ -695,6 +696,7 @@ static int launch(cg_ctx* c, StageArgs a) {
   if (c->last_rows > c->ext_rows) return fail(c, CG_ERR_STATE, "extra-field partial buffer too small");
   // stage-1 normalisation needs the register path (it transforms the metric on its way into LDS)
   const bool dma = c->dma_allowed && c->dma_pick && !a.normalize && dma_fits(c, *c->var);
+  if (a.s_norm_out >= 0 && !dma) return fail(c, CG_ERR_STATE, "normalised copy without DMA staging");
   int rc = (dma ? c->var->launch_dma : c->var->launch)(a, c->stream);
   if (rc != 0) return fail(c, CG_ERR_HIP, std::string("stage launch: ") + hipGetErrorString((hipError_t)rc));
   if (c->timing) {
@@ -863,8 +865,10 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
   // dust: the sources of stage n use W, detg, g_ij of stage n-1's input
   int prev_in = 0, prev_norm = 0;
   double prev_phiF = 0.0;
-  auto stage = [&](int n, int rin, int rout, int normalize) -> int {
+  auto stage = [&](int n, int rin, int rout, int normalize, int in_norm = -1, int norm_out = -1) -> int {
     StageArgs a = make_args(c, n, rin, 0, 1, rout, normalize);
+    a.s_in_norm = in_norm;
+    a.s_norm_out = norm_out;
     if (c->dreg) {
       if (int rc = dust_flux(c, rin, normalize)) return rc;
       if (n > 1) {
@@ -883,6 +887,10 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
     return (rc || !flrw) ? rc : ext_combine(c, c->last_rows, false);
   };
   c->ext_lazy = false;
+  // With DMA staging, stage 4 also writes the normalised copy of the new
+  // state (stepInit's _a) into the dead slot Y, so that the next step's
+  // stage 1 runs the DMA kernel instead of normalising halos on the fly.
+  bool have_norm = false;
   for (int s = 0; s < nsteps; ++s) {
     int rc;
     c->sP = 0; c->sF = 1;
@@ -896,12 +904,15 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
       // stepInit's setExtraFieldData equals the last K4Finalize's unless the
       // state changed in between; only the ConformalFLRW lapse reads it
       if (flrw && !c->ext_fresh && (rc = ext_from_state(c))) return rc;
-      if ((rc = stage(1, 0, X, c->p.normalize_metric))) return rc;
+      if ((rc = have_norm ? stage(1, 0, X, 0, Y) : stage(1, 0, X, c->p.normalize_metric))) return rc;
       if ((rc = stage(2, X, Y, 0))) return rc;
     }
     const int in3 = ko_norm ? X : Y, out3 = ko_norm ? Y : X;
     if ((rc = stage(3, in3, out3, 0))) return rc;
-    if ((rc = stage(4, out3, 0, 0))) return rc;
+    const bool pre = !ko_norm && !c->dreg && c->p.normalize_metric && s + 1 < nsteps && c->var &&
+                     c->dma_allowed && c->dma_pick && dma_fits(c, *c->var);
+    if ((rc = stage(4, out3, 0, 0, -1, pre ? Y : -1))) return rc;
+    have_norm = pre;
     c->sA = 0; c->sC = out3; c->a_is_p = true;
     c->ext_fresh = flrw;
     if (!flrw) {
