@@ -143,6 +143,12 @@ __device__ __forceinline__ void dma16(const void* base, unsigned voff, unsigned 
   asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(bs), "s"(ls)
                : "memory", "m0");
 }
+// The same with a per-lane 64-bit address (lanes may read different arrays).
+__device__ __forceinline__ void dma16v(const void* addr, unsigned lds) {
+  const unsigned ls = (unsigned)__builtin_amdgcn_readfirstlane((int)lds);
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(addr), "s"(ls)
+               : "memory", "m0");
+}
 #pragma clang diagnostic pop
 
 // End of one step of the DMA staging: this wave's vector memory operations
@@ -184,6 +190,24 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   constexpr int HJ = TJ + 2 * H, HK = TK + 2 * KH, HI = TI + 2 * H;
   constexpr int HV = HI * HJ * HK;
   constexpr int SI = HJ * HK, SJ = HK, SK = 1;
+
+  // DMA path geometry.  Halo tiles move in 16-byte groups (FP64 pairs, FP32
+  // quads), 64 lanes = one 1 KB chunk per instruction; the three staging
+  // buffers are padded to whole chunks.  After the stencil phase the same
+  // LDS holds each wave's prefetch of the epilogue's _p and _f values: one
+  // 1 KB DMA per FP64-stored field (lanes 0-31 _p, 32-63 _f of the wave's 64
+  // points) and one per two FP32 fields.
+  constexpr bool SPLIT = sizeof(TS) == 4;
+  constexpr int NW = NT / 64;                                   // waves per block
+  constexpr int NPR = HV / 2, NQD = HV / 4;                     // FP64 pairs, FP32 quads per tile
+  constexpr int CPR = (NPR + 63) / 64, CQD = (NQD + 63) / 64;   // chunks per field
+  constexpr int P64 = CPR * 1024, P32 = SPLIT ? CQD * 1024 : P64;  // staging buffer bytes
+  constexpr bool PF = DMA && MODE == 0;                         // epilogue prefetch
+  constexpr int N64 = SPLIT ? 7 : (LIVE_ALPHA ? NF : NF - 1);   // FP64-stored fields updated (ALPHA last)
+  constexpr int N32 = SPLIT ? (LIVE_ALPHA ? NF - 7 : NF - 8) : 0;  // FP32-stored fields updated
+  constexpr int PFR = (N64 + (N32 + 1) / 2) * 1024;             // prefetch bytes per wave
+  constexpr int LSTAGE = 2 * P64 + P32, LPF = PF ? NW * PFR : 0;
+  __shared__ __align__(16) char ldma[DMA ? (LSTAGE > LPF ? LSTAGE : LPF) : 16];
 
   // Register path: the halo tile is loaded in pairs of k-neighbours (HK is even, so a pair
   // never straddles a row): one 16-byte (FP64) or 8-byte (FP32) load per
@@ -360,6 +384,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   for (int f = 0; f < 6; ++f) v[f] = (TC)gc[f];
   v[PHI] = (TC)phi_c;
 
+  // static dust density: the DMA kernels load it with the point values, so
+  // that waiting for it later does not also wait for the epilogue prefetch
+  // (vector memory operations retire in order)
+  TC Dpt = TC(0);
+  if constexpr (DMA) {
+    if (a.matter == MATTER_STATIC) Dpt = (TC)ldo((const TS*)a.DIFFD, idx);
+  }
+
   // inverse conformal metric from DIFF variables (bssn.cc:447-455)
   TC gi[6];
   {
@@ -450,23 +482,17 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
 
   if constexpr (DMA) {
     static_assert(std::is_same<TS, TC>::value, "DMA staging differentiates the stored type");
-    constexpr bool SPLIT = sizeof(TS) == 4;
     static_assert(!(SPLIT && FLUXES), "the dust fluid is FP64 only");
-    static_assert(NT % 64 == 0 && HK % 4 == 0, "whole waves, whole 16-byte groups per halo row");
-    constexpr int NW = NT / 64;                                   // waves per block
-    constexpr int NPR = HV / 2, NQD = HV / 4;                     // 16-byte groups: FP64 pairs, FP32 quads
-    constexpr int CPR = (NPR + 63) / 64, CQD = (NQD + 63) / 64;   // 64-lane chunks per field
+    static_assert(NT % 64 == 0 && HK % 4 == 0 && TK % 4 == 0, "whole waves, whole 16-byte groups per row");
     constexpr int MPR = (CPR + NW - 1) / NW, MQD = (CQD + NW - 1) / NW;  // chunks per wave
-    using TB1 = typename std::conditional<SPLIT, float, double>::type;
-    // Three field tiles, padded to whole chunks; in split precision buffer 1
-    // only ever holds FP32 fields.  Every wave issues the same number of
-    // chunks with all lanes active (no branches around the DMA loads, which
-    // keeps the compiler's per-buffer wait tracking exact): surplus chunks
-    // repeat the last chunk (the same bytes to the same place), surplus lanes
-    // of the last chunk load element 0 into the padding.
-    __shared__ __align__(16) double lb0[CPR * 128];
-    __shared__ __align__(16) TB1 lb1[SPLIT ? CQD * 256 : CPR * 128];
-    __shared__ __align__(16) double lb2[CPR * 128];
+    // Three field tiles; in split precision buffer 1 only ever holds FP32
+    // fields.  Every wave issues the same number of chunks with all lanes
+    // active: surplus chunks repeat the last chunk (the same bytes to the
+    // same place), surplus lanes of the last chunk load element 0 into the
+    // padding.
+    char* const lb0 = ldma;
+    char* const lb1 = ldma + P64;
+    char* const lb2 = ldma + P64 + P32;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), ln = tid & 63;
     // wrapped global element offset of halo element e (e % 4 == 0 for quads,
     // e % 2 == 0 for pairs: the group never leaves its row or wraps inside)
@@ -497,8 +523,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       }
     }
     typedef __attribute__((address_space(3))) char lds_char;
-    const unsigned lds0 = (unsigned)(size_t)(lds_char*)lb0, lds1 = (unsigned)(size_t)(lds_char*)lb1,
-                   lds2 = (unsigned)(size_t)(lds_char*)lb2;
+    const unsigned lds0 = (unsigned)(size_t)(lds_char*)lb0, lds1 = lds0 + P64, lds2 = lds0 + P64 + P32;
     auto dma = [&](const int f, const int b) {
       const unsigned dst = b == 0 ? lds0 : b == 1 ? lds1 : lds2;
       if (!SPLIT || metric_field(f)) {
@@ -530,6 +555,45 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         const int f = seq(s);
         const void* const buf = s % 3 == 0 ? (const void*)lb0 : s % 3 == 1 ? (const void*)lb1 : (const void*)lb2;
         if (valid) { CG_DERIVE(f, buf); }
+      }
+    }
+    // Every point value has arrived by now (the last staging wait drained
+    // the vector memory counter).  The compiler does not see the asm waits:
+    // touching the values here puts its own waits for them here, where they
+    // cost nothing, instead of at their first use in the algebra, where the
+    // epilogue prefetch is in flight (vector memory operations retire in
+    // order, so those waits would also wait for the prefetch).
+  #pragma unroll
+    for (int f = 0; f < NF; ++f) asm volatile("" ::"v"(v[f]));
+    asm volatile("" ::"v"(Dpt));
+    if constexpr (PF) {
+      if (a.stage >= 1) {
+        // The staging buffers are free once every wave has differentiated
+        // the last field.  Each wave then loads the _p (and from stage 2 the
+        // _f) values of its own 64 points into its own LDS region; it waits
+        // for them (vmcnt) before its epilogue, while the algebra runs.
+        dma_wait_barrier<0>();
+        auto poff = [&](int q) -> unsigned {  // element offset of block point q, 0 outside the grid
+          const int qk = q % TK, qj = (q / TK) % TJ, qi = q / (TK * TJ);
+          const int x = i0 + qi, y = j0 + qj, z = k0 + qk;
+          return (x < a.nx && y < a.ny && z < a.nz) ? (unsigned)((x * a.ny + y) * a.nz + z) : 0u;
+        };
+        const unsigned o2 = poff(64 * wv + 2 * (ln & 31)), o4 = poff(64 * wv + 4 * (ln & 15));
+        const unsigned reg = lds0 + wv * PFR;
+        const bool need_f = a.stage >= 2;
+        const bool f64 = (ln >> 5) != 0;  // FP64 chunk: lanes 32-63 load _f
+  #pragma unroll
+        for (int m = 0; m < N64; ++m) {
+          const char* b = m < 7 ? (const char*)RM(m, f64 ? a.s_f : a.s_p) : (const char*)RS(m, f64 ? a.s_f : a.s_p);
+          if (!f64 || need_f) dma16v(b + (size_t)o2 * 8u, reg + m * 1024);
+        }
+        const bool f32 = ((ln >> 4) & 1) != 0;  // FP32 chunk: lanes 16-31 and 48-63 load _f
+  #pragma unroll
+        for (int i = 0; i < (N32 + 1) / 2; ++i) {
+          const int n = 2 * i + (ln >> 5);
+          const char* b = (const char*)RS(7 + (n < N32 ? n : 0), f32 ? a.s_f : a.s_p);
+          if (n < N32 && (!f32 || need_f)) dma16v(b + (size_t)o4 * 4u, reg + (N64 + i) * 1024);
+        }
       }
     }
   } else {
@@ -567,7 +631,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     TC Si[3] = {TC(0), TC(0), TC(0)}, STF[6] = {TC(0), TC(0), TC(0), TC(0), TC(0), TC(0)};
     TC Dc = TC(0), Sc[3] = {TC(0), TC(0), TC(0)}, Wc = TC(1), vc[3] = {TC(0), TC(0), TC(0)};
     if (a.matter == MATTER_STATIC) {
-      const TC D = (TC)ldo((const TS*)a.DIFFD, idx);
+      const TC D = DMA ? Dpt : (TC)ldo((const TS*)a.DIFFD, idx);
       DIFFr = exp((TC)(-6.0 * (phi_c + fv.phi))) * D + rhoF * expm1(TC(-6) * v[PHI]);
     }
     if constexpr (DUST) {
@@ -900,6 +964,15 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     const int st = a.stage;
     rho_new = (double)DIFFr;
     double gn[6], An[6];  // stage 4: the new DIFFgamma_ij and A_ij as stored
+    // DMA kernels: _p and _f of this point from the wave's LDS prefetch
+    const char* const pfr = ldma + (tid >> 6) * PFR;
+    const int pq = tid & 63;
+    auto lpf = [&](int f, int kind) -> double {
+      if (!SPLIT || f < 7) return *reinterpret_cast<const double*>(pfr + f * 1024 + kind * 512 + pq * 8);
+      const int n = f - 7;
+      return (double)*reinterpret_cast<const TS*>(pfr + (N64 + (n >> 1)) * 1024 + (n & 1) * 512 + kind * 256 + pq * 4);
+    };
+    if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   #pragma unroll
     for (int f = 0; f < NF; ++f) {
       if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
@@ -910,9 +983,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
           sto(o, idx, cc);
           continue;
         }
-        const double p = ldo((const double*)RM(f, a.s_p), idx);
+        const double p = PF ? lpf(f, 0) : ldo((const double*)RM(f, a.s_p), idx);
         double* __restrict__ fr = RM(f, a.s_f);
-        const double fo = st >= 2 ? ldo((const double*)fr, idx) : 0.0;
+        const double fo = st >= 2 ? (PF ? lpf(f, 1) : ldo((const double*)fr, idx)) : 0.0;
         const double inc = a.cf * cc;
         const double nv = st == 4 ? fo + (inc + p) : p + a.ca * cc;
         if (f == PHI) phi_new = nv;
@@ -929,9 +1002,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
           sto(o, idx, (TS)cc);
           continue;
         }
-        const double p = (double)ldo((const TS*)RS(f, a.s_p), idx);
+        const double p = PF ? lpf(f, 0) : (double)ldo((const TS*)RS(f, a.s_p), idx);
         TS* __restrict__ fr = RS(f, a.s_f);
-        const double fo = st >= 2 ? (double)ldo((const TS*)fr, idx) : 0.0;
+        const double fo = st >= 2 ? (PF ? lpf(f, 1) : (double)ldo((const TS*)fr, idx)) : 0.0;
         const double inc = a.cf * cc;
         const TS nv = st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc);
         if (f == KK) K_new = (double)nv;
