@@ -387,7 +387,12 @@ int cg_batch_autotune(cg_batch* b, int reps, char* report, size_t report_len) {
     for (int d = 0; d < 2; ++d) {
       b->dma_pick = d == 1;
       if (d == 1 && !batch_dma_fits(b, v)) continue;
-      for (int order = 0; order < 3; ++order) {
+      for (int order = 0; order < 4; ++order) {
+        if (order == 3) {
+          bool ok = true;
+          for (auto* c : b->m) ok = ok && morton_fits((c->p.nx + v.ti - 1) / v.ti, (c->p.ny + v.tj - 1) / v.tj);
+          if (!ok) continue;
+        }
         for (auto* c : b->m) c->tile_order = order;
         if (int rc = build_tables(b, &v)) return rc;
         for (int r = 0; r < n; ++r) {  // retarget stage 2 to slots 0 -> 3

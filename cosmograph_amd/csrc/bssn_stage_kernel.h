@@ -161,6 +161,21 @@ __device__ __forceinline__ void dma_wait_barrier() {
   asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(K) : "memory");
 }
 
+// Morton (Z-order) index m -> tile column (x, y) on a 2^lx x 2^ly grid: the
+// low 2*min(lx, ly) bits of m interleave x (even bits) and y (odd bits), the
+// remaining high bits extend the longer side.
+__device__ __forceinline__ void morton_xy(int m, int lx, int ly, int& x, int& y) {
+  const int l = lx < ly ? lx : ly;
+  x = 0;
+  y = 0;
+  for (int b = 0; b < l; ++b) {
+    x |= ((m >> (2 * b)) & 1) << b;
+    y |= ((m >> (2 * b + 1)) & 1) << b;
+  }
+  const int hi = m >> (2 * l);
+  if (lx > ly) x |= hi << l; else y |= hi << l;
+}
+
 // WPE: minimum waves per SIMD the register allocation must allow.
 // LIVE_ALPHA = false: synchronous gauge with DIFFalpha identically zero (Static
 // lapse, zero initial lapse perturbation).  Every lapse derivative is then an
@@ -224,14 +239,21 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   // and only the halo along the slowest axis comes back from memory.  The
   // slowest axis should be the one with the smallest halo share (the
   // autotuner picks): 0 = j slowest (k, i, j), 1 = k slowest (i, j, k),
-  // 2 = i slowest (k, j, i)
+  // 2 = i slowest (k, j, i).  3 = k fastest, then the (i, j) tile columns in
+  // Morton (Z) order (tile counts along i and j powers of two): a column's
+  // i and j neighbours were visited shortly before or after it at every
+  // scale, so both its i and j halos tend to come from L2.
   int tx, ty, tz;
-  if (a.tile_order == 0) {
+  const bool pow2 = !(a.ntx & (a.ntx - 1)) && !(a.nty & (a.nty - 1));
+  if (a.tile_order == 0 || (a.tile_order == 3 && !pow2)) {
     tz = t % a.ntz; tx = (t / a.ntz) % a.ntx; ty = t / (a.ntz * a.ntx);
   } else if (a.tile_order == 1) {
     tx = t % a.ntx; ty = (t / a.ntx) % a.nty; tz = t / (a.ntx * a.nty);
-  } else {
+  } else if (a.tile_order == 2) {
     tz = t % a.ntz; ty = (t / a.ntz) % a.nty; tx = t / (a.ntz * a.nty);
+  } else {
+    tz = t % a.ntz;
+    morton_xy(t / a.ntz, __builtin_ctz((unsigned)a.ntx), __builtin_ctz((unsigned)a.nty), tx, ty);
   }
   const int i0 = tx * TI, j0 = ty * TJ, k0 = tz * TK;
 

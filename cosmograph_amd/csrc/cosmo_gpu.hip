@@ -380,6 +380,9 @@ bool dma_fits(const cg_ctx* c, const StageVariant& v) {
   return c->p.nz % 4 == 0 && c->p.nz >= kh && c->p.nz > v.tk + kh - 2;
 }
 
+// tile order 3 (Morton columns) needs power-of-two tile counts along i and j
+bool morton_fits(int ntx, int nty) { return ntx > 0 && nty > 0 && !(ntx & (ntx - 1)) && !(nty & (nty - 1)); }
+
 bool alpha_live(const cg_ctx* c) { return c->p.lapse != CG_LAPSE_STATIC || c->alpha_nonzero; }
 
 }  // namespace cg
@@ -616,6 +619,12 @@ int cg_set_lambda(cg_ctx* c, double lambda) {
   return CG_OK;
 }
 
+int cg_set_tile_order(cg_ctx* c, int order) {
+  if (!c || order < 0 || order > 3) return CG_ERR_ARG;
+  c->tile_order = order;
+  return CG_OK;
+}
+
 int cg_set_dma(cg_ctx* c, int on) {
   if (!c) return CG_ERR_ARG;
   c->dma_allowed = on != 0;
@@ -691,6 +700,7 @@ static int launch(cg_ctx* c, StageArgs a) {
   a.ntx = (c->p.nx + c->var->ti - 1) / c->var->ti;
   a.nty = (c->p.ny + c->var->tj - 1) / c->var->tj;
   a.ntz = (c->p.nz + c->var->tk - 1) / c->var->tk;
+  if (a.tile_order == 3 && !morton_fits(a.ntx, a.nty)) a.tile_order = 0;
   a.per_xcd = (a.ntx * a.nty * a.ntz + 7) / 8;
   c->last_rows = (long)a.ntx * a.nty * a.ntz;
   if (c->last_rows > c->ext_rows) return fail(c, CG_ERR_STATE, "extra-field partial buffer too small");
@@ -1078,7 +1088,8 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
     for (int d = 0; d < 2; ++d) {
       if (d == 1 && !(c->dma_allowed && dma_fits(c, v))) continue;
       c->dma_pick = d == 1;
-      for (int order = 0; order < 3; ++order) {
+      for (int order = 0; order < 4; ++order) {
+        if (order == 3 && !morton_fits((c->p.nx + v.ti - 1) / v.ti, (c->p.ny + v.tj - 1) / v.tj)) continue;
         c->tile_order = order;
         StageArgs a = make_args(c, 2, 0, 0, 1, 3, 0);
         int rc = launch(c, a);

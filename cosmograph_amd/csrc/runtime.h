@@ -167,6 +167,7 @@ void* reg_ptr(cg_ctx* c, int f, int slot);
 int halo_for(int order, double ko);
 bool variant_fits(const cg_ctx* c, const StageVariant& v);
 bool dma_fits(const cg_ctx* c, const StageVariant& v);
+bool morton_fits(int ntx, int nty);
 bool alpha_live(const cg_ctx* c);
 StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int normalize);
 int ext_resolve(cg_ctx* c);

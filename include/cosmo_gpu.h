@@ -143,6 +143,13 @@ int cg_autotune(cg_ctx* ctx, int reps, char* report, size_t report_len);
  * No counterpart in the reference (its stencils read memory directly). */
 int cg_set_dma(cg_ctx* ctx, int on);
 
+/* Traversal order of the stage kernel's tiles (the autotuner sets it):
+ * 0, 1, 2 = the j, k, i axis slowest; 3 = k fastest, then the (i, j) tile
+ * columns in Morton order (where the tile counts along i and j are powers
+ * of two; order 0 otherwise).  Results do not depend on it.  No
+ * counterpart in the reference. */
+int cg_set_tile_order(cg_ctx* ctx, int order);
+
 /* Average duration of the last `n` stage-kernel launches recorded with HIP
  * events on the context's stream (ms); enable with cg_set_timing(ctx, 1). */
 int cg_set_timing(cg_ctx* ctx, int on);
