@@ -170,9 +170,9 @@ int cg_batch_create(const cg_params* params, int n, cg_batch** out) {
   }
   BCHK(b, hipSetDevice(p0.device));
   BCHK(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  BCHK(b, hipMalloc((void**)&b->tab, sizeof(StageArgs) * 4 * n));
+  BCHK(b, hipMalloc((void**)&b->tab, sizeof(StageArgs) * 5 * n));
   BCHK(b, hipMalloc((void**)&b->tstart, sizeof(int) * n));
-  BCHK(b, hipHostMalloc((void**)&b->tab_h, sizeof(StageArgs) * 4 * n));
+  BCHK(b, hipHostMalloc((void**)&b->tab_h, sizeof(StageArgs) * 5 * n));
   BCHK(b, hipHostMalloc((void**)&b->tstart_h, sizeof(int) * n));
   BCHK(b, hipEventCreate(&b->e0));
   BCHK(b, hipEventCreate(&b->e1));
@@ -204,17 +204,28 @@ int cg_batch_set_graph(cg_batch* b, int on) {
 // 2: X -> Y, 3: Y -> X, 4: X -> P; _p = slot 0, _f = slot 1
 static const int kIn[4] = {0, 2, 3, 2}, kOut[4] = {2, 3, 2, 0};
 
-// argument tables of the four stages and the tile offsets for variant v
+// argument tables of the four stages and the tile offsets for variant v.
+// With LDS-DMA staging, stage 4 also writes the normalised copy of the new
+// state (stepInit's _a) into the dead slot 3, and table 4 is the stage 1 of
+// every step but the first, which reads DIFFgamma_ij and A_ij from there
+// instead of normalising halos on the fly.
 static int build_tables(cg_batch* b, const StageVariant* v) {
   const int n = (int)b->m.size();
+  const bool dma = batch_dma_fits(b, *v);
   int total = 0;
   for (int r = 0; r < n; ++r) {
     cg_ctx* c = b->m[r];
     const int ntx = (c->p.nx + v->ti - 1) / v->ti, nty = (c->p.ny + v->tj - 1) / v->tj,
               ntz = (c->p.nz + v->tk - 1) / v->tk;
     b->tstart_h[r] = total;
-    for (int st = 1; st <= 4; ++st) {
-      StageArgs a = make_args(c, st, kIn[st - 1], 0, 1, kOut[st - 1], st == 1 ? c->p.normalize_metric : 0);
+    for (int st = 1; st <= 5; ++st) {
+      const int sn = st == 5 ? 1 : st;
+      StageArgs a = make_args(c, sn, kIn[sn - 1], 0, 1, kOut[sn - 1], sn == 1 ? c->p.normalize_metric : 0);
+      if (st == 5) {
+        a.normalize = 0;
+        a.s_in_norm = c->p.normalize_metric ? 3 : -1;
+      }
+      if (st == 4 && dma && c->p.normalize_metric) a.s_norm_out = 3;
       a.ntx = ntx; a.nty = nty; a.ntz = ntz;
       a.per_xcd = 0;          // unused: the batch kernel maps blocks to tiles itself
       a.ext_part = nullptr;   // averages of a batch step are formed on demand per grid
@@ -225,7 +236,7 @@ static int build_tables(cg_batch* b, const StageVariant* v) {
     total += (int)t;
   }
   b->total = total;
-  BCHK(b, hipMemcpyAsync(b->tab, b->tab_h, sizeof(StageArgs) * 4 * n, hipMemcpyHostToDevice, b->stream));
+  BCHK(b, hipMemcpyAsync(b->tab, b->tab_h, sizeof(StageArgs) * 5 * n, hipMemcpyHostToDevice, b->stream));
   BCHK(b, hipMemcpyAsync(b->tstart, b->tstart_h, sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
   return CG_OK;
 }
@@ -247,14 +258,16 @@ static int enqueue_steps(cg_batch* b, const StageVariant* v, int nsteps) {
   const bool dma = batch_dma_fits(b, *v), norm1 = batch_norm_stage1(b);
   for (int s = 0; s < nsteps; ++s)
     for (int st = 1; st <= 4; ++st) {
+      // stage 1 after the first step reads the normalised copy (table 4)
+      const bool pre = dma && s > 0 && st == 1;
       BatchArgs ba{};
-      ba.tab = b->tab + (st - 1) * n;
+      ba.tab = b->tab + (pre ? 4 : st - 1) * n;
       ba.tstart = b->tstart;
       ba.frw = b->frw + ((size_t)s * 4 + (st - 1)) * n * FRW_N;
       ba.nreal = n;
       ba.total = b->total;
       ba.per_xcd = (b->total + 7) / 8;
-      int rc = (dma && !(st == 1 && norm1) ? v->launch_batch_dma : v->launch_batch)(ba, b->stream);
+      int rc = (dma && (pre || !(st == 1 && norm1)) ? v->launch_batch_dma : v->launch_batch)(ba, b->stream);
       if (rc) return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
     }
   return CG_OK;

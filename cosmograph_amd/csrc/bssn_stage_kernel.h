@@ -198,13 +198,18 @@ __device__ __forceinline__ void morton_xy(int m, int lx, int ly, int& x, int& y)
 template <typename TS, typename TC, int R, int H, int TI, int TJ, int TK, bool LIVE_ALPHA, int MODE, bool DMA = false>
 __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, const FrwV fv) {
   constexpr int NT = TI * TJ * TK;
-  // halo along k: H, or for DMA staging H rounded up to a multiple of 4, so
-  // that every 16-byte group of a halo row (2 FP64 or 4 FP32 elements)
-  // starts at z = 0 mod 4 and stays contiguous across the periodic wrap
-  constexpr int KH = DMA ? (H + 3) / 4 * 4 : H;
-  constexpr int HJ = TJ + 2 * H, HK = TK + 2 * KH, HI = TI + 2 * H;
-  constexpr int HV = HI * HJ * HK;
+  // Halo along k: H; with DMA staging rounded up so that every 16-byte group
+  // of a halo row starts at a multiple of its element count in z and stays
+  // contiguous across the periodic wrap: to even for FP64 pairs (KH), to a
+  // multiple of 4 for FP32 quads (KH4, split precision).  The two tile
+  // geometries differ only along k.
+  constexpr bool SPLIT = sizeof(TS) == 4;
+  constexpr int KH = DMA ? (H + 1) / 2 * 2 : H;
+  constexpr int KH4 = DMA && SPLIT ? (H + 3) / 4 * 4 : KH;
+  constexpr int HJ = TJ + 2 * H, HK = TK + 2 * KH, HI = TI + 2 * H, HK4 = TK + 2 * KH4;
+  constexpr int HV = HI * HJ * HK, HV4 = HI * HJ * HK4;
   constexpr int SI = HJ * HK, SJ = HK, SK = 1;
+  constexpr int SI4 = HJ * HK4, SJ4 = HK4;  // FP32 tiles of the DMA path
 
   // DMA path geometry.  Halo tiles move in 16-byte groups (FP64 pairs, FP32
   // quads), 64 lanes = one 1 KB chunk per instruction; the three staging
@@ -212,9 +217,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   // LDS holds each wave's prefetch of the epilogue's _p and _f values: one
   // 1 KB DMA per FP64-stored field (lanes 0-31 _p, 32-63 _f of the wave's 64
   // points) and one per two FP32 fields.
-  constexpr bool SPLIT = sizeof(TS) == 4;
   constexpr int NW = NT / 64;                                   // waves per block
-  constexpr int NPR = HV / 2, NQD = HV / 4;                     // FP64 pairs, FP32 quads per tile
+  constexpr int NPR = HV / 2, NQD = HV4 / 4;                    // FP64 pairs, FP32 quads per tile
   constexpr int CPR = (NPR + 63) / 64, CQD = (NQD + 63) / 64;   // chunks per field
   constexpr int P64 = CPR * 1024, P32 = SPLIT ? CQD * 1024 : P64;  // staging buffer bytes
   constexpr bool PF = DMA && MODE == 0;                         // epilogue prefetch
@@ -264,6 +268,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   const bool valid = (px < a.nx) && (py < a.ny) && (pz < a.nz);
   const unsigned idx = valid ? (unsigned)((px * a.ny + py) * a.nz + pz) : 0u;
   const int c = ((li + H) * HJ + (lj + H)) * HK + (lk + KH);
+  const int c4 = ((li + H) * HJ + (lj + H)) * HK4 + (lk + KH4);  // in the FP32 tiles
 
   // wrapped global offset of the first element of each of this thread's
   // halo pairs (pair p = tid + q*NT, elements 2p and 2p+1); the same share is
@@ -461,51 +466,52 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   // point, folded into the per-point tensors (a macro: the same code as a
   // lambda allocates registers differently and spills)
 #define CG_DERIVE(f, BUF) \
-    if (metric_field(f)) {                                                                                          \
-      const double* __restrict__ L = reinterpret_cast<const double*>(BUF);                                          \
-      if (KO_ORDER && use_ko) ko[f] = (TC)(a.ko_pref * ko_sum<KO_ORDER, double>(L, c, SI, SJ, SK));                 \
-      const double d0 = fd1<R>(L, c, SI) * i1, d1 = fd1<R>(L, c, SJ) * i1, d2 = fd1<R>(L, c, SK) * i1;              \
-      const double e00 = fd2<R>(L, c, SI) * i2, e11 = fd2<R>(L, c, SJ) * i2, e22 = fd2<R>(L, c, SK) * i2;           \
-      const double e01 = fdm<R>(L, c, SI, SJ) * i2, e02 = fdm<R>(L, c, SI, SK) * i2,                                \
-                   e12 = fdm<R>(L, c, SJ, SK) * i2;                                                                 \
-      if (f == PHI) {                                                                                               \
-        dphi[0] = (TC)d0; dphi[1] = (TC)d1; dphi[2] = (TC)d2;                                                       \
-        ddphi[0] = (TC)e00; ddphi[1] = (TC)e01; ddphi[2] = (TC)e02;                                                 \
-        ddphi[3] = (TC)e11; ddphi[4] = (TC)e12; ddphi[5] = (TC)e22;                                                 \
-      } else {                                                                                                      \
-        dg[0][f] = (TC)d0; dg[1][f] = (TC)d1; dg[2][f] = (TC)d2;                                                    \
-        T1[f] = gi[0] * (TC)e00 + gi[3] * (TC)e11 + gi[5] * (TC)e22 +                                               \
-                TC(2) * (gi[1] * (TC)e01 + gi[2] * (TC)e02 + gi[4] * (TC)e12);                                      \
-      }                                                                                                             \
-    } else {                                                                                                        \
-      const TC* __restrict__ L = reinterpret_cast<const TC*>(BUF);                                                  \
-      if (KO_ORDER && use_ko && f < NF) ko[f] = (TC)a.ko_pref * ko_sum<KO_ORDER, TC>(L, c, SI, SJ, SK);             \
-      if (f == ALPHA) {                                                                                             \
-        const TC idx2 = (TC)a.inv_dx2;                                                                              \
-        dal[0] = fd1<R>(L, c, SI) * idx1; dal[1] = fd1<R>(L, c, SJ) * idx1; dal[2] = fd1<R>(L, c, SK) * idx1;       \
-        ddal[0] = fd2<R>(L, c, SI) * idx2; ddal[3] = fd2<R>(L, c, SJ) * idx2; ddal[5] = fd2<R>(L, c, SK) * idx2;    \
-        ddal[1] = fdm<R>(L, c, SI, SJ) * idx2; ddal[2] = fdm<R>(L, c, SI, SK) * idx2;                               \
-        ddal[4] = fdm<R>(L, c, SJ, SK) * idx2;                                                                      \
-      } else if (f == KK) {                                                                                         \
-        dK[0] = fd1<R>(L, c, SI) * idx1; dK[1] = fd1<R>(L, c, SJ) * idx1; dK[2] = fd1<R>(L, c, SK) * idx1;          \
-      } else if (f >= GAM1 && f <= GAM3) {                                                                          \
-        const int kk = f - GAM1;                                                                                    \
-        dG[0][kk] = fd1<R>(L, c, SI) * idx1; dG[1][kk] = fd1<R>(L, c, SJ) * idx1;                                   \
-        dG[2][kk] = fd1<R>(L, c, SK) * idx1;                                                                        \
-      } else if (CONS && f >= A11 && f <= A33) {                                                                    \
-        const int q = f - A11;                                                                                      \
-        dA[0][q] = fd1<R>(L, c, SI) * idx1; dA[1][q] = fd1<R>(L, c, SJ) * idx1; dA[2][q] = fd1<R>(L, c, SK) * idx1; \
-      } else if (FLUXES && f >= FLUX0) {                                                                            \
-        /* dust.cc:203-224: flux q*3 + i is differentiated along i only */                                         \
-        const int q = (f - FLUX0) / 3, dir = (f - FLUX0) % 3;                                                       \
-        ddust[q] += fd1<R>(L, c, dir == 0 ? SI : dir == 1 ? SJ : SK) * idx1;                                        \
-      }                                                                                                             \
+    if (metric_field(f)) {                                                                                               \
+      const double* __restrict__ L = reinterpret_cast<const double*>(BUF);                                               \
+      if (KO_ORDER && use_ko) ko[f] = (TC)(a.ko_pref * ko_sum<KO_ORDER, double>(L, c, SI, SJ, SK));                      \
+      const double d0 = fd1<R>(L, c, SI) * i1, d1 = fd1<R>(L, c, SJ) * i1, d2 = fd1<R>(L, c, SK) * i1;                   \
+      const double e00 = fd2<R>(L, c, SI) * i2, e11 = fd2<R>(L, c, SJ) * i2, e22 = fd2<R>(L, c, SK) * i2;                \
+      const double e01 = fdm<R>(L, c, SI, SJ) * i2, e02 = fdm<R>(L, c, SI, SK) * i2,                                     \
+                   e12 = fdm<R>(L, c, SJ, SK) * i2;                                                                      \
+      if (f == PHI) {                                                                                                    \
+        dphi[0] = (TC)d0; dphi[1] = (TC)d1; dphi[2] = (TC)d2;                                                            \
+        ddphi[0] = (TC)e00; ddphi[1] = (TC)e01; ddphi[2] = (TC)e02;                                                      \
+        ddphi[3] = (TC)e11; ddphi[4] = (TC)e12; ddphi[5] = (TC)e22;                                                      \
+      } else {                                                                                                           \
+        dg[0][f] = (TC)d0; dg[1][f] = (TC)d1; dg[2][f] = (TC)d2;                                                         \
+        T1[f] = gi[0] * (TC)e00 + gi[3] * (TC)e11 + gi[5] * (TC)e22 +                                                    \
+                TC(2) * (gi[1] * (TC)e01 + gi[2] * (TC)e02 + gi[4] * (TC)e12);                                           \
+      }                                                                                                                  \
+    } else {                                                                                                             \
+      const TC* __restrict__ L = reinterpret_cast<const TC*>(BUF);                                                       \
+      if (KO_ORDER && use_ko && f < NF) ko[f] = (TC)a.ko_pref * ko_sum<KO_ORDER, TC>(L, c4, SI4, SJ4, SK);               \
+      if (f == ALPHA) {                                                                                                  \
+        const TC idx2 = (TC)a.inv_dx2;                                                                                   \
+        dal[0] = fd1<R>(L, c4, SI4) * idx1; dal[1] = fd1<R>(L, c4, SJ4) * idx1; dal[2] = fd1<R>(L, c4, SK) * idx1;       \
+        ddal[0] = fd2<R>(L, c4, SI4) * idx2; ddal[3] = fd2<R>(L, c4, SJ4) * idx2; ddal[5] = fd2<R>(L, c4, SK) * idx2;    \
+        ddal[1] = fdm<R>(L, c4, SI4, SJ4) * idx2; ddal[2] = fdm<R>(L, c4, SI4, SK) * idx2;                               \
+        ddal[4] = fdm<R>(L, c4, SJ4, SK) * idx2;                                                                         \
+      } else if (f == KK) {                                                                                              \
+        dK[0] = fd1<R>(L, c4, SI4) * idx1; dK[1] = fd1<R>(L, c4, SJ4) * idx1; dK[2] = fd1<R>(L, c4, SK) * idx1;          \
+      } else if (f >= GAM1 && f <= GAM3) {                                                                               \
+        const int kk = f - GAM1;                                                                                         \
+        dG[0][kk] = fd1<R>(L, c4, SI4) * idx1; dG[1][kk] = fd1<R>(L, c4, SJ4) * idx1;                                    \
+        dG[2][kk] = fd1<R>(L, c4, SK) * idx1;                                                                            \
+      } else if (CONS && f >= A11 && f <= A33) {                                                                         \
+        const int q = f - A11;                                                                                           \
+        dA[0][q] = fd1<R>(L, c4, SI4) * idx1; dA[1][q] = fd1<R>(L, c4, SJ4) * idx1; dA[2][q] = fd1<R>(L, c4, SK) * idx1; \
+      } else if (FLUXES && f >= FLUX0) {                                                                                 \
+        /* dust.cc:203-224: flux q*3 + i is differentiated along i only */                                               \
+        const int q = (f - FLUX0) / 3, dir = (f - FLUX0) % 3;                                                            \
+        ddust[q] += fd1<R>(L, c4, dir == 0 ? SI4 : dir == 1 ? SJ4 : SK) * idx1;                                          \
+      }                                                                                                                  \
     }
 
   if constexpr (DMA) {
     static_assert(std::is_same<TS, TC>::value, "DMA staging differentiates the stored type");
     static_assert(!(SPLIT && FLUXES), "the dust fluid is FP64 only");
-    static_assert(NT % 64 == 0 && HK % 4 == 0 && TK % 4 == 0, "whole waves, whole 16-byte groups per row");
+    static_assert(NT % 64 == 0 && HK % 2 == 0 && HK4 % 4 == 0 && TK % 4 == 0,
+                  "whole waves, whole 16-byte groups per row");
     constexpr int MPR = (CPR + NW - 1) / NW, MQD = (CQD + NW - 1) / NW;  // chunks per wave
     // Three field tiles; in split precision buffer 1 only ever holds FP32
     // fields.  Every wave issues the same number of chunks with all lanes
@@ -516,11 +522,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     char* const lb1 = ldma + P64;
     char* const lb2 = ldma + P64 + P32;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), ln = tid & 63;
-    // wrapped global element offset of halo element e (e % 4 == 0 for quads,
-    // e % 2 == 0 for pairs: the group never leaves its row or wraps inside)
-    auto goff16 = [&](int e) -> unsigned {
-      const int hk = e % HK, hj = (e / HK) % HJ, hi = e / (HK * HJ);
-      int x = i0 + hi - H, y = j0 + hj - H, z = k0 + hk - KH;
+    // wrapped global element offset of halo element e of a tile with rows of
+    // hk_n elements starting kh before the tile (e % 4 == 0 for quads, e % 2
+    // == 0 for pairs: the group never leaves its row or wraps inside)
+    auto goff16 = [&](int e, int hk_n, int kh) -> unsigned {
+      const int hk = e % hk_n, hj = (e / hk_n) % HJ, hi = e / (hk_n * HJ);
+      int x = i0 + hi - H, y = j0 + hj - H, z = k0 + hk - kh;
       x += x < 0 ? a.nx : 0; x -= x >= a.nx ? a.nx : 0;
       y += y < 0 ? a.ny : 0; y -= y >= a.ny ? a.ny : 0;
       z += z < 0 ? a.nz : 0; z -= z >= a.nz ? a.nz : 0;
@@ -534,14 +541,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
 #pragma unroll
     for (int m = 0; m < MPR; ++m) {
       const int g = pchunk(m) * 64 + ln;
-      poff[m] = goff16(2 * (g < NPR ? g : 0));
+      poff[m] = goff16(2 * (g < NPR ? g : 0), HK, KH);
     }
     unsigned qoff[SPLIT ? MQD : 1];
     if constexpr (SPLIT) {
 #pragma unroll
       for (int m = 0; m < MQD; ++m) {
         const int g = qchunk(m) * 64 + ln;
-        qoff[m] = goff16(4 * (g < NQD ? g : 0));
+        qoff[m] = goff16(4 * (g < NQD ? g : 0), HK4, KH4);
       }
     }
     typedef __attribute__((address_space(3))) char lds_char;
