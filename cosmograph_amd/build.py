@@ -22,7 +22,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
 SOURCES = ["stage_inst_f32.hip", "stage_inst_f64.hip", "stage_inst_misc.hip", "stage_inst_dust.hip", "cons_inst.hip",
-           "cosmo_gpu.hip", "batch.hip"]
+           "cosmo_gpu.hip", "batch.hip", "ics.hip"]
 
 
 def _compile(src: str, build_dir: str, verbose: bool) -> str:
@@ -40,7 +40,7 @@ def build(verbose: bool = True) -> str:
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(lambda s: _compile(s, build_dir, verbose), SOURCES))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhipfft"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

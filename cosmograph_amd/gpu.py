@@ -114,6 +114,7 @@ def load_library():
         "cg_kernel_info": ([vp, ip, ip, ip, ip, ip, ip], C.c_int),
         "cg_set_timing": ([vp, C.c_int], C.c_int),
         "cg_set_dma": ([vp, C.c_int], C.c_int),
+        "cg_ics_static_dust": ([vp, dp, C.c_size_t, C.c_double, C.c_double, C.c_int, dp, C.POINTER(C.c_int)], C.c_int),
         "cg_set_tile_order": ([vp, C.c_int], C.c_int),
         "cg_autotune": ([vp, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
         "cg_constraints": ([vp, dp, dp, dp, dp, dp], C.c_int),
@@ -263,6 +264,17 @@ class BSSN:
 
     def set_timing(self, on: bool):
         self._chk(self.L.cg_set_timing(self.h, int(on)))
+
+    def ics_static_dust(self, modes, rms: float, tol: float = 1e-15, max_iter: int = 200):
+        """cg_ics_static_dust: static-dust initial data from the half-spectrum
+        `modes` ([nx][ny][nz/2+1] complex), computed on the device.  Returns
+        (shift c, iterations)."""
+        m = np.ascontiguousarray(modes, dtype=np.complex128)
+        flat = m.view(np.float64).reshape(-1)
+        c, it = C.c_double(), C.c_int()
+        self._chk(self.L.cg_ics_static_dust(self.h, _dptr(flat), flat.size, float(rms), float(tol), int(max_iter),
+                                            C.byref(c), C.byref(it)))
+        return c.value, it.value
 
     def set_dma(self, on: bool):
         """LDS-DMA halo staging where it applies (default on); off = register staging."""

@@ -34,13 +34,20 @@ def _kgrid(n, box):
     return np.sqrt(KX**2 + KY**2 + KZ**2), (2.0 * np.pi / box) ** 2 * (KX**2 + KY**2 + KZ**2)
 
 
-def gaussian_delta(n: int, spectral_index: float, rms: float, seed: int, box: float = 1.0) -> np.ndarray:
-    kmag, _ = _kgrid(n, box)
+def gaussian_modes(n: int, spectral_index: float, seed: int) -> np.ndarray:
+    """Half-spectrum [n][n][n/2+1] of the density contrast before scaling:
+    |k|^(n_s/2) (g1 + i g2) on the shell 1 <= |k| <= N/4, g from a seeded
+    MT19937 stream (the role of the reference's f_field, ICs/ICs.cc:38-134)."""
+    kmag, _ = _kgrid(n, 1.0)
     rng = np.random.Generator(np.random.MT19937(seed))
     amp = np.zeros_like(kmag)
     shell = (kmag >= 1.0) & (kmag <= n / 4.0)
     amp[shell] = kmag[shell] ** (spectral_index / 2.0)
-    modes = amp * (rng.standard_normal(kmag.shape) + 1j * rng.standard_normal(kmag.shape))
+    return amp * (rng.standard_normal(kmag.shape) + 1j * rng.standard_normal(kmag.shape))
+
+
+def gaussian_delta(n: int, spectral_index: float, rms: float, seed: int, box: float = 1.0) -> np.ndarray:
+    modes = gaussian_modes(n, spectral_index, seed)
     delta = np.fft.irfftn(modes, s=(n, n, n), axes=(0, 1, 2))
     delta -= delta.mean()
     s = np.sqrt(np.mean(delta**2))
@@ -80,6 +87,15 @@ def static_dust_ics(n: int, spectral_index: float = -2.0, rms: float = 1e-3, see
     K_frw = -np.sqrt(24.0 * np.pi * RHO_BAR)
     frw = {"phi": 0.0, "K": K_frw, "fluids": [(RHO_BAR, 0.0)]}
     return fields, frw
+
+
+def static_dust_ics_device(g, spectral_index: float = -2.0, rms: float = 1e-3, seed: int = 9,
+                           tol: float = 1e-15, max_iter: int = 200):
+    """static_dust_ics computed on the GPU context g (cosmograph_amd.BSSN of an
+    n^3 grid): the same modes, then the inverse FFT, the constraint solve and
+    the register writes on the device (cg_ics_static_dust).  Returns (c, iterations)."""
+    n = g.params.shape()[0]
+    return g.ics_static_dust(gaussian_modes(n, spectral_index, seed), rms, tol, max_iter)
 
 
 def save_ics(directory: str, fields: dict, frw: dict, lam: float = 0.0) -> None:

@@ -120,6 +120,23 @@ int cg_extra_field_data(cg_ctx* ctx, double out[4]);
  * whose float-rounded value is NaN (CosmoSim::simNumNaNs, sims/sim.cc). */
 int cg_num_nans(cg_ctx* ctx, const char* name, long long* count);
 
+/* Static-dust initial data on the device (hipFFT), this build's recipe (the
+ * reference's static ICs go from a chosen phi to rho, static_ic.cc:185-199,
+ * and its constraint solver is an absent submodule); the host version is
+ * cosmograph_amd/ics.py.  `modes` is the half-spectrum [nx][ny][nz/2+1] of
+ * complex amplitudes (re, im interleaved; n_modes = 2*nx*ny*(nz/2+1)), as the
+ * reference's set_gaussian_random_Phi_N fills its Fourier field
+ * (ICs/ICs.cc:38-134).  delta = its inverse real FFT (the kz = 0 and nz/2
+ * planes taken Hermitian), mean removed, scaled to `rms`; drho = rho_bar delta
+ * (rho_bar = 3/(8 pi)).  lap(u) = -2 pi u^5 (drho - c) with <u^5 (drho - c)> = 0
+ * is solved by spectral fixed-point iteration until max|du| < tol (at most
+ * max_iter).  Writes DIFFphi_p = log u, DIFFD_a = e^{6 phi}(rho_bar + drho - c)
+ * - rho_bar, zero in every other _p register, and the FRW background phi = 0,
+ * K = -sqrt(24 pi rho_bar), one w = 0 fluid of rho_bar (static_ic.cc:372-378).
+ * shift and iters (optional) receive c and the iteration count. */
+int cg_ics_static_dust(cg_ctx* ctx, const double* modes, size_t n_modes, double rms, double tol, int max_iter,
+                       double* shift, int* iters);
+
 /* Synchronise the context's stream. */
 int cg_sync(cg_ctx* ctx);
 
