@@ -42,14 +42,14 @@ def b_alg_words():
     return 17 * F_FIELDS + 4 * G_FIELDS
 
 
-def make_ctx(cg, n, order, precision, ics):
-    fields, frw = ics
+def make_ctx(cg, n, order, precision, modes, rms):
+    """A context with the static-dust ICs of `modes` computed on the device
+    (cg_ics_static_dust: inverse FFT, constraint solve, register writes)."""
     g = cg.BSSN(cg.GpuParams(n=n, order=order, precision=precision, matter="static"))
-    for name, arr in fields.items():
-        g.upload(name, arr)
-    g.frw_set(frw["phi"], frw["K"])
-    for rho, w in frw["fluids"]:
-        g.frw_add_fluid(rho, w)
+    t0 = time.perf_counter()
+    _, iters = g.ics_static_dust(modes, rms)
+    print(f"[bench] device ICs ({precision}): {iters} iterations, {time.perf_counter() - t0:.2f} s",
+          file=sys.stderr, flush=True)
     return g
 
 
@@ -124,13 +124,9 @@ def ensemble_leg(cg, steps, warmup, log):
             params.append(cg.GpuParams(n=n, order=4, precision="fp32_split", matter="static"))
     log(f"[bench] ensemble: {len(params)} realisations, ICs")
     b = cg.Batch(params)
+    from cosmograph_amd.ics import gaussian_modes
     for m, (n, rms, sd) in zip(b.members, spec):
-        fields, frw = cg.static_dust_ics(n, -2.0, rms, sd)
-        for name, arr in fields.items():
-            m.upload(name, arr)
-        m.frw_set(frw["phi"], frw["K"])
-        for rho, w in frw["fluids"]:
-            m.frw_add_fluid(rho, w)
+        m.ics_static_dust(gaussian_modes(n, -2.0, sd), rms)
     tune = b.autotune(3)
     b.step(warmup)
     t0 = time.perf_counter()
@@ -185,11 +181,13 @@ def main():
     import cosmograph_amd as cg
     n, order = args.n, args.order
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    log(f"[bench] ICs {n}^3 P(k)~k^{args.spectral_index} rms {args.rms} seed {args.seed}")
-    ics = cg.static_dust_ics(n, args.spectral_index, args.rms, args.seed)
+    log(f"[bench] ICs {n}^3 P(k)~k^{args.spectral_index} rms {args.rms} seed {args.seed} (modes on the host, "
+        f"transform and constraint solve on the device)")
+    from cosmograph_amd.ics import gaussian_modes
+    modes = gaussian_modes(n, args.spectral_index, args.seed)
 
     log("[bench] FP32 split")
-    g32 = make_ctx(cg, n, order, "fp32_split", ics)
+    g32 = make_ctx(cg, n, order, "fp32_split", modes, args.rms)
     t32, k32_ms, nl32, tune32 = time_steps(g32, args.warmup, args.steps, not args.no_autotune)
     info32 = g32.kernel_info()
     log(f"[bench] FP32 split: {1e3 * t32 / args.steps:.3f} ms/step, kernel {info32}")
@@ -199,7 +197,7 @@ def main():
     res = {}
     if not args.skip_fp64:
         log("[bench] FP64 drop-in")
-        g64 = make_ctx(cg, n, order, "fp64", ics)
+        g64 = make_ctx(cg, n, order, "fp64", modes, args.rms)
         t64, k64_ms, _, tune64 = time_steps(g64, args.warmup, args.steps, not args.no_autotune)
         rate64 = pts * args.steps / t64
         errs = {}
