@@ -509,7 +509,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
 
   if constexpr (DMA) {
     static_assert(std::is_same<TS, TC>::value, "DMA staging differentiates the stored type");
-    static_assert(!(SPLIT && FLUXES), "the dust fluid is FP64 only");
+    static_assert(!(SPLIT && FLUXES), "split-precision dust uses register staging");
     static_assert(NT % 64 == 0 && HK % 2 == 0 && HK4 % 4 == 0 && TK % 4 == 0,
                   "whole waves, whole 16-byte groups per row");
     constexpr int MPR = (CPR + NW - 1) / NW, MQD = (CQD + NW - 1) / NW;  // chunks per wave

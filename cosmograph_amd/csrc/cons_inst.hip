@@ -1,6 +1,6 @@
 // Instantiations of the constraint-diagnostics mode of the stage kernel
 // (BSSN::setConstraintCalcs), FP64 arithmetic from either storage type;
-// with the dust fluid's matter sources for dust contexts (FP64 storage).
+// with the dust fluid's matter sources for dust contexts.
 #include "bssn_stage_kernel.h"
 
 namespace cg {
@@ -18,9 +18,11 @@ static int cons_by_order(int order, const StageArgs& a, hipStream_t s) {
 
 int launch_constraints(int prec, int order, bool live_alpha, bool dust, const StageArgs& a, hipStream_t s) {
   if (dust) {
-    if (prec != 0) return (int)hipErrorInvalidValue;
-    return live_alpha ? cons_by_order<double, true, MODE_DUST_CONS>(order, a, s)
-                      : cons_by_order<double, false, MODE_DUST_CONS>(order, a, s);
+    if (prec == 0)
+      return live_alpha ? cons_by_order<double, true, MODE_DUST_CONS>(order, a, s)
+                        : cons_by_order<double, false, MODE_DUST_CONS>(order, a, s);
+    return live_alpha ? cons_by_order<float, true, MODE_DUST_CONS>(order, a, s)
+                      : cons_by_order<float, false, MODE_DUST_CONS>(order, a, s);
   }
   if (prec == 0)
     return live_alpha ? cons_by_order<double, true, MODE_CONS>(order, a, s)

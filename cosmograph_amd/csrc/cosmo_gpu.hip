@@ -431,9 +431,6 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   if (p.stencil_order != 2 && p.stencil_order != 4 && p.stencil_order != 6 && p.stencil_order != 8) return CG_ERR_ARG;
   if (p.precision != CG_FP64 && p.precision != CG_FP32_SPLIT) return CG_ERR_ARG;
   if (p.matter != CG_MATTER_NONE && p.matter != CG_MATTER_STATIC && p.matter != CG_MATTER_DUST) return CG_ERR_ARG;
-  // the dust fluid evolves with an empty or full FRW reference (dust_ic.cc),
-  // so its fields are not small perturbations: FP64 storage only
-  if (p.matter == CG_MATTER_DUST && p.precision != CG_FP64) return CG_ERR_ARG;
   if (p.lapse < 0 || p.lapse >= LAPSE_N) return CG_ERR_ARG;
   if (!(p.dx > 0) || !std::isfinite(p.dt)) return CG_ERR_ARG;
   cg_ctx* c = new cg_ctx();
@@ -782,8 +779,12 @@ static int step_init_t(cg_ctx* c) {
 static int dust_reg_init(cg_ctx* c, int rp, int ra, int rf) {
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
-  hipLaunchKernelGGL(k_dust_reg_init<double>, dim3(nb), dim3(bs), 0, c->stream, (double*)c->dreg, c->rstride, rp, ra,
-                     rf, c->npts);
+  if (c->prec == CG_FP64)
+    hipLaunchKernelGGL(k_dust_reg_init<double>, dim3(nb), dim3(bs), 0, c->stream, (double*)c->dreg, c->rstride, rp,
+                       ra, rf, c->npts);
+  else
+    hipLaunchKernelGGL(k_dust_reg_init<float>, dim3(nb), dim3(bs), 0, c->stream, (float*)c->dreg, c->rstride, rp,
+                       ra, rf, c->npts);
   HIPCHK(c, hipGetLastError());
   return CG_OK;
 }
@@ -792,9 +793,14 @@ static int dust_reg_init(cg_ctx* c, int rp, int ra, int rf) {
 static int dust_flux(cg_ctx* c, int slot, int normalize) {
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
-  hipLaunchKernelGGL(k_dust_flux<double>, dim3(nb), dim3(bs), 0, c->stream, (const double*)c->met,
-                     (const double*)reg_ptr(c, ALPHA, 0), (const double*)c->dreg, c->rstride, slot, normalize,
-                     c->frw.g_phi, (double*)c->flux, c->npts);
+  if (c->prec == CG_FP64)
+    hipLaunchKernelGGL(k_dust_flux<double>, dim3(nb), dim3(bs), 0, c->stream, (const double*)c->met,
+                       (const double*)reg_ptr(c, ALPHA, 0), (const double*)c->dreg, c->rstride, slot, normalize,
+                       c->frw.g_phi, (double*)c->flux, c->npts);
+  else
+    hipLaunchKernelGGL(k_dust_flux<float>, dim3(nb), dim3(bs), 0, c->stream, (const double*)c->met,
+                       (const float*)reg_ptr(c, ALPHA, 0), (const float*)c->dreg, c->rstride, slot, normalize,
+                       c->frw.g_phi, (float*)c->flux, c->npts);
   HIPCHK(c, hipGetLastError());
   return CG_OK;
 }
@@ -850,8 +856,12 @@ int cg_k_finalize(cg_ctx* c, int n) {
   if (c->dreg) {
     const int bs = 256;
     const long nb = (c->npts + bs - 1) / bs;
-    hipLaunchKernelGGL(k_dust_finalize<double>, dim3(nb), dim3(bs), 0, c->stream, (double*)c->dreg, c->rstride,
-                       c->sP, c->sC, c->sF, c->npts, n, c->p.dt);
+    if (c->prec == CG_FP64)
+      hipLaunchKernelGGL(k_dust_finalize<double>, dim3(nb), dim3(bs), 0, c->stream, (double*)c->dreg, c->rstride,
+                         c->sP, c->sC, c->sF, c->npts, n, c->p.dt);
+    else
+      hipLaunchKernelGGL(k_dust_finalize<float>, dim3(nb), dim3(bs), 0, c->stream, (float*)c->dreg, c->rstride,
+                         c->sP, c->sC, c->sF, c->npts, n, c->p.dt);
     HIPCHK(c, hipGetLastError());
   }
   int rc = c->prec == CG_FP64 ? finalize_t<double>(c, n, src) : finalize_t<float>(c, n, src);

@@ -2,7 +2,8 @@
 sequence sims/dust.cc:53-169) against the CPU oracle: the metric and the
 four fluid registers after fused steps and after every method-level call,
 the constraints with the fluid's sources, and the averages.  FP64 bar:
-<= 1e-12 relative L2 on every register."""
+<= 1e-12 relative L2 on every register; FP32 split (dust registers, fluxes,
+A_ij, K, Gamma^i in FP32, the metric in FP64): <= 1e-5."""
 import sys
 
 import numpy as np
@@ -29,12 +30,12 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def dust_pair(oracle_mod, gpu, shape, order=4, frw=EMPTY_FRW, lam=0.0, seed=61, lapse="Static"):
+def dust_pair(oracle_mod, gpu, shape, order=4, frw=EMPTY_FRW, lam=0.0, seed=61, lapse="Static", precision="fp64"):
     st = smooth_random_state(shape, seed=seed, with_alpha=(lapse != "Static"))
     if not frw["fluids"]:
         st["DIFFK"] = st["DIFFK"] - 0.5  # an expanding box without a reference background
     o, g = setup_pair(oracle_mod, gpu, {"n": shape, "order": order, "matter": "dust", "lam": lam,
-                                        "lapse": lapse, "frw": frw}, state=st)
+                                        "lapse": lapse, "frw": frw, "precision": precision}, state=st)
     rng = np.random.default_rng(seed + 1)
     x = [np.arange(n) / n for n in shape]
     X, Y, Z = np.meshgrid(*x, indexing="ij")
@@ -111,3 +112,16 @@ def test_dust_constraints_and_averages(oracle_mod, gpu):
         for q in (0, 2, 6, 7):
             w = want[c][q]
             assert abs(got[c][q] - w) <= 1e-10 * max(abs(w), abs(want[c][2])), (c, q, got[c][q], w)
+
+
+@pytest.mark.parametrize("order,frw,lam", [(4, "empty", 0.0), (6, "empty", 0.02), (6, "full", 0.0)])
+def test_dust_fp32_split(oracle_mod, gpu, order, frw, lam):
+    shape = (18, 16, 20)
+    _say(f"case dust fp32 split order={order} frw={frw} lam={lam}")
+    o, g = dust_pair(oracle_mod, gpu, shape, order, EMPTY_FRW if frw == "empty" else FULL_FRW, lam,
+                     precision="fp32_split")
+    o.run_steps(3)
+    g.step(3)
+    check(g, o, tol=1e-5)
+    want = o.field("DIFFr_a")
+    assert rel_l2(g.download("DIFFr_a"), want) <= 1e-5
