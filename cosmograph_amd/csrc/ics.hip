@@ -18,7 +18,10 @@
 //   3. DIFFphi_p = log u, DIFFD_a = e^{6 phi}(rho_bar + drho - c) - rho_bar
 //      (the conserved density of components/static/static.cc:32-33), every
 //      other _p register zero, FRW phi = 0, K = -sqrt(24 pi rho_bar), one
-//      w = 0 fluid of density rho_bar (components/static/static_ic.cc:372-378).
+//      w = 0 fluid of density rho_bar (components/static/static_ic.cc:372-378);
+//      for a dust-fluid context the same geometry as full fields with an
+//      empty FRW reference (dust_ic.cc): dust_D_p = e^{6 phi} rho at rest and
+//      DIFFK_p = -sqrt(24 pi (rho_bar + lambda)).
 // Reductions are per-block partials in a fixed order, summed on the host in
 // block order: the result is deterministic.
 #include <hip/hip_runtime.h>
@@ -152,13 +155,17 @@ __global__ void k_update(double* __restrict__ u, const double* __restrict__ x, l
   if (threadIdx.x == 0) part[2 * blockIdx.x] = sh[0];
 }
 
+// DIFFphi = log u and the conserved density e^{6 phi} rho - sub (static dust:
+// sub = rho_bar, DIFFD; dust fluid at rest: sub = 0, D); kk (dust fluid
+// only, else null) = the uniform extrinsic curvature
 template <typename TS>
-__global__ void k_out(const double* __restrict__ u, const double* __restrict__ drho, double c, double rho_bar, long n,
-                      double* __restrict__ phi, TS* __restrict__ D) {
+__global__ void k_out(const double* __restrict__ u, const double* __restrict__ drho, double c, double rho_bar,
+                      double sub, long n, double* __restrict__ phi, TS* __restrict__ D, TS* __restrict__ kk, double K) {
   for (long i = blockIdx.x * (long)kT + threadIdx.x; i < n; i += (long)kT * gridDim.x) {
     const double p = log(u[i]);
     phi[i] = p;
-    D[i] = (TS)(exp(6.0 * p) * ((rho_bar + drho[i]) - c) - rho_bar);
+    D[i] = (TS)(exp(6.0 * p) * ((rho_bar + drho[i]) - c) - sub);
+    if (kk) kk[i] = (TS)K;
   }
 }
 
@@ -259,19 +266,37 @@ int cg_ics_static_dust(cg_ctx* c, const double* modes, size_t n_modes, double rm
   }
   if (it > max_iter) it = max_iter;
 
-  // 3. registers and background
+  // 3. registers and background.  Static dust: the perturbation around the
+  // FRW background.  Dust fluid: the full fields with an empty FRW reference,
+  // as the reference's dust ICs leave it (components/dust_fluid/dust_ic.cc):
+  // dust_D = e^{6 phi} rho (the fluid at rest, S_i = 0) and
+  // DIFFK = -sqrt(24 pi (rho_bar + lambda)), lambda the Lambda energy density.
+  const bool fluid = c->dreg != nullptr;
+  const double K0 = -std::sqrt(24.0 * kPi * (rho_bar + (fluid ? c->lambda : 0.0)));
   for (int f = 0; f < NF; ++f)
     if (f != PHI) HIPCHK(c, hipMemsetAsync(reg_ptr(c, f, c->sP), 0, n * (f <= PHI ? sizeof(double) : c->ts), s));
-  double* phi = (double*)reg_ptr(c, PHI, c->sP);
-  if (c->ts == 8)
-    hipLaunchKernelGGL(k_out<double>, dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, n, phi, (double*)c->DIFFD);
+  if (fluid)
+    for (int q = 0; q < NDUST; ++q)
+      HIPCHK(c, hipMemsetAsync((char*)c->dreg + ((long)q * 4 + c->sP) * c->rstride * (long)c->ts, 0, n * c->ts, s));
   else
-    hipLaunchKernelGGL(k_out<float>, dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, n, phi, (float*)c->DIFFD);
+    HIPCHK(c, hipMemsetAsync(c->DIFFD, 0, n * c->ts, s));
+  double* phi = (double*)reg_ptr(c, PHI, c->sP);
+  void* D = fluid ? (void*)((char*)c->dreg + (long)c->sP * c->rstride * (long)c->ts) : c->DIFFD;
+  void* kk = fluid ? reg_ptr(c, KK, c->sP) : nullptr;
+  const double sub = fluid ? 0.0 : rho_bar;
+  if (c->ts == 8)
+    hipLaunchKernelGGL(k_out<double>, dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, sub, n, phi, (double*)D,
+                       (double*)kk, K0);
+  else
+    hipLaunchKernelGGL(k_out<float>, dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, sub, n, phi, (float*)D,
+                       (float*)kk, K0);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(s));
   c->frw = Frw{};
-  c->frw.set(0.0, -std::sqrt(24.0 * kPi * rho_bar));
-  c->frw.add_fluid(rho_bar, 0.0);
+  if (!fluid) {
+    c->frw.set(0.0, K0);
+    c->frw.add_fluid(rho_bar, 0.0);
+  }
   c->ext_fresh = false;
   c->alpha_nonzero = false;
   c->cons_valid = false;

@@ -133,6 +133,10 @@ int cg_num_nans(cg_ctx* ctx, const char* name, long long* count);
  * max_iter).  Writes DIFFphi_p = log u, DIFFD_a = e^{6 phi}(rho_bar + drho - c)
  * - rho_bar, zero in every other _p register, and the FRW background phi = 0,
  * K = -sqrt(24 pi rho_bar), one w = 0 fluid of rho_bar (static_ic.cc:372-378).
+ * A dust-fluid context gets the same geometry as full fields with an empty
+ * FRW reference, as the reference's dust ICs leave it (dust_ic.cc): dust_D_p
+ * = e^{6 phi}(rho_bar + drho - c), S_i = 0, DIFFK_p = -sqrt(24 pi (rho_bar +
+ * lambda)) with the Lambda energy density set by cg_set_lambda beforehand.
  * shift and iters (optional) receive c and the iteration count. */
 int cg_ics_static_dust(cg_ctx* ctx, const double* modes, size_t n_modes, double rms, double tol, int max_iter,
                        double* shift, int* iters);

@@ -63,3 +63,35 @@ def test_device_ics_evolve_like_host_ics(gpu):
     for f in FIELDS:
         ref = a.download(f + "_p")
         assert rel_l2(b.download(f + "_p"), ref) <= (1e-11 if np.any(ref) else 1e-30), f
+
+
+def test_dust_fluid_ics_are_the_same_geometry(gpu):
+    """A dust-fluid context gets the same state as full fields with an empty
+    FRW reference: the same phi, D = DIFFD + rho_bar, DIFFK = K_FRW, and
+    therefore the same Hamiltonian and momentum constraints as the static
+    context; with Lambda, K includes its energy density."""
+    from cosmograph_amd.ics import RHO_BAR, gaussian_modes
+    n = 32
+    _say(f"case ics dust fluid n={n}")
+    modes = gaussian_modes(n, -3.0, 7)
+    s = gpu.BSSN(gpu.GpuParams(n=n, order=6))
+    s.ics_static_dust(modes, 1e-2)
+    d = gpu.BSSN(gpu.GpuParams(n=n, order=6, matter="dust"))
+    d.ics_static_dust(modes, 1e-2)
+    assert np.array_equal(d.download("DIFFphi_p"), s.download("DIFFphi_p"))
+    assert rel_l2(d.download("dust_D_p"), s.download("DIFFD_a") + RHO_BAR) <= 1e-15
+    for q in ("dust_S1_p", "dust_S2_p", "dust_S3_p"):
+        assert not np.any(d.download(q))
+    K0 = -np.sqrt(24 * np.pi * RHO_BAR)
+    assert np.all(d.download("DIFFK_p") == K0)
+    assert list(d.frw_get()) == [0.0, 0.0, 0.0, 0.0]
+    # H is a cancellation of terms of size 16 pi rho ~ 6: the two formulations
+    # agree to the round-off of those terms
+    hs, hd = s.setConstraintCalcs(), d.setConstraintCalcs()
+    for c in "HM":
+        for q in (0, 2, 7):
+            assert abs(hd[c][q] - hs[c][q]) <= 1e-13 + 1e-9 * abs(hs[c][q]), (c, q, hd[c][q], hs[c][q])
+    lam = 0.7 * RHO_BAR
+    e = gpu.BSSN(gpu.GpuParams(n=n, order=6, matter="dust", lambda_=lam))
+    e.ics_static_dust(modes, 1e-2)
+    assert np.allclose(e.download("DIFFK_p"), -np.sqrt(24 * np.pi * (RHO_BAR + lam)), rtol=1e-15, atol=0)
