@@ -143,6 +143,64 @@ def ensemble_leg(cg, steps, warmup, log):
     return out
 
 
+C5_FIELDS = ["DIFFphi", "DIFFgamma11", "A11", "DIFFK", "Gamma1", "dust_D", "dust_S1"]
+
+
+def c5_leg(cg, args, log):
+    """512^3 BSSN + dust fluid (D, S_i) + Lambda (energy density 0.7 rho_bar,
+    i.e. Lambda = 0.7 * 3 H_I^2), 6th-order stencils, KO coefficient 0.1
+    (the reference's KO is identically zero at order 6), delta ~ k^-3 with rms
+    3e-2, ICs on the device (cg_ics_static_dust: the fluid at rest, empty FRW
+    reference).  FP32 split and FP64 runs of the same ICs; reports the
+    rate, the stage kernel's HBM fraction of B_alg = (17*22)*4 B per
+    point-update, and the Hamiltonian-constraint growth (H L2 after / before)
+    of both.  Prints one JSON line."""
+    from cosmograph_amd.ics import RHO_BAR, gaussian_modes
+    n, order, steps = args.n, args.order, args.steps
+    lam = 0.7 * RHO_BAR
+    log(f"[c5] {n}^3 dust + Lambda, order {order}, {steps} steps; modes")
+    modes = gaussian_modes(n, args.spectral_index, args.seed)
+    runs, keep = {}, {}
+    for prec in ("fp32_split", "fp64"):
+        g = cg.BSSN(cg.GpuParams(n=n, order=order, precision=prec, matter="dust", lambda_=lam, ko_coeff=0.1))
+        t0 = time.perf_counter()
+        _, iters = g.ics_static_dust(modes, args.rms)
+        log(f"[c5] {prec}: device ICs {iters} iterations, {time.perf_counter() - t0:.1f} s")
+        h0 = g.setConstraintCalcs()["H"][7]
+        t, k_ms, nl, tune = time_steps(g, args.warmup, steps, not args.no_autotune)
+        h1 = g.setConstraintCalcs()["H"][7]
+        runs[prec] = {"value": n ** 3 * steps / t, "ms_per_step": 1e3 * t / steps, "stage_kernel_ms": k_ms,
+                      "H_L2_start": h0, "H_L2_end": h1, "H_growth": h1 / h0 if h0 else None,
+                      "kernel": g.kernel_info(), "nan_points": g.numNaNs("DIFFphi_p")}
+        log(f"[c5] {prec}: {runs[prec]['ms_per_step']:.2f} ms/step, H growth {runs[prec]['H_growth']:.4g}")
+        if prec == "fp32_split":
+            keep = {f: g.download(f + "_p") for f in C5_FIELDS}
+        else:
+            errs = {}
+            for f in C5_FIELDS:
+                b = g.download(f + "_p")
+                nb = float(np.sqrt(np.sum(b * b)))
+                d = float(np.sqrt(np.sum((keep[f] - b) ** 2)))
+                errs[f] = float(f"{(d / nb if nb > 0 else d):.3e}")
+            runs["rel_l2_fp32_vs_fp64"] = errs
+        del g
+    b_alg = (17 * 22) * 4   # FP32 words of the dust-fluid step (F = 22, G = 0)
+    achieved = b_alg / 4.0 * n ** 3 / (runs["fp32_split"]["stage_kernel_ms"] * 1e-3) / 1e9
+    r32, r64 = runs["fp32_split"], runs["fp64"]
+    print(json.dumps({
+        "metric": f"BSSN + dust fluid + Lambda RK4 point-updates/s at {n}^3, FP32 split",
+        "value": r32["value"], "unit": "point-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (FP64 metric)", "data": "synthetic: seeded Gaussian random field ICs, constraint-solved",
+        "config": {"workload": f"dust fluid + Lambda {n}^3, order {order}, k^{args.spectral_index}, "
+                               f"rms {args.rms}, Lambda energy density 0.7 rho_bar"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "b_alg_bytes_per_point_update": b_alg,
+        "fp32": r32, "fp64": r64, "fp32_over_fp64": r32["value"] / r64["value"],
+        "rel_l2_fp32_vs_fp64": runs["rel_l2_fp32_vs_fp64"]}))
+
+
 def load_traffic(workload):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(path):
@@ -154,12 +212,14 @@ def load_traffic(workload):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--config", choices=["c3", "c5"], default="c3",
+                    help="c3: the 256^3 static-dust workload (default); c5: 512^3 dust fluid + Lambda")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 200 for c3, 400 for c5)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--n", type=int, default=None, help="grid side (default 256 for c3, 512 for c5)")
     ap.add_argument("--order", type=int, default=6)
     ap.add_argument("--seed", type=int, default=9)
-    ap.add_argument("--rms", type=float, default=1e-2)
+    ap.add_argument("--rms", type=float, default=None, help="rms of delta (default 1e-2 for c3, 3e-2 for c5)")
     ap.add_argument("--spectral-index", type=float, default=-3.0)
     ap.add_argument("--cpu-sample", type=int, default=128, help="grid side of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--skip-fp64", action="store_true")
@@ -168,6 +228,10 @@ def main():
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (PMC byte-counter calibration)")
     args = ap.parse_args()
+    c5 = args.config == "c5"
+    args.steps = args.steps if args.steps is not None else (400 if c5 else 200)
+    args.n = args.n if args.n is not None else (512 if c5 else 256)
+    args.rms = args.rms if args.rms is not None else (3e-2 if c5 else 1e-2)
 
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
@@ -181,6 +245,9 @@ def main():
     import cosmograph_amd as cg
     n, order = args.n, args.order
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if c5:
+        c5_leg(cg, args, log)
+        return
     log(f"[bench] ICs {n}^3 P(k)~k^{args.spectral_index} rms {args.rms} seed {args.seed} (modes on the host, "
         f"transform and constraint solve on the device)")
     from cosmograph_amd.ics import gaussian_modes

@@ -1092,6 +1092,9 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
   c->timing = false;
   int best_order = 0;
   bool best_dma = false;
+  if (c->dreg) {  // dust: the fluxes of the state the timed launches read
+    if (int rc = dust_flux(c, 0, 0)) { c->timing = timing; return rc; }
+  }
   for (const auto& v : stage_variants()) {
     if (!variant_matches(c, v)) continue;
     c->var = &v;
@@ -1102,6 +1105,7 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
         if (order == 3 && !morton_fits((c->p.nx + v.ti - 1) / v.ti, (c->p.ny + v.tj - 1) / v.tj)) continue;
         c->tile_order = order;
         StageArgs a = make_args(c, 2, 0, 0, 1, 3, 0);
+        a.rho_out = nullptr;  // dust: DIFFr_a keeps the last real stage's source
         int rc = launch(c, a);
         if (rc) { c->timing = timing; return rc; }
         HIPCHK(c, hipEventRecord(e0, c->stream));

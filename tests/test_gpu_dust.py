@@ -125,3 +125,21 @@ def test_dust_fp32_split(oracle_mod, gpu, order, frw, lam):
     check(g, o, tol=1e-5)
     want = o.field("DIFFr_a")
     assert rel_l2(g.download("DIFFr_a"), want) <= 1e-5
+
+
+def test_dust_autotune_leaves_the_state(oracle_mod, gpu):
+    """The autotuner's timing launches (stage-2 shape into dead registers)
+    neither change the evolved state nor the kept source DIFFr_a."""
+    shape = (16, 16, 20)
+    _say("case dust autotune")
+    o, a = dust_pair(oracle_mod, gpu, shape, 4, FULL_FRW, 0.0)
+    _, b = dust_pair(oracle_mod, gpu, shape, 4, FULL_FRW, 0.0)
+    a.step(1)
+    b.step(1)
+    r0 = b.download("DIFFr_a")
+    b.autotune(2)
+    assert np.array_equal(b.download("DIFFr_a"), r0)
+    a.step(2)
+    b.step(2)
+    for f in FIELDS + DUST:
+        assert np.array_equal(a.download(f + "_p"), b.download(f + "_p")), f
