@@ -76,6 +76,7 @@ struct StageArgs {
   double* ext_part;       // per-wave partial sums of setExtraFieldData over the output (null: off)
   double phi_F_next;      // FRW phi after this stage's FRW step (weights of the averages)
   double lambda;
+  int dust_bg;            // dust fluid: its sources minus the FRW totals rho_F, S_F (perturbations)
   double ko_pref;         // KO prefactor (sign included), per unit stencil sum
   double gd_c, dw_mu_l, dw_p;  // gauge parameters
   double a_adj, k_damp;

@@ -717,6 +717,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   #pragma unroll
         for (int J = I; J < 3; ++J)
           STF[sy(I, J)] = Sc[I] * Sc[J] / Wl / Dc / detg - TC(1.0 / 3.0) * (e4l * gl[sy(I, J)]) * DIFFS;
+      if (a.dust_bg) {
+        // perturbation form: the FRW fluids carry the mean (their Lambda
+        // fluid cancels the Lambda source added below); the trace-free
+        // stress above is the dust's own
+        DIFFr -= rhoF;
+        DIFFS -= SF;
+      }
     }
     DIFFr += (TC)a.lambda;
     DIFFS += TC(-3) * (TC)a.lambda;

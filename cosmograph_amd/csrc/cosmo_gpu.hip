@@ -616,6 +616,13 @@ int cg_set_lambda(cg_ctx* c, double lambda) {
   return CG_OK;
 }
 
+int cg_set_dust_background(cg_ctx* c, int on) {
+  if (!c) return CG_ERR_ARG;
+  if (int rc = ext_resolve(c)) return rc;
+  c->dust_bg = on != 0;
+  return CG_OK;
+}
+
 int cg_set_tile_order(cg_ctx* c, int order) {
   if (!c || order < 0 || order > 3) return CG_ERR_ARG;
   c->tile_order = order;
@@ -665,6 +672,7 @@ StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int
   a.gd_c = c->gd_c; a.dw_mu_l = c->dw_mu_l; a.dw_p = c->dw_p;
   a.phi_F_next = c->frw.g_phi;
   a.lambda = c->lambda;
+  a.dust_bg = c->dust_bg ? 1 : 0;
   a.dreg = c->dreg;
   a.flux = c->flux;
   a.s_lag = rin;

@@ -21,7 +21,9 @@
 //      w = 0 fluid of density rho_bar (components/static/static_ic.cc:372-378);
 //      for a dust-fluid context the same geometry as full fields with an
 //      empty FRW reference (dust_ic.cc): dust_D_p = e^{6 phi} rho at rest and
-//      DIFFK_p = -sqrt(24 pi (rho_bar + lambda)).
+//      DIFFK_p = -sqrt(24 pi (rho_bar + lambda)); with the dust background on,
+//      DIFFK_p = 0 and that K with the fluids rho_bar (w = 0) and lambda
+//      (w = -1) in the FRW reference.
 // Reductions are per-block partials in a fixed order, summed on the host in
 // block order: the result is deterministic.
 #include <hip/hip_runtime.h>
@@ -273,6 +275,9 @@ int cg_ics_static_dust(cg_ctx* c, const double* modes, size_t n_modes, double rm
   // DIFFK = -sqrt(24 pi (rho_bar + lambda)), lambda the Lambda energy density.
   const bool fluid = c->dreg != nullptr;
   const double K0 = -std::sqrt(24.0 * kPi * (rho_bar + (fluid ? c->lambda : 0.0)));
+  // dust fluid as perturbations (cg_set_dust_background): the mean goes to
+  // the FRW fluids (the dust and, for Lambda, a w = -1 fluid) and DIFFK = 0
+  const bool bg = fluid && c->dust_bg;
   for (int f = 0; f < NF; ++f)
     if (f != PHI) HIPCHK(c, hipMemsetAsync(reg_ptr(c, f, c->sP), 0, n * (f <= PHI ? sizeof(double) : c->ts), s));
   if (fluid)
@@ -282,7 +287,7 @@ int cg_ics_static_dust(cg_ctx* c, const double* modes, size_t n_modes, double rm
     HIPCHK(c, hipMemsetAsync(c->DIFFD, 0, n * c->ts, s));
   double* phi = (double*)reg_ptr(c, PHI, c->sP);
   void* D = fluid ? (void*)((char*)c->dreg + (long)c->sP * c->rstride * (long)c->ts) : c->DIFFD;
-  void* kk = fluid ? reg_ptr(c, KK, c->sP) : nullptr;
+  void* kk = fluid && !bg ? reg_ptr(c, KK, c->sP) : nullptr;
   const double sub = fluid ? 0.0 : rho_bar;
   if (c->ts == 8)
     hipLaunchKernelGGL(k_out<double>, dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, sub, n, phi, (double*)D,
@@ -293,9 +298,10 @@ int cg_ics_static_dust(cg_ctx* c, const double* modes, size_t n_modes, double rm
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(s));
   c->frw = Frw{};
-  if (!fluid) {
+  if (!fluid || bg) {
     c->frw.set(0.0, K0);
     c->frw.add_fluid(rho_bar, 0.0);
+    if (bg && c->lambda != 0.0) c->frw.add_fluid(c->lambda, -1.0);
   }
   c->ext_fresh = false;
   c->alpha_nonzero = false;

@@ -106,6 +106,7 @@ struct cg_ctx {
   // the autotuner; used where the variant has it and the grid fits
   bool dma_allowed = true, dma_pick = true;
   double gd_c = 1.0, dw_mu_l = 0.0, dw_p = 0.0;  // gauge parameters (BSSNGaugeHandler.h:139-149)
+  bool dust_bg = false;                   // dust sources as perturbations of the FRW fluids (cg_set_dust_background)
   bool alpha_nonzero = false;             // a non-zero DIFFalpha was uploaded
   double* cons = nullptr;                 // [CONS_N][pad] per-point constraint values (lazy)
   double* cons_part = nullptr;            // reduction partials

@@ -116,6 +116,7 @@ def load_library():
         "cg_set_dma": ([vp, C.c_int], C.c_int),
         "cg_ics_static_dust": ([vp, dp, C.c_size_t, C.c_double, C.c_double, C.c_int, dp, C.POINTER(C.c_int)], C.c_int),
         "cg_set_tile_order": ([vp, C.c_int], C.c_int),
+        "cg_set_dust_background": ([vp, C.c_int], C.c_int),
         "cg_autotune": ([vp, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
         "cg_constraints": ([vp, dp, dp, dp, dp, dp], C.c_int),
         "cg_stage_time_ms": ([vp, dp, ip], C.c_int),
@@ -279,6 +280,10 @@ class BSSN:
     def set_dma(self, on: bool):
         """LDS-DMA halo staging where it applies (default on); off = register staging."""
         self._chk(self.L.cg_set_dma(self.h, int(on)))
+
+    def set_dust_background(self, on: bool):
+        """Dust fluid as perturbations of the FRW fluids (cg_set_dust_background)."""
+        self._chk(self.L.cg_set_dust_background(self.h, int(on)))
 
     def set_tile_order(self, order: int):
         """Tile traversal order of the stage kernel (0-3; the autotuner sets it)."""

@@ -136,10 +136,23 @@ int cg_num_nans(cg_ctx* ctx, const char* name, long long* count);
  * A dust-fluid context gets the same geometry as full fields with an empty
  * FRW reference, as the reference's dust ICs leave it (dust_ic.cc): dust_D_p
  * = e^{6 phi}(rho_bar + drho - c), S_i = 0, DIFFK_p = -sqrt(24 pi (rho_bar +
- * lambda)) with the Lambda energy density set by cg_set_lambda beforehand.
+ * lambda)) with the Lambda energy density set by cg_set_lambda beforehand;
+ * with cg_set_dust_background on, DIFFK_p = 0 and the FRW reference holds
+ * phi = 0, that K and the fluids rho_bar (w = 0) and lambda (w = -1).
  * shift and iters (optional) receive c and the iteration count. */
 int cg_ics_static_dust(cg_ctx* ctx, const double* modes, size_t n_modes, double rms, double tol, int max_iter,
                        double* shift, int* iters);
+
+/* Dust fluid as perturbations (on = 1; default 0 = the reference's form).
+ * The reference's dust sources (dust.cc:226-264) are the full density and
+ * stress with an empty FRW reference (dust_ic.cc), so the evolved DIFFK is
+ * the full K.  With the background on, the dust's DIFFr and DIFFS are taken
+ * minus the FRW totals rho_F and S_F: the FRW fluids (e.g. the mean dust,
+ * w = 0, and Lambda as a w = -1 fluid of the context's Lambda density) carry
+ * the mean, and DIFFK, DIFFr stay perturbations, as FP32 storage needs.
+ * With an empty FRW both forms are the same.  No counterpart in the
+ * reference. */
+int cg_set_dust_background(cg_ctx* ctx, int on);
 
 /* Synchronise the context's stream. */
 int cg_sync(cg_ctx* ctx);

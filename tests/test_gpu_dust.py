@@ -143,3 +143,42 @@ def test_dust_autotune_leaves_the_state(oracle_mod, gpu):
     b.step(2)
     for f in FIELDS + DUST:
         assert np.array_equal(a.download(f + "_p"), b.download(f + "_p")), f
+
+
+def _dust_ics(gpu, n, prec, lam, bg, modes):
+    g = gpu.BSSN(gpu.GpuParams(n=n, order=6, precision=prec, matter="dust", lambda_=lam))
+    g.set_dust_background(bg)
+    g.ics_static_dust(modes, 3e-2)
+    return g
+
+
+def _totals(g):
+    phiF, KF = g.frw_get()[:2]
+    out = {f: g.download(f + "_p") for f in FIELDS + DUST}
+    out["DIFFphi"] = out["DIFFphi"] + phiF
+    out["DIFFK"] = out["DIFFK"] + KF
+    return out
+
+
+def test_dust_background_is_the_same_physics(gpu):
+    """Dust as perturbations of the FRW fluids (mean dust w = 0, Lambda
+    w = -1) evolves the same full fields as the reference's form with an empty
+    FRW reference; FP32 split needs the perturbation form and matches FP64."""
+    from cosmograph_amd.ics import RHO_BAR, gaussian_modes
+    n, lam = 32, 0.7 * 3.0 / (8.0 * np.pi)
+    _say(f"case dust background n={n}")
+    modes = gaussian_modes(n, -3.0, 11)
+    ref = _dust_ics(gpu, n, "fp64", lam, False, modes)
+    pert = _dust_ics(gpu, n, "fp64", lam, True, modes)
+    assert np.all(pert.download("DIFFK_p") == 0.0)
+    assert abs(pert.frw_get()[2] - (RHO_BAR + lam)) <= 1e-15
+    f32 = _dust_ics(gpu, n, "fp32_split", lam, True, modes)
+    for g in (ref, pert, f32):
+        g.step(3)
+    a, b, c = _totals(ref), _totals(pert), _totals(f32)
+    for f in FIELDS + DUST:
+        if not np.any(a[f]):
+            assert not np.any(b[f]) and not np.any(c[f]), f
+            continue
+        assert rel_l2(b[f], a[f]) <= 1e-10, (f, rel_l2(b[f], a[f]))
+        assert rel_l2(c[f], b[f]) <= 1e-5, (f, rel_l2(c[f], b[f]))
