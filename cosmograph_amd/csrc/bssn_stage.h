@@ -105,9 +105,11 @@ struct BatchArgs {
 
 // per-point outputs of the constraint mode (BSSN::setConstraintCalcs,
 // bssn.cc:965-1071): value, scale and value/scale of H, M, G, A, S, then the
-// ricci_a and AijAij_a arrays of prepBSSNOutput (sims/sim.cc:214-231)
+// ricci_a and AijAij_a arrays of prepBSSNOutput (sims/sim.cc:214-231), the
+// three momentum-constraint components M_I (set1DConstraintOutput,
+// bssn.cc:903-938) and the sources DIFFr, DIFFS of the evaluated state
 enum { CONS_H = 0, CONS_M = 3, CONS_G = 6, CONS_A = 9, CONS_S = 12, CONS_RICCI = 15, CONS_AIJAIJ = 16,
-       CONS_N = 17 };
+       CONS_MI = 17, CONS_RHO = 20, CONS_SRCS = 21, CONS_N = 22 };
 int launch_constraints(int prec, int order, bool live_alpha, bool dust, const StageArgs& a, hipStream_t s);
 
 // BSSN::setExtraFieldData (bssn.cc:165-170) partial sums, one row per

@@ -727,7 +727,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     }
     DIFFr += (TC)a.lambda;
     DIFFS += TC(-3) * (TC)a.lambda;
-    if constexpr (DUST && !CONS) {
+    if constexpr (DUST) {
       if (a.rho_out) a.rho_out[idx] = (double)DIFFr;
     }
 
@@ -897,6 +897,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         const double MS = e6 * (fabs(2.0 / 3.0 * dK[I]) + fabs(PI8S) + 6.0 * fabs(t6) + fabs(td));
         M2 += MI * MI;
         Ms2 += MS * MS;
+        put(CONS_MI + I, MI);  // momentumConstraintCalc(bd, I + 1) (bssn.cc:1102-1118)
       }
       const double Mv = sqrt(M2), Msc = sqrt(Ms2);
       put(CONS_M, Mv); put(CONS_M + 1, Msc); put(CONS_M + 2, Mv / Msc);
@@ -933,6 +934,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       }
       put(CONS_RICCI, ricci);
       put(CONS_AIJAIJ, AijAij);
+      put(CONS_RHO, DIFFr);   // the sources the output step sees (DIFFr_a, DIFFS_a)
+      put(CONS_SRCS, DIFFS);
       return;
     }
 

@@ -6,6 +6,7 @@
 // entry point replaces.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -489,6 +490,7 @@ int cg_destroy(cg_ctx* c) {
   if (c->met) (void)hipFree(c->met);
   if (c->cons) (void)hipFree(c->cons);
   if (c->cons_part) (void)hipFree(c->cons_part);
+  if (c->obuf) (void)hipFree(c->obuf);
   if (c->nan_dev) (void)hipFree(c->nan_dev);
   if (c->ext) (void)hipFree(c->ext);
   if (c->ext_part) (void)hipFree(c->ext_part);
@@ -508,10 +510,16 @@ static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err, size
   *esz = c->ts;
   std::string s(name ? name : "");
   if (s == "DIFFD_a") return c->DIFFD;
-  if (s == "DIFFr_a" && c->rho_src) {  // dust: the last stage's source (read only)
+  if (s == "DIFFr_a" || s == "DIFFS_a") {  // sources (read only)
     if (for_write) { *err = 1; return nullptr; }
     *esz = 8;
-    return c->rho_src;
+    // dust: the source of the last stage, or of the state cg_constraints
+    // evaluated (what initDustStep leaves in DIFFr_a, sims/dust.cc:53-63)
+    if (s == "DIFFr_a" && c->rho_src) return c->rho_src;
+    // otherwise the sources of the state cg_constraints evaluated
+    // (initStaticStep's clearSrc + addBSSNSrc, sims/static.cc:74-82)
+    if (!c->cons_valid) { *err = 3; return nullptr; }
+    return c->cons + (long)(s == "DIFFr_a" ? CONS_RHO : CONS_SRCS) * c->rstride;
   }
   if (s == "ricci_a" || s == "AijAij_a") {
     if (for_write || !c->cons_valid) { *err = 3; return nullptr; }
@@ -1006,8 +1014,12 @@ int cg_kernel_info(const cg_ctx* cc, int* vgprs, int* scratch, int* lds, int* ti
   return rc ? CG_ERR_HIP : CG_OK;
 }
 
-int cg_constraints(cg_ctx* c, double H[8], double M[8], double G[7], double A[7], double S[7]) {
-  if (!c) return CG_ERR_ARG;
+// the per-point constraint pass (prepBSSNOutput's set_bd_values on every
+// point, sims/sim.cc:214-231, with the constraint values of
+// setConstraintCalcs) into c->cons
+static int cons_eval(cg_ctx* c) {
+  // the pending K4Finalize averages read the dust source this pass rewrites
+  if (int rc = ext_resolve(c)) return rc;
   const long pad = c->rstride;
   const int nb = 1024;
   if (!c->cons) {
@@ -1046,6 +1058,14 @@ int cg_constraints(cg_ctx* c, double H[8], double M[8], double G[7], double A[7]
   int rc = launch_constraints(c->prec, c->p.stencil_order, alpha_live(c), c->dreg != nullptr, a, c->stream);
   if (rc) return fail(c, CG_ERR_HIP, std::string("constraint launch: ") + hipGetErrorString((hipError_t)rc));
   c->cons_valid = true;
+  return CG_OK;
+}
+
+int cg_constraints(cg_ctx* c, double H[8], double M[8], double G[7], double A[7], double S[7]) {
+  if (!c) return CG_ERR_ARG;
+  if (int rc = cons_eval(c)) return rc;
+  const long pad = c->rstride;
+  const int nb = 1024;
   double* part = c->cons_part;
   double* fin = part + (size_t)nb * 5 * kStats;  // 30 + 10 + 10 doubles
   hipLaunchKernelGGL(k_cons_partial, dim3(nb), dim3(kRedThreads), 0, c->stream, c->cons, pad, c->npts, part);
@@ -1081,6 +1101,167 @@ int cg_constraints(cg_ctx* c, double H[8], double M[8], double G[7], double A[7]
     o[6] = tot[q * kStats + 5];
     if (q < 2) o[7] = sqrt(tot[q * kStats + 3] * dx3);  // H_L2, M_L2 (bssn.cc:989, 994, 1067)
   }
+  return CG_OK;
+}
+
+// ------------------------------------------------ output-step readers --
+// device staging of small results (strips, partial sums), grown on demand
+static double* out_buf(cg_ctx* c, size_t n) {
+  if (c->obuf_n < n) {
+    if (c->obuf) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipFree(c->obuf);
+      c->obuf = nullptr;
+      c->obuf_n = 0;
+    }
+    if (hipMalloc((void**)&c->obuf, n * sizeof(double)) != hipSuccess) {
+      fail(c, CG_ERR_HIP, "out of device memory for output staging");
+      return nullptr;
+    }
+    c->obuf_n = n;
+  }
+  return c->obuf;
+}
+
+static long wrap_index(long v, long n) { return ((v % n) + n) % n; }
+
+// out[x] = f[base + (x mod wrap) * run], x < n
+template <typename T>
+__global__ void k_strip(const T* __restrict__ f, long base, long run, long wrap, long n, double* __restrict__ out) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n) out[x] = (double)f[base + (x % wrap) * run];
+}
+
+static int strip_gather(cg_ctx* c, const void* d, size_t esz, long base, long run, long wrap, long n, double* dev) {
+  const int bs = 256;
+  const long nb = (n + bs - 1) / bs;
+  if (esz == 8)
+    hipLaunchKernelGGL(k_strip<double>, dim3(nb), dim3(bs), 0, c->stream, (const double*)d, base, run, wrap, n, dev);
+  else
+    hipLaunchKernelGGL(k_strip<float>, dim3(nb), dim3(bs), 0, c->stream, (const float*)d, base, run, wrap, n, dev);
+  HIPCHK(c, hipGetLastError());
+  return CG_OK;
+}
+
+int cg_field_strip(cg_ctx* c, const char* name, int axis, long n1, long n2, double* out, size_t n) {
+  if (!c || !out || axis < 1 || axis > 3) return CG_ERR_ARG;
+  const long nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+  const long ext[3] = {nx, ny, nz};
+  if ((long)n != ext[axis - 1]) return fail(c, CG_ERR_ARG, "a strip holds the grid's extent along its axis");
+  int e;
+  size_t esz;
+  const void* d = resolve(c, name, false, &e, &esz);
+  if (!d) return fail(c, e == 3 ? CG_ERR_STATE : CG_ERR_ARG,
+                      e == 3 ? std::string(name) + " is filled by cg_constraints" : std::string("unknown field ") + name);
+  // field(i, n1, n2), field(n1, j, n2), field(n1, n2, k) with the offsets
+  // wrapped as CosmoArray::idx wraps them (utils/Array.h:149-159)
+  long base, run;
+  if (axis == 1) { base = wrap_index(n1, ny) * nz + wrap_index(n2, nz); run = ny * nz; }
+  else if (axis == 2) { base = wrap_index(n1, nx) * ny * nz + wrap_index(n2, nz); run = nz; }
+  else { base = (wrap_index(n1, nx) * ny + wrap_index(n2, ny)) * nz; run = 1; }
+  double* dev = out_buf(c, n);
+  if (!dev) return CG_ERR_HIP;
+  if (int rc = strip_gather(c, d, esz, base, run, (long)n, (long)n, dev)) return rc;
+  HIPCHK(c, hipMemcpyAsync(out, dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CG_OK;
+}
+
+int cg_constraint_strip(cg_ctx* c, int axis, long n1, long n2, double* H, double* M, size_t n) {
+  if (!c || !H || !M || axis < 1 || axis > 3) return CG_ERR_ARG;
+  const long nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+  const long ext[3] = {nx, ny, nz};
+  if ((long)n != ext[axis - 1]) return fail(c, CG_ERR_ARG, "a strip holds the grid's extent along its axis");
+  if (int rc = cons_eval(c)) return rc;
+  // set1DConstraintOutput passes the running index as i for every axis
+  // (bssn.cc:918-934), so the points are (x, n1, n2) with x wrapped in i
+  const long base = wrap_index(n1, ny) * nz + wrap_index(n2, nz), run = ny * nz;
+  double* dev = out_buf(c, 2 * n);
+  if (!dev) return CG_ERR_HIP;
+  const double* cons = c->cons;
+  if (int rc = strip_gather(c, cons + (long)CONS_H * c->rstride, 8, base, run, nx, (long)n, dev)) return rc;
+  if (int rc = strip_gather(c, cons + (long)(CONS_MI + axis - 1) * c->rstride, 8, base, run, nx, (long)n, dev + n))
+    return rc;
+  HIPCHK(c, hipMemcpyAsync(H, dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(M, dev + n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CG_OK;
+}
+
+// conformal_average / conformal_standard_deviation / volume_average partial
+// sums (utils/math.h:1382-1415, 1458-1470): one row per block of
+// sum w*f, sum w (pass 0) or sum w*(avg - f)^2 (pass 1), w = exp(6(DIFFphi + phi_FRW))
+constexpr int kCwThreads = 256;
+template <typename T>
+__global__ void __launch_bounds__(kCwThreads) k_conformal_sums(const T* __restrict__ f, const double* __restrict__ dphi,
+                                                             double phiF, long n, int pass, double avg,
+                                                             double* __restrict__ part) {
+  __shared__ double s0[kCwThreads], s1[kCwThreads];
+  double a0 = 0.0, a1 = 0.0;
+  for (long x = (long)blockIdx.x * kCwThreads + threadIdx.x; x < n; x += (long)gridDim.x * kCwThreads) {
+    const double w = exp(6.0 * (dphi[x] + phiF));
+    const double v = (double)f[x];
+    if (pass == 0) {
+      a0 += w * v;
+      a1 += w;
+    } else {
+      const double d = avg - v;
+      a0 += w * (d * d);
+    }
+  }
+  s0[threadIdx.x] = a0;
+  s1[threadIdx.x] = a1;
+  __syncthreads();
+  for (int h = kCwThreads / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s0[threadIdx.x] += s0[threadIdx.x + h];
+      s1[threadIdx.x] += s1[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s0[0];
+    part[2 * blockIdx.x + 1] = s1[0];
+  }
+}
+
+int cg_field_stats(cg_ctx* c, const char* name, double out[3]) {
+  if (!c || !out) return CG_ERR_ARG;
+  int e;
+  size_t esz, pesz;
+  const void* d = resolve(c, name, false, &e, &esz);
+  if (!d) return fail(c, e == 3 ? CG_ERR_STATE : CG_ERR_ARG,
+                      e == 3 ? std::string(name) + " is filled by cg_constraints" : std::string("unknown field ") + name);
+  const double* dphi = (const double*)resolve(c, "DIFFphi_a", false, &e, &pesz);
+  const double phiF = c->frw.g_phi;
+  const long n = c->npts;
+  const int nb = (int)std::min<long>(1024, (n + kCwThreads - 1) / kCwThreads);
+  double* dev = out_buf(c, 2 * (size_t)nb);
+  if (!dev) return CG_ERR_HIP;
+  std::vector<double> part(2 * (size_t)nb);
+  auto pass = [&](int which, double avg) -> int {
+    if (esz == 8)
+      hipLaunchKernelGGL(k_conformal_sums<double>, dim3(nb), dim3(kCwThreads), 0, c->stream, (const double*)d, dphi,
+                         phiF, n, which, avg, dev);
+    else
+      hipLaunchKernelGGL(k_conformal_sums<float>, dim3(nb), dim3(kCwThreads), 0, c->stream, (const float*)d, dphi,
+                         phiF, n, which, avg, dev);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(part.data(), dev, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CG_OK;
+  };
+  if (int rc = pass(0, 0.0)) return rc;
+  double swf = 0.0, sw = 0.0;
+  for (int b = 0; b < nb; ++b) { swf += part[2 * b]; sw += part[2 * b + 1]; }  // block order: deterministic
+  const double vol = sw / (double)n;
+  const double avg = swf / (double)n / vol;
+  if (int rc = pass(1, avg)) return rc;
+  double sdev = 0.0;
+  for (int b = 0; b < nb; ++b) sdev += part[2 * b];
+  out[0] = avg;
+  out[1] = sqrt(sdev / (double)(n - 1) / vol);
+  out[2] = vol;
   return CG_OK;
 }
 

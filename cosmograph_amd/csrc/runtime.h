@@ -111,6 +111,8 @@ struct cg_ctx {
   double* cons = nullptr;                 // [CONS_N][pad] per-point constraint values (lazy)
   double* cons_part = nullptr;            // reduction partials
   bool cons_valid = false;                // ricci_a / AijAij_a filled
+  double* obuf = nullptr;                 // device staging of strips and partial sums (lazy)
+  size_t obuf_n = 0;
   long npts = 0, fstride = 0, rstride = 0;
   void* regs = nullptr;   // [11][4][npts] A_ij, DIFFK, Gamma^i, DIFFalpha in the storage type
   double* met = nullptr;  // [7][4][npts] DIFFgamma11..33, DIFFphi registers, always FP64

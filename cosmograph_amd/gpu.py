@@ -119,6 +119,9 @@ def load_library():
         "cg_set_dust_background": ([vp, C.c_int], C.c_int),
         "cg_autotune": ([vp, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
         "cg_constraints": ([vp, dp, dp, dp, dp, dp], C.c_int),
+        "cg_constraint_strip": ([vp, C.c_int, C.c_long, C.c_long, dp, dp, C.c_size_t], C.c_int),
+        "cg_field_strip": ([vp, C.c_char_p, C.c_int, C.c_long, C.c_long, dp, C.c_size_t], C.c_int),
+        "cg_field_stats": ([vp, C.c_char_p, dp], C.c_int),
         "cg_stage_time_ms": ([vp, dp, ip], C.c_int),
         "cg_extra_field_data": ([vp, dp], C.c_int),
         "cg_batch_create": ([C.POINTER(_Params), C.c_int, C.POINTER(vp)], C.c_int),
@@ -234,6 +237,28 @@ class BSSN:
         G, A, S = (C.c_double * 7)(), (C.c_double * 7)(), (C.c_double * 7)()
         self._chk(self.L.cg_constraints(self.h, H, M, G, A, S))
         return {"H": list(H), "M": list(M), "G": list(G), "A": list(A), "S": list(S)}
+
+    def set1DConstraintOutput(self, axis: int = 1, n1: int = 0, n2: int = 0):
+        """BSSN::set1DConstraintOutput: (H, M_axis) at the points (x, n1, n2),
+        x < N_axis (the running index is i for every axis, as in the reference)."""
+        n = self.params.shape()[axis - 1]
+        H, M = np.zeros(n), np.zeros(n)
+        self._chk(self.L.cg_constraint_strip(self.h, int(axis), int(n1), int(n2), _dptr(H), _dptr(M), n))
+        return H, M
+
+    def strip(self, name: str, axis: int = 1, n1: int = 0, n2: int = 0) -> np.ndarray:
+        """The values io_dump_strip writes for a field-map array along `axis`."""
+        n = self.params.shape()[axis - 1]
+        out = np.zeros(n)
+        self._chk(self.L.cg_field_strip(self.h, name.encode(), int(axis), int(n1), int(n2), _dptr(out), n))
+        return out
+
+    def field_stats(self, name: str):
+        """conformal_average, conformal_standard_deviation of a field and the
+        volume_average of DIFFphi_a (io_bssn_dump_statistics' DETAILS)."""
+        out = (C.c_double * 3)()
+        self._chk(self.L.cg_field_stats(self.h, name.encode(), out))
+        return tuple(out)
 
     def extra_field_data(self):
         """BSSN::setExtraFieldData results: K_min, K_avg, avg_vol, rho_avg."""

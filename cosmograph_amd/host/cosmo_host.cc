@@ -43,6 +43,14 @@ std::string ConfigParser::operator()(const std::string& param, const std::string
 }
 
 // ------------------------------------------------------------------ IOData
+std::vector<std::string> ConfigParser::keys(const std::string& prefix) const {
+  std::vector<std::string> out;
+  for (auto it = config.lower_bound(prefix); it != config.end() && it->first.compare(0, prefix.size(), prefix) == 0;
+       ++it)
+    out.push_back(it->first);
+  return out;
+}
+
 IOData::IOData(const std::string& dir_in, bool q) : quiet(q) {
   std::string d = dir_in.empty() ? "output" : dir_in;
   if (d.size() > 1 && d.back() == '/') d.pop_back();
@@ -174,6 +182,20 @@ void BSSN::setConstraintCalcs(double H[8], double M[8], double G[7], double A[7]
   check(cg_constraints(ctx, H, M, G, A, S));
 }
 
+void BSSN::set1DConstraintOutput(double* H, double* M, int axis, long n1, long n2) {
+  const int ext[3] = {p.nx, p.ny, p.nz};
+  check(cg_constraint_strip(ctx, axis, n1, n2, H, M, (size_t)ext[axis - 1]));
+}
+
+bool BSSN::strip(const std::string& name, int axis, long n1, long n2, std::vector<double>& out) {
+  const int ext[3] = {p.nx, p.ny, p.nz};
+  if (axis < 1 || axis > 3) return false;
+  out.assign((size_t)ext[axis - 1], 0.0);
+  return cg_field_strip(ctx, name.c_str(), axis, n1, n2, out.data(), out.size()) == CG_OK;
+}
+
+void BSSN::fieldStats(const std::string& name, double out[3]) { check(cg_field_stats(ctx, name.c_str(), out)); }
+
 void BSSN::setExtraFieldData() {
   double e[4];
   check(cg_extra_field_data(ctx, e));
@@ -289,14 +311,135 @@ void CosmoSim::runCommonStepTasks() {
   }
 }
 
-// output of the step before it is taken (sims/static.cc:111-129): the
-// constraint series of IO/io.cc:240-275 and the FRW statistics of
-// IO/io.cc:341-376, byte-compatible gzip text
+// one line of tab-terminated "%.15g" values appended to <dir><file>
+// (io_dump_strip, IO/io.cc:730-771)
+static void dump_line(IOData* io, const std::vector<double>& v, const std::string& file) {
+  const std::string fn = io->dir() + file;
+  gzFile f = gzopen(fn.c_str(), "ab");
+  if (!f) {
+    io->log("Error opening file: " + fn);
+    return;
+  }
+  char data[35];
+  for (double x : v) {
+    std::snprintf(data, sizeof data, "%.15g\t", x);
+    gzwrite(f, data, (unsigned)std::strlen(data));
+  }
+  gzwrite(f, "\n", 1);
+  gzclose(f);
+}
+
+// raw little-endian doubles, where the reference writes HDF5
+// (io_dump_3dslice / io_dump_2dslice, IO/io.cc:655-717; HDF5 is not here)
+static void dump_raw(IOData* io, const double* v, size_t n, const std::string& file) {
+  const std::string fn = io->dir() + file;
+  FILE* f = std::fopen(fn.c_str(), "wb");
+  if (!f) {
+    io->log("Error opening file: " + fn);
+    return;
+  }
+  std::fwrite(v, sizeof(double), n, f);
+  std::fclose(f);
+}
+
+static bool due(const ConfigParser& cfg, const std::string& key, long step) {
+  const long iv = std::stol(cfg(key, "0"));
+  return iv > 0 && step % iv == 0;
+}
+
+// output of the step before it is taken (sims/static.cc:111-129,
+// sims/dust.cc:94-112): prepBSSNOutput, then io_bssn_fields_snapshot,
+// io_bssn_dump_statistics and io_bssn_constraint_violation (IO/io.cc)
 void CosmoSim::outputStep() {
-  const long ci = std::stol(cfg("IO_constraint_interval", "0"));
-  if (ci > 0 && step % ci == 0) {
-    double H[8] = {0}, M[8] = {0}, G[7] = {0}, A[7] = {0}, S[7] = {0};
-    bssnSim->setConstraintCalcs(H, M, G, A, S);
+  const bool strips = due(cfg, "IO_1D_grid_interval", step);
+  const bool slices2 = due(cfg, "IO_2D_grid_interval", step);
+  const bool slices3 = due(cfg, "IO_3D_grid_interval", step);
+  const bool stats = due(cfg, "IO_bssnstats_interval", step);
+  const bool cons = due(cfg, "IO_constraint_interval", step);
+  const bool cons_snap = std::stol(cfg("IO_constraint_interval", "0")) > 0 &&
+                         step % std::stol(cfg("IO_constraint_snapshot_interval", "999999999")) == 0;
+  double H[8] = {0}, M[8] = {0}, G[7] = {0}, A[7] = {0}, S[7] = {0};
+  // prepBSSNOutput (sims/sim.cc:214-231): ricci_a, AijAij_a and the sources
+  // of this state come from the constraint pass
+  if (strips || slices2 || slices3 || stats || cons || cons_snap) bssnSim->setConstraintCalcs(H, M, G, A, S);
+  const std::string step_str = std::to_string(step);
+
+  // io_bssn_fields_snapshot (IO/io.cc:106-158): config keys IO_<d>D_<name>
+  auto selected = [&](const char* dim) {
+    std::vector<std::string> names;
+    const std::string pre = std::string("IO_") + dim + "_";
+    for (const std::string& key : cfg.keys(pre)) {
+      const std::string name = key.substr(pre.size());
+      auto ends = [&](const char* s) {
+        const size_t l = std::strlen(s);
+        return name.size() > l && name.compare(name.size() - l, l, s) == 0;
+      };
+      if (name == "grid_interval" || ends("_axis") || ends("_xoffset") || ends("_yoffset")) continue;
+      if (std::stoi(cfg(key, "0"))) names.push_back(name);
+    }
+    return names;
+  };
+  if (slices3 || slices2) {
+    for (int d = 3; d >= 2; --d) {
+      if (!(d == 3 ? slices3 : slices2)) continue;
+      for (const std::string& name : selected(d == 3 ? "3D" : "2D")) {
+        std::vector<double> v((size_t)bssnSim->points());
+        if (cg_field_download(bssnSim->context(), name.c_str(), v.data(), v.size()) != CG_OK) {
+          iodata->log("No field " + name + " to write");
+          continue;
+        }
+        const cg_params& q = bssnSim->params();
+        if (d == 3)
+          dump_raw(iodata, v.data(), v.size(), "3D_" + name + step_str + ".3d_grid.f64");
+        else  // the first NY*NZ points: the i = 0 plane
+          dump_raw(iodata, v.data(), (size_t)q.ny * q.nz, "2D_" + name + step_str + ".2d_grid.f64");
+      }
+    }
+  }
+  if (strips) {
+    for (const std::string& name : selected("1D")) {
+      const std::string key = "IO_1D_" + name;
+      std::vector<double> v;
+      if (!bssnSim->strip(name, std::stoi(cfg(key + "_axis", "1")), std::stol(cfg(key + "_xoffset", "0")),
+                          std::stol(cfg(key + "_yoffset", "0")), v)) {
+        iodata->log("No field " + name + " to write");
+        continue;
+      }
+      dump_line(iodata, v, "1D_" + name + ".strip.dat.gz");
+    }
+  }
+
+  // io_bssn_dump_statistics (IO/io.cc:341-403)
+  if (stats) {
+    double f[4];
+    bssnSim->frwGet(f);
+    // phi, K, rho, S of the reference FRW, each followed by a tab
+    const std::string base = cfg("dump_file", "calculated");
+    for (int q = 0; q < 4; ++q) dump_value(iodata, f[q], base + ".frwdat.gz", q < 3 ? "\t" : "\t\n");
+    // DETAILS: conformal average and standard deviation of DIFFphi, DIFFK,
+    // DIFFr, ricci, then the volume average
+    std::vector<double> row;
+    double vol = 1.0;
+    for (const char* name : {"DIFFphi_a", "DIFFK_a", "DIFFr_a", "ricci_a"}) {
+      double st[3];
+      bssnSim->fieldStats(name, st);
+      row.push_back(st[0]);
+      row.push_back(st[1]);
+      vol = st[2];
+    }
+    row.push_back(vol);
+    dump_line(iodata, row, base + ".dat.gz");
+  }
+
+  // io_bssn_constraint_violation (IO/io.cc:172-275)
+  if (cons_snap) {
+    const long nx = bssnSim->params().nx;
+    std::vector<double> h((size_t)nx), m((size_t)nx);
+    bssnSim->set1DConstraintOutput(h.data(), m.data(), 1, 0, 0);
+    dump_line(iodata, h, "1D_hamiltonian_constraints.strip.dat.gz");
+    dump_line(iodata, m, "1D_momentum_constraints.strip.dat.gz");
+  }
+  if (cons) {
     // columns of IO/io.cc:249-275: mean, stdev, max of C/[C], max C (and L2)
     const double* rows[5] = {H, M, G, A, S};
     const char* names[5] = {"H_violations.dat.gz", "M_violations.dat.gz", "G_violations.dat.gz",
@@ -310,14 +453,6 @@ void CosmoSim::outputStep() {
       dump_value(iodata, v[2], names[c], l2 ? "\t" : "\n");
       if (l2) dump_value(iodata, v[7], names[c], "\n");
     }
-  }
-  const long si = std::stol(cfg("IO_bssnstats_interval", "0"));
-  if (si > 0 && step % si == 0) {
-    double f[4];
-    bssnSim->frwGet(f);
-    // phi, K, rho, S of the reference FRW, each followed by a tab
-    const std::string file = cfg("dump_file", "calculated") + ".frwdat.gz";
-    for (int q = 0; q < 4; ++q) dump_value(iodata, f[q], file, q < 3 ? "\t" : "\t\n");
   }
 }
 

@@ -32,6 +32,8 @@ class ConfigParser {
   std::string operator[](const std::string& param) const;                      // required
   std::string operator()(const std::string& param, const std::string& dflt) const;  // optional
   bool has(const std::string& param) const { return config.count(param) != 0; }
+  // parameter names beginning with `prefix`, in sorted order
+  std::vector<std::string> keys(const std::string& prefix) const;
 
  private:
   std::map<std::string, std::string> config;
@@ -79,6 +81,14 @@ class BSSN {
   // one or more whole RK4 steps in the StaticSim/DustSim sequence, fused
   void step(int nsteps = 1);
   void setConstraintCalcs(double H[8], double M[8], double G[7], double A[7], double S[7]);
+  // bssn.cc:903-938: H and M_axis at the N_axis points (x, n1, n2)
+  void set1DConstraintOutput(double* H, double* M, int axis, long n1, long n2);
+  // the values io_dump_strip writes (IO/io.cc:719-771); false if the field
+  // map has no such array here
+  bool strip(const std::string& name, int axis, long n1, long n2, std::vector<double>& out);
+  // conformal_average, conformal_standard_deviation of a field and the
+  // volume_average of DIFFphi_a (IO/io.cc:8-14, utils/math.h:1382-1470)
+  void fieldStats(const std::string& name, double out[3]);
   // refreshes K_min, K_avg, avg_vol, rho_avg (bssn.cc:165-170)
   void setExtraFieldData();
   double K_min = 0, K_avg = 0, avg_vol = 1, rho_avg = 0;

@@ -103,8 +103,33 @@ int cg_rk4_steps(cg_ctx* ctx, int nsteps);
  * Christoffel (G), trace-free A (A) and unit-determinant (S) constraints;
  * H[7], M[7] = sqrt(sum C^2 dx^3).  FP64 arithmetic and deterministic FP64
  * reductions in every precision.  Also fills "ricci_a" and "AijAij_a" for
- * cg_field_download (prepBSSNOutput, sims/sim.cc:214-231). */
+ * cg_field_download (prepBSSNOutput, sims/sim.cc:214-231), and the sources
+ * of the evaluated state as "DIFFr_a" / "DIFFS_a" (what initStaticStep and
+ * initDustStep leave there before the output step, sims/static.cc:74-82,
+ * sims/dust.cc:53-63; with the dust fluid "DIFFr_a" otherwise holds the last
+ * stage's source). */
 int cg_constraints(cg_ctx* ctx, double H[8], double M[8], double G[7], double A[7], double S[7]);
+
+/* BSSN::set1DConstraintOutput (bssn.cc:903-938), read by the
+ * 1D_hamiltonian_constraints / 1D_momentum_constraints strips of
+ * io_bssn_constraint_violation (IO/io.cc:186-235): H and the momentum
+ * constraint component M_axis at n = N_axis points.  As in the reference the
+ * running index is the i index for every axis (points (x mod NX, n1, n2));
+ * offsets wrap periodically.  Evaluates the state as cg_constraints does. */
+int cg_constraint_strip(cg_ctx* ctx, int axis, long n1, long n2, double* H, double* M, size_t n);
+
+/* The values io_dump_strip writes (IO/io.cc:719-771): field(i, n1, n2),
+ * field(n1, j, n2) or field(n1, n2, k) for axis 1, 2, 3, n = N_axis values,
+ * offsets wrapped as CosmoArray::idx wraps them (utils/Array.h:149-159).
+ * Any name cg_field_download accepts. */
+int cg_field_strip(cg_ctx* ctx, const char* name, int axis, long n1, long n2, double* out, size_t n);
+
+/* The DETAILS statistics of io_bssn_dump_statistics (IO/io.cc:8-14,
+ * 341-403): out = {conformal_average, conformal_standard_deviation,
+ * volume_average} (utils/math.h:1382-1415, 1458-1470) of a field-map array,
+ * weighted by exp(6 (DIFFphi_a + phi_FRW)) with the current FRW phi.  FP64
+ * sums, reduced in a fixed order (deterministic). */
+int cg_field_stats(cg_ctx* ctx, const char* name, double out[3]);
 
 /* BSSN::setExtraFieldData results (bssn.cc:165-170; the public members
  * K_min, K_avg, avg_vol, rho_avg of bssn.h:48): out = {K_min, K_avg,

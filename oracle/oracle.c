@@ -638,6 +638,23 @@ void or_bssn_rk_evolve(or_sim *s) {
 }
 
 /* CosmoSim::prepBSSNOutput, sims/sim.cc:214-231 */
+/* BSSN::set1DConstraintOutput, bssn.cc:903-938: H and the momentum
+ * component M_axis at N_axis points; the running index is passed as i for
+ * every axis (bssn.cc:921, 931), and the stencils wrap it periodically. */
+void or_constraint_strip(or_sim *s, int axis, long n1, long n2, double *H, double *M) {
+  const long ext[3] = {s->p.nx, s->p.ny, s->p.nz};
+  const long nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+  const long j = ((n1 % ny) + ny) % ny, k = ((n2 % nz) + nz) % nz;
+  or_point bd;
+  for (long x = 0; x < ext[axis - 1]; ++x) {
+    double Mv[3];
+    fill_point(s, x % nx, j, k, &bd, 0, 1);
+    or_point_momentum(&bd, Mv);
+    H[x] = bd.H;
+    M[x] = Mv[axis - 1];
+  }
+}
+
 void or_bssn_prep_output(or_sim *s) {
   const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
 #pragma omp parallel for schedule(static)
@@ -717,9 +734,20 @@ long or_num_nans(or_sim *s, const char *name) {
   return n;
 }
 
+/* conformal_standard_deviation, utils/math.h:1458-1484 */
+static double conformal_stdev(const or_sim *s, const double *f, const double *dphi, double phiF) {
+  const double avg = conformal_average(s, f, dphi, phiF);
+  double sum = 0.0, vsum = 0.0;
+  for (long x = 0; x < s->npts; ++x) sum += exp(6.0 * (dphi[x] + phiF)) * PW2(avg - f[x]);
+  for (long x = 0; x < s->npts; ++x) vsum += exp(6.0 * (dphi[x] + phiF));
+  double vol = vsum / (double)s->npts;
+  return sqrt(sum / ((double)s->npts - 1.0) / vol);
+}
+
 /* reductions exported for the bitwise checks against utils/math.h:
  * 0 average (1359-1371), 1 volume_average (1382-1393), 2 conformal_average
- * (1403-1415), 3 min (1506-1518), 4 max (1491-1503), 5 numNaNs (1524-1541) */
+ * (1403-1415), 3 min (1506-1518), 4 max (1491-1503), 5 numNaNs (1524-1541),
+ * 6 conformal_standard_deviation (1458-1484) */
 double or_reduction(or_sim *s, int kind, const double *f, const double *dphi, double phiF) {
   const long n = s->npts;
   double acc = 0.0;
@@ -731,6 +759,7 @@ double or_reduction(or_sim *s, int kind, const double *f, const double *dphi, do
     for (long x = 0; x < n; ++x) acc += exp(6.0 * (dphi[x] + phiF));
     return acc / (double)n;
   case 2: return conformal_average(s, f, dphi, phiF);
+  case 6: return conformal_stdev(s, f, dphi, phiF);
   case 3: {
     double m = f[0];
     for (long x = 0; x < n; ++x)

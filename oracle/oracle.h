@@ -130,6 +130,7 @@ int or_run_steps(or_sim *s, int nsteps);
 
 /* reductions (utils/math.h) and RK4Register arithmetic, for bitwise checks */
 double or_reduction(or_sim *s, int kind, const double *f, const double *dphi, double phiF);
+void or_constraint_strip(or_sim *s, int axis, long n1, long n2, double *H, double *M);
 void or_rk4_demo(const double *x0, long n, double h, double c0, double c1, int nsteps, double *traj,
                  double *last_stage_a);
 

@@ -103,6 +103,7 @@ def lib():
         L.or_stencil.restype = C.c_double
         L.or_reduction.argtypes = [vp, C.c_int, dp, dp, C.c_double]
         L.or_reduction.restype = C.c_double
+        L.or_constraint_strip.argtypes = [vp, C.c_int, C.c_long, C.c_long, dp, dp]
         L.or_rk4_demo.argtypes = [dp, C.c_long, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]
         _lib = L
     return _lib
@@ -183,6 +184,19 @@ class Sim:
 
     def num_nans(self, name="DIFFphi_a"):
         return self.L.or_num_nans(self.h, name.encode())
+
+    def constraint_strip(self, axis=1, n1=0, n2=0):
+        n = (self.p.nx, self.p.ny, self.p.nz)[axis - 1]
+        H, M = np.zeros(n), np.zeros(n)
+        self.L.or_constraint_strip(self.h, axis, n1, n2, dptr(H), dptr(M))
+        return H, M
+
+    def conformal_stats(self, name):
+        """conformal_average, conformal_standard_deviation of a field and
+        volume_average of DIFFphi_a, with the current FRW phi."""
+        phiF = self.frw_get()[0]
+        f, ph = dptr(self.field(name)), dptr(self.field("DIFFphi_a"))
+        return tuple(self.L.or_reduction(self.h, k, f, ph, phiF) for k in (2, 6, 1))
 
 
 def point_compute(params: Params, pt: Point) -> Point:

@@ -63,8 +63,9 @@ void ref_ko(const double *f, double coeff, double *out) {
   LOOP3(i, j, k) out[NP_INDEX(i, j, k)] = KO_dissipation_Q(i, j, k, a, coeff);
 }
 
-// out: average, volume_average, conformal_average, min, max, numNaNs
-void ref_reductions(const double *f, const double *dphi, double phiF, double out[6]) {
+// out: average, volume_average, conformal_average, min, max, numNaNs,
+// conformal_standard_deviation
+void ref_reductions(const double *f, const double *dphi, double phiF, double out[7]) {
   arr_t a(NX, NY, NZ), p(NX, NY, NZ);
   load(a, f);
   load(p, dphi);
@@ -74,6 +75,7 @@ void ref_reductions(const double *f, const double *dphi, double phiF, double out
   out[3] = cosmo::min(a);
   out[4] = cosmo::max(a);
   out[5] = (double)numNaNs(a);
+  out[6] = conformal_standard_deviation(a, p, phiF);
 }
 
 // Static::addBSSNSrc on a cleared DIFFr with the given FRW (phi, rho via one

@@ -8,7 +8,7 @@ Vectors (inputs are regenerated from the seeds recorded here):
     for, orders 2/4/6/8 -> sha256 of the FP64 output bytes plus 16 sample
     values per array;
   * reductions: average, volume_average, conformal_average, min, max,
-    numNaNs (utils/math.h:1359-1541);
+    numNaNs, conformal_standard_deviation (utils/math.h:1359-1541);
   * static source: Static::addBSSNSrc (components/static/static.cc:19-36);
   * RK4Register trajectory (utils/RK4Register.h) for x' = c0 cos x + c1 x;
   * FRW<double> stage values (utils/FRW.h) for four fluids.
@@ -61,7 +61,7 @@ def main():
                "double_derivative": {"sha256": digest(d2), "samples": samples(d2)},
                "ko": {"sha256": digest(ko), "samples": samples(ko)}}
         g = np.ascontiguousarray(0.01 * field_input((nx, ny, nz), SEED_FIELD + 100 + order))
-        red = np.zeros(6)
+        red = np.zeros(7)
         R.ref_reductions(po.dptr(f), po.dptr(g), 0.0123, po.dptr(red))
         ent["reductions"] = [float.hex(float(x)) for x in red]
         if order == 4:

@@ -75,6 +75,16 @@ IO_constraint_interval = 1
 IO_bssnstats_interval = 1
 dump_file = calculated
 IO_final_state = 1
+IO_1D_grid_interval = 1
+IO_1D_DIFFphi_a = 1
+IO_1D_DIFFK_a = 1
+IO_1D_DIFFK_a_axis = 3
+IO_1D_DIFFK_a_xoffset = 4
+IO_1D_DIFFK_a_yoffset = 7
+IO_1D_DIFFr_a = 1
+IO_1D_A11_a = 0
+IO_3D_grid_interval = 2
+IO_3D_DIFFK_a = 1
 """)
     r = run_driver(cfg)
     assert "RK steps: 3" in r.stdout                        # steps = 2 runs 3 RK4 steps
@@ -87,9 +97,19 @@ IO_final_state = 1
     o.frw_set(frw["phi"], frw["K"])
     for rho, w in frw["fluids"]:
         o.frw_add_fluid(rho, w)
-    want_H = []
+    want_H, want_strips, want_stats, want_3d = [], [], [], {}
     for s in range(steps + 1):
         want_H.append(oracle_output_constraints(oracle_mod, o, kind == "dust")["H"])
+        want_strips.append({"DIFFphi_a": o.field("DIFFphi_a")[:, 0, 0].copy(),
+                            "DIFFK_a": o.field("DIFFK_a")[4, 7, :].copy(),
+                            "DIFFr_a": o.field("DIFFr_a")[:, 0, 0].copy()})
+        # (value, scale of the field it summarises)
+        want_stats.append([(v, np.abs(o.field(name)).max()) for name in ("DIFFphi_a", "DIFFK_a", "DIFFr_a", "ricci_a")
+                           for v in o.conformal_stats(name)[:2]] + [(o.conformal_stats("DIFFphi_a")[2], 1.0)])
+        if s == 0:
+            want_1d_H, want_1d_M = o.constraint_strip(1, 0, 0)
+        if s % 2 == 0:
+            want_3d[s] = o.field("DIFFK_a").copy()
         o.run_steps(1)
     names = (out / "final_fields.txt").read_text().split()
     assert len(names) == (22 if kind == "dust" else 18)
@@ -109,3 +129,32 @@ IO_final_state = 1
         assert len(rows(out / (f + ".dat.gz"))) == steps + 1
     frw_rows = rows(out / "calculated.frwdat.gz")
     assert len(frw_rows) == steps + 1 and all(len(r_) == 4 for r_ in frw_rows)
+
+    # io_bssn_fields_snapshot: one strip per step and field (IO/io.cc:140-158)
+    assert not (out / "1D_A11_a.strip.dat.gz").exists()
+    for name in ("DIFFphi_a", "DIFFK_a", "DIFFr_a"):
+        got = rows(out / f"1D_{name}.strip.dat.gz")
+        assert len(got) == steps + 1, name
+        for s in range(steps + 1):
+            w = want_strips[s][name]
+            assert len(got[s]) == w.size, name
+            np.testing.assert_allclose(got[s], w, rtol=0, atol=1e-12 * max(np.abs(w).max(), 1e-300), err_msg=name)
+    for s, w in want_3d.items():
+        got = np.fromfile(out / f"3D_DIFFK_a{s}.3d_grid.f64", dtype="<f8").reshape(n, n, n)
+        assert rel_l2(got, w) <= 1e-12
+    assert not (out / "3D_DIFFK_a1.3d_grid.f64").exists()
+    # io_bssn_dump_statistics: conformal average / stdev of DIFFphi, DIFFK,
+    # DIFFr, ricci and the volume average (IO/io.cc:386-403)
+    st = rows(out / "calculated.dat.gz")
+    assert len(st) == steps + 1 and all(len(r_) == 9 for r_ in st)
+    for s in range(steps + 1):
+        for col in range(9):
+            w, scale = want_stats[s][col]
+            assert abs(st[s][col] - w) <= 1e-10 * abs(w) + 1e-13 * scale, (s, col, st[s][col], w)
+    # io_bssn_constraint_violation's 1D strips, at step 0 only (the default
+    # IO_constraint_snapshot_interval, IO/io.cc:186-235)
+    h1 = rows(out / "1D_hamiltonian_constraints.strip.dat.gz")
+    m1 = rows(out / "1D_momentum_constraints.strip.dat.gz")
+    assert len(h1) == 1 and len(m1) == 1 and len(h1[0]) == n
+    np.testing.assert_allclose(h1[0], want_1d_H, rtol=0, atol=1e-10 * np.abs(want_1d_H).max())
+    np.testing.assert_allclose(m1[0], want_1d_M, rtol=0, atol=1e-10 * np.abs(want_1d_M).max())

@@ -67,7 +67,7 @@ def test_reductions_bitwise_vs_reference(oracle_mod, order):
     g = np.ascontiguousarray(0.01 * field_input(shape, GOLD["seed_field"] + 100 + order))
     s = oracle_mod.Sim(oracle_mod.make_params(shape, order=order))
     L = oracle_mod.lib()
-    got = [L.or_reduction(s.h, kind, oracle_mod.dptr(f), oracle_mod.dptr(g), 0.0123) for kind in range(6)]
+    got = [L.or_reduction(s.h, kind, oracle_mod.dptr(f), oracle_mod.dptr(g), 0.0123) for kind in range(7)]
     want = [float.fromhex(x) for x in GOLD["stencils"][str(order)]["reductions"]]
     assert got == want
 
