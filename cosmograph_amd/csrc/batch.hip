@@ -25,6 +25,8 @@
 
 using namespace cg;
 
+constexpr int kTables = 4;  // sets of four stage tables (see build_tables)
+
 struct cg_batch {
   std::vector<cg_ctx*> m;
   int prec = 0, order = 4, H = 2;
@@ -32,6 +34,7 @@ struct cg_batch {
   const StageVariant* var = nullptr;
   int req_ti = 0, req_tj = 0, req_tk = 0;
   bool dma_allowed = true, dma_pick = true;  // LDS-DMA halo staging (as cg_ctx)
+  bool incr_pick = true;                     // RK4 form picked by the autotuner (as cg_ctx)
   // device tables: stage arguments [4][n], first tile of each grid [n],
   // FRW values [steps][4][n][FRW_N]
   StageArgs* tab = nullptr;
@@ -48,9 +51,10 @@ struct cg_batch {
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
   const StageVariant* graph_var = nullptr;
-  bool graph_dma = false;
+  bool graph_dma = false, graph_incr = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double last_ms = 0.0;
+  unsigned* nan_scratch = nullptr;  // NaN count of the autotuner's stage-4 launches
   std::string err;
 };
 
@@ -129,6 +133,7 @@ int cg_batch_destroy(cg_batch* b) {
   if (b->tstart) (void)hipFree(b->tstart);
   if (b->frw) (void)hipFree(b->frw);
   if (b->tab_h) (void)hipHostFree(b->tab_h);
+  if (b->nan_scratch) (void)hipFree(b->nan_scratch);
   if (b->tstart_h) (void)hipHostFree(b->tstart_h);
   if (b->frw_h) (void)hipHostFree(b->frw_h);
   if (b->e0) (void)hipEventDestroy(b->e0);
@@ -170,9 +175,9 @@ int cg_batch_create(const cg_params* params, int n, cg_batch** out) {
   }
   BCHK(b, hipSetDevice(p0.device));
   BCHK(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  BCHK(b, hipMalloc((void**)&b->tab, sizeof(StageArgs) * 5 * n));
+  BCHK(b, hipMalloc((void**)&b->tab, sizeof(StageArgs) * 4 * kTables * n));
   BCHK(b, hipMalloc((void**)&b->tstart, sizeof(int) * n));
-  BCHK(b, hipHostMalloc((void**)&b->tab_h, sizeof(StageArgs) * 5 * n));
+  BCHK(b, hipHostMalloc((void**)&b->tab_h, sizeof(StageArgs) * 4 * kTables * n));
   BCHK(b, hipHostMalloc((void**)&b->tstart_h, sizeof(int) * n));
   BCHK(b, hipEventCreate(&b->e0));
   BCHK(b, hipEventCreate(&b->e1));
@@ -200,43 +205,68 @@ int cg_batch_set_graph(cg_batch* b, int on) {
   return CG_OK;
 }
 
-// stage slots of the fused step: 1: P -> X (normalised on the fly),
+// every member may step in the accumulator-free form (StageArgs::incr)
+static bool batch_incr_allowed(const cg_batch* b) {
+  for (auto* c : b->m)
+    if (!fused_incr(c)) return false;
+  return true;
+}
+static bool batch_incr(const cg_batch* b) { return b->incr_pick && batch_incr_allowed(b); }
+
+// stage slots of the reference's form: 1: P -> X (normalised on the fly),
 // 2: X -> Y, 3: Y -> X, 4: X -> P; _p = slot 0, _f = slot 1
 static const int kIn[4] = {0, 2, 3, 2}, kOut[4] = {2, 3, 2, 0};
 
-// argument tables of the four stages and the tile offsets for variant v.
-// With LDS-DMA staging, stage 4 also writes the normalised copy of the new
-// state (stepInit's _a) into the dead slot 3, and table 4 is the stage 1 of
-// every step but the first, which reads DIFFgamma_ij and A_ij from there
-// instead of normalising halos on the fly.
+// argument tables of the stages (kTables sets of four, [set*4 + stage-1][n])
+// and the tile offsets for variant v.  With LDS-DMA staging, stage 4 also
+// writes the normalised copy of the new state (stepInit's _a) into a dead
+// slot, and stage 1 of every step but the first reads DIFFgamma_ij and A_ij
+// from there instead of normalising halos on the fly.
+//   Accumulator-free form: set 0 is the first step (a1, a2, a3 in slots 1,
+//   2, 3); set m = 1..3 a step whose stage 1 reads the copy in slot m
+//   (slots from incr_slots).
+//   Reference form: set 0 (P -> X -> Y -> X -> P, _f in slot 1; the copy
+//   goes to slot 3), and set 1's stage 1 reads the copy.
 static int build_tables(cg_batch* b, const StageVariant* v) {
   const int n = (int)b->m.size();
-  const bool dma = batch_dma_fits(b, *v);
+  const bool dma = batch_dma_fits(b, *v), incr = batch_incr(b);
   int total = 0;
   for (int r = 0; r < n; ++r) {
     cg_ctx* c = b->m[r];
     const int ntx = (c->p.nx + v->ti - 1) / v->ti, nty = (c->p.ny + v->tj - 1) / v->tj,
               ntz = (c->p.nz + v->tk - 1) / v->tk;
     b->tstart_h[r] = total;
-    for (int st = 1; st <= 5; ++st) {
-      const int sn = st == 5 ? 1 : st;
-      StageArgs a = make_args(c, sn, kIn[sn - 1], 0, 1, kOut[sn - 1], sn == 1 ? c->p.normalize_metric : 0);
-      if (st == 5) {
-        a.normalize = 0;
-        a.s_in_norm = c->p.normalize_metric ? 3 : -1;
+    const int norm = c->p.normalize_metric;
+    for (int set = 0; set < kTables; ++set)
+      for (int st = 1; st <= 4; ++st) {
+        StageArgs a;
+        if (incr) {
+          int s1, s2, s3;
+          incr_slots(set, s1, s2, s3);
+          const int in[4] = {0, s1, s2, s3}, out[4] = {s1, s2, s3, 0};
+          a = make_args(c, st, in[st - 1], 0, st == 4 ? s1 : 1, out[st - 1], st == 1 && set == 0 ? norm : 0);
+          a.incr = 1;
+          if (st == 1 && set > 0 && norm) a.s_in_norm = set;
+          if (st == 4) {
+            a.s_a2 = s2;
+            if (dma && norm) a.s_norm_out = s1;
+          }
+        } else {
+          a = make_args(c, st, kIn[st - 1], 0, 1, kOut[st - 1], st == 1 && set == 0 ? norm : 0);
+          if (st == 1 && set > 0 && norm) a.s_in_norm = 3;
+          if (st == 4 && dma && norm) a.s_norm_out = 3;
+        }
+        a.ntx = ntx; a.nty = nty; a.ntz = ntz;
+        a.per_xcd = 0;          // unused: the batch kernel maps blocks to tiles itself
+        a.ext_part = nullptr;   // averages of a batch step are formed on demand per grid
+        b->tab_h[(set * 4 + st - 1) * n + r] = a;
       }
-      if (st == 4 && dma && c->p.normalize_metric) a.s_norm_out = 3;
-      a.ntx = ntx; a.nty = nty; a.ntz = ntz;
-      a.per_xcd = 0;          // unused: the batch kernel maps blocks to tiles itself
-      a.ext_part = nullptr;   // averages of a batch step are formed on demand per grid
-      b->tab_h[(st - 1) * n + r] = a;
-    }
     const long t = (long)ntx * nty * ntz;
     if (total + t > (1L << 30)) return bfail(b, CG_ERR_ARG, "too many tiles in one batch");
     total += (int)t;
   }
   b->total = total;
-  BCHK(b, hipMemcpyAsync(b->tab, b->tab_h, sizeof(StageArgs) * 5 * n, hipMemcpyHostToDevice, b->stream));
+  BCHK(b, hipMemcpyAsync(b->tab, b->tab_h, sizeof(StageArgs) * 4 * kTables * n, hipMemcpyHostToDevice, b->stream));
   BCHK(b, hipMemcpyAsync(b->tstart, b->tstart_h, sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
   return CG_OK;
 }
@@ -253,15 +283,28 @@ static int ensure_frw(cg_batch* b, int nsteps) {
   return CG_OK;
 }
 
+// the table set of step s (see build_tables) and the stage-4 input slot
+// of the last step
+static int step_set(bool dma, bool incr, int s) {
+  if (!dma || s == 0) return 0;
+  return incr ? (s - 1) % 3 + 1 : 1;  // the copy of step s-1 sits over its a1
+}
+static int last_stage4_input(bool dma, bool incr, int nsteps) {
+  if (!incr) return kIn[3];
+  int s1, s2, s3;
+  incr_slots(step_set(dma, incr, nsteps - 1), s1, s2, s3);
+  return s3;
+}
+
 static int enqueue_steps(cg_batch* b, const StageVariant* v, int nsteps) {
   const int n = (int)b->m.size();
-  const bool dma = batch_dma_fits(b, *v), norm1 = batch_norm_stage1(b);
+  const bool dma = batch_dma_fits(b, *v), norm1 = batch_norm_stage1(b), incr = batch_incr(b);
   for (int s = 0; s < nsteps; ++s)
     for (int st = 1; st <= 4; ++st) {
-      // stage 1 after the first step reads the normalised copy (table 4)
+      // stage 1 after the first step reads the normalised copy
       const bool pre = dma && s > 0 && st == 1;
       BatchArgs ba{};
-      ba.tab = b->tab + (pre ? 4 : st - 1) * n;
+      ba.tab = b->tab + (step_set(dma, incr, s) * 4 + st - 1) * n;
       ba.tstart = b->tstart;
       ba.frw = b->frw + ((size_t)s * 4 + (st - 1)) * n * FRW_N;
       ba.nreal = n;
@@ -309,7 +352,8 @@ int cg_batch_step(cg_batch* b, int nsteps) {
   BCHK(b, hipEventRecord(b->e0, b->stream));
   if (b->use_graph) {
     const bool dma = batch_dma_fits(b, *v);
-    if (!b->graph || b->graph_steps != nsteps || b->graph_var != v || b->graph_dma != dma) {
+    const bool incr = batch_incr(b);
+    if (!b->graph || b->graph_steps != nsteps || b->graph_var != v || b->graph_dma != dma || b->graph_incr != incr) {
       drop_graph(b);
       hipGraph_t g = nullptr;
       BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -323,6 +367,7 @@ int cg_batch_step(cg_batch* b, int nsteps) {
       b->graph_steps = nsteps;
       b->graph_var = v;
       b->graph_dma = dma;
+      b->graph_incr = incr;
     }
     BCHK(b, hipGraphLaunch(b->graph, b->stream));
   } else {
@@ -335,14 +380,15 @@ int cg_batch_step(cg_batch* b, int nsteps) {
   b->last_ms = ms;
 
   // member state as after a fused step: _a, _f alias _p; the averages of
-  // K4Finalize are formed on demand from _p and the stage-4 input (slot 2)
+  // K4Finalize are formed on demand from _p and the stage-4 input
+  const int in4 = last_stage4_input(batch_dma_fits(b, *v), batch_incr(b), nsteps);
   std::string nan_list;
   for (int r = 0; r < n; ++r) {
     cg_ctx* c = b->m[r];
-    c->sP = 0; c->sF = 1; c->sA = 0; c->sC = 2; c->a_is_p = true;
+    c->sP = 0; c->sF = 1; c->sA = 0; c->sC = in4; c->a_is_p = true;
     c->ext_fresh = false;
     c->ext_lazy = true;
-    c->ext_lazy_in = 2;
+    c->ext_lazy_in = in4;
     c->ext_lazy_src = src4[r];
     c->ext_lazy_phiF = c->frw.g_phi;
     unsigned nn = 0;
@@ -381,7 +427,7 @@ int cg_batch_autotune(cg_batch* b, int reps, char* report, size_t report_len) {
     BCHK(b, hipStreamSynchronize(c->stream));
   }
   if (int rc = ensure_frw(b, 1)) return rc;
-  // stage-2-shaped launches: stencils of _p into the dead _f (slot 1) and spare (slot 3)
+  if (!b->nan_scratch) BCHK(b, hipMalloc((void**)&b->nan_scratch, 256));
   for (int r = 0; r < n; ++r) {
     cg_ctx* c = b->m[r];
     double* f = b->frw_h + (size_t)r * FRW_N;
@@ -394,7 +440,7 @@ int cg_batch_autotune(cg_batch* b, int reps, char* report, size_t report_len) {
   const StageVariant* best = nullptr;
   double best_ms = 1e30;
   int best_order = 0;
-  bool best_dma = false;
+  bool best_dma = false, best_incr = false;
   for (const auto& v : stage_variants()) {
     if (!batch_fits(b, v)) continue;
     for (int d = 0; d < 2; ++d) {
@@ -407,45 +453,68 @@ int cg_batch_autotune(cg_batch* b, int reps, char* report, size_t report_len) {
           if (!ok) continue;
         }
         for (auto* c : b->m) c->tile_order = order;
-        if (int rc = build_tables(b, &v)) return rc;
-        for (int r = 0; r < n; ++r) {  // retarget stage 2 to slots 0 -> 3
-          StageArgs& a = b->tab_h[1 * n + r];
-          a.s_in = 0; a.s_out = 3;
-        }
-        BCHK(b, hipMemcpyAsync(b->tab + n, b->tab_h + n, sizeof(StageArgs) * n, hipMemcpyHostToDevice, b->stream));
-        BatchArgs ba{};
-        ba.tab = b->tab + n;
-        ba.tstart = b->tstart;
-        ba.frw = b->frw;
-        ba.nreal = n;
-        ba.total = b->total;
-        ba.per_xcd = (b->total + 7) / 8;
         auto go = d ? v.launch_batch_dma : v.launch_batch;
-        int rc = go(ba, b->stream);  // warm-up
-        if (rc) return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
-        BCHK(b, hipEventRecord(b->e0, b->stream));
-        for (int q = 0; q < reps; ++q)
-          if ((rc = go(ba, b->stream)))
-            return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
-        BCHK(b, hipEventRecord(b->e1, b->stream));
-        BCHK(b, hipEventSynchronize(b->e1));
-        float ms = 0.f;
-        BCHK(b, hipEventElapsedTime(&ms, b->e0, b->e1));
-        const double avg = ms / reps;
-        int vg = 0, sc = 0, ld = 0;
-        (d ? v.info_dma : v.info)(&vg, &sc, &ld);
-        char buf[256];
-        snprintf(buf, sizeof buf,
-                 "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"dma\":%d,\"order\":%d,\"ms\":%.4f,\"vgprs\":%d,"
-                 "\"scratch\":%d}",
-                 rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, d, order, avg, vg, sc);
-        rep += buf;
-        if (avg < best_ms) { best_ms = avg; best = &v; best_order = order; best_dma = d == 1; }
+        for (int form = batch_incr_allowed(b) ? 1 : 0; form >= 0; --form) {
+          b->incr_pick = form == 1;
+          if (int rc = build_tables(b, &v)) return rc;
+          // stage-1, -2 and -4 tables of the first set retargeted to write
+          // the spare slot 3 and read dead slots (stage 1: stencils of _p
+          // and slot 2 as the normalised copy with DMA staging, as in a
+          // later step; stages 2 and 4: stencils of slot 2, stage 4's a1 in
+          // slot 1 and a2 in slot 3; NaN counts to scratch), so the evolved
+          // state is untouched
+          for (int q = 0; q <= 3; ++q)
+            for (int r = 0; r < n; ++r) {
+              StageArgs& a = b->tab_h[q * n + r];
+              a.s_in = q == 0 ? 0 : 2; a.s_out = 3;
+              a.s_f = 1; a.s_a2 = 3;
+              a.s_norm_out = -1;
+              a.nan_count = b->nan_scratch;
+              if (q == 0 && d == 1 && b->m[r]->p.normalize_metric) { a.normalize = 0; a.s_in_norm = 2; }
+            }
+          BCHK(b, hipMemcpyAsync(b->tab, b->tab_h, sizeof(StageArgs) * 4 * n, hipMemcpyHostToDevice, b->stream));
+          double tq[3] = {0.0, 0.0, 0.0};
+          for (int q = 0; q < 3; ++q) {
+            BatchArgs ba{};
+            ba.tab = b->tab + (q == 0 ? 0 : q == 1 ? 1 : 3) * n;
+            ba.tstart = b->tstart;
+            ba.frw = b->frw;
+            ba.nreal = n;
+            ba.total = b->total;
+            ba.per_xcd = (b->total + 7) / 8;
+            auto launch_q = go;  // with DMA, stage 1 reads the stand-in copy instead of normalising
+            int rc = launch_q(ba, b->stream);  // warm-up
+            if (rc) return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
+            BCHK(b, hipEventRecord(b->e0, b->stream));
+            for (int k = 0; k < reps; ++k)
+              if ((rc = launch_q(ba, b->stream)))
+                return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
+            BCHK(b, hipEventRecord(b->e1, b->stream));
+            BCHK(b, hipEventSynchronize(b->e1));
+            float ms = 0.f;
+            BCHK(b, hipEventElapsedTime(&ms, b->e0, b->e1));
+            tq[q] = ms / reps;
+          }
+          const double avg = 0.25 * (tq[0] + 2.0 * tq[1] + tq[2]);  // mean stage of a step
+          int vg = 0, sc = 0, ld = 0;
+          (d ? v.info_dma : v.info)(&vg, &sc, &ld);
+          char buf[320];
+          snprintf(buf, sizeof buf,
+                   "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"dma\":%d,\"order\":%d,\"incr\":%d,\"ms\":%.4f,"
+                   "\"ms1\":%.4f,\"ms2\":%.4f,\"ms4\":%.4f,\"vgprs\":%d,\"scratch\":%d}",
+                   rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, d, order, form, avg, tq[0], tq[1], tq[2], vg,
+                   sc);
+          rep += buf;
+          if (avg < best_ms) {
+            best_ms = avg; best = &v; best_order = order; best_dma = d == 1; best_incr = form == 1;
+          }
+        }
       }
     }
   }
   for (auto* c : b->m) c->tile_order = best_order;
   b->dma_pick = best_dma || !best;
+  b->incr_pick = best_incr || !best;
   rep += "]";
   if (report && report_len) {
     strncpy(report, rep.c_str(), report_len - 1);

@@ -47,6 +47,12 @@ struct StageArgs {
   // (bssn.cc:61-119) into slot s_norm_out for the next step's stage 1
   // (-1: off).
   int s_in_norm, s_norm_out;
+  // Accumulator-free RK4 (fused steps): stages 1-3 write only the next
+  // input a_n = p + ca*c (no _f); stage 4 rebuilds the accumulator of
+  // K3Finalize from the three stage outputs, f = ((a1-p) + 2(a2-p) + (a3-p))/3
+  // with a1 in slot s_f, a2 in slot s_a2 and a3 the stencil input, then
+  // p' = f + (cf*c + p).  13 register sweeps per step instead of 17.
+  int incr, s_a2;
   const void* DIFFD;      // static dust: conserved density perturbation
   // dust fluid (components/dust_fluid/dust.h:32-46): registers of D, S1..S3
   // at dreg + (q*4 + slot)*rstride (storage type); the 12 fluxes

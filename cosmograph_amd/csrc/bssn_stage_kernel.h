@@ -609,7 +609,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         };
         const unsigned o2 = poff(64 * wv + 2 * (ln & 31)), o4 = poff(64 * wv + 4 * (ln & 15));
         const unsigned reg = lds0 + wv * PFR;
-        const bool need_f = a.stage >= 2;
+        const bool need_f = a.incr ? a.stage == 4 : a.stage >= 2;
         const bool f64 = (ln >> 5) != 0;  // FP64 chunk: lanes 32-63 load _f
   #pragma unroll
         for (int m = 0; m < N64; ++m) {
@@ -1012,46 +1012,68 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       return (double)*reinterpret_cast<const TS*>(pfr + (N64 + (n >> 1)) * 1024 + (n & 1) * 512 + kind * 256 + pq * 4);
     };
     if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    // one loop per RK4 form, each with only its own loads and stores
+    auto epilogue = [&](auto incr_tag) {
+      constexpr bool INC = decltype(incr_tag)::value;
   #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
-      const double cc = (double)rhs[f];
-      if (metric_field(f)) {
-        double* __restrict__ o = RM(f, a.s_out);
-        if (st == 0) {
-          sto(o, idx, cc);
-          continue;
-        }
-        const double p = PF ? lpf(f, 0) : ldo((const double*)RM(f, a.s_p), idx);
-        double* __restrict__ fr = RM(f, a.s_f);
-        const double fo = st >= 2 ? (PF ? lpf(f, 1) : ldo((const double*)fr, idx)) : 0.0;
-        const double inc = a.cf * cc;
-        const double nv = st == 4 ? fo + (inc + p) : p + a.ca * cc;
-        if (f == PHI) phi_new = nv;
-        if (f <= G33) gn[f] = nv;
-        sto(o, idx, nv);
-        if (st == 4) {
-          if (f == PHI && nan_as_float(nv)) atomicAdd(a.nan_count, 1u);
+      for (int f = 0; f < NF; ++f) {
+        if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
+        const double cc = (double)rhs[f];
+        if (metric_field(f)) {
+          double* __restrict__ o = RM(f, a.s_out);
+          if (st == 0) {
+            sto(o, idx, cc);
+            continue;
+          }
+          const double p = PF ? lpf(f, 0) : ldo((const double*)RM(f, a.s_p), idx);
+          double* __restrict__ fr = RM(f, a.s_f);
+          double fo = 0.0;
+          if constexpr (!INC) {
+            if (st >= 2) fo = PF ? lpf(f, 1) : ldo((const double*)fr, idx);
+          } else if (st == 4) {
+            const double a1 = PF ? lpf(f, 1) : ldo((const double*)fr, idx);
+            const double a2 = ldo((const double*)RM(f, a.s_a2), idx);
+            const double a3 = ldo(INM(f), idx);
+            fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
+          }
+          const double inc = a.cf * cc;
+          const double nv = st == 4 ? fo + (inc + p) : p + a.ca * cc;
+          if (f == PHI) phi_new = nv;
+          if (f <= G33) gn[f] = nv;
+          sto(o, idx, nv);
+          if (st == 4) {
+            if (f == PHI && nan_as_float(nv)) atomicAdd(a.nan_count, 1u);
+          } else if (!INC) {
+            sto(fr, idx, fo + inc);
+          }
         } else {
-          sto(fr, idx, fo + inc);
+          TS* __restrict__ o = RS(f, a.s_out);
+          if (st == 0) {
+            sto(o, idx, (TS)cc);
+            continue;
+          }
+          const double p = PF ? lpf(f, 0) : (double)ldo((const TS*)RS(f, a.s_p), idx);
+          TS* __restrict__ fr = RS(f, a.s_f);
+          double fo = 0.0;
+          if constexpr (!INC) {
+            if (st >= 2) fo = PF ? lpf(f, 1) : (double)ldo((const TS*)fr, idx);
+          } else if (st == 4) {
+            const double a1 = PF ? lpf(f, 1) : (double)ldo((const TS*)fr, idx);
+            const double a2 = (double)ldo((const TS*)RS(f, a.s_a2), idx);
+            const double a3 = (double)ldo(INS(f), idx);
+            fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
+          }
+          const double inc = a.cf * cc;
+          const TS nv = st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc);
+          if (f == KK) K_new = (double)nv;
+          if (f >= A11 && f <= A33) An[f - A11] = (double)nv;
+          sto(o, idx, nv);
+          if (st != 4 && !INC) sto(fr, idx, (TS)(fo + inc));
         }
-      } else {
-        TS* __restrict__ o = RS(f, a.s_out);
-        if (st == 0) {
-          sto(o, idx, (TS)cc);
-          continue;
-        }
-        const double p = PF ? lpf(f, 0) : (double)ldo((const TS*)RS(f, a.s_p), idx);
-        TS* __restrict__ fr = RS(f, a.s_f);
-        const double fo = st >= 2 ? (PF ? lpf(f, 1) : (double)ldo((const TS*)fr, idx)) : 0.0;
-        const double inc = a.cf * cc;
-        const TS nv = st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc);
-        if (f == KK) K_new = (double)nv;
-        if (f >= A11 && f <= A33) An[f - A11] = (double)nv;
-        sto(o, idx, nv);
-        if (st != 4) sto(fr, idx, (TS)(fo + inc));
       }
-    }
+    };
+    if (a.incr) epilogue(std::true_type{});
+    else epilogue(std::false_type{});
     if (DMA && st == 4 && a.s_norm_out >= 0) {
       // set_DIFFgamma_Aij_norm (bssn.cc:61-119) of the new state, as the
       // next step's stepInit applies it to _a (DMA kernels only: the

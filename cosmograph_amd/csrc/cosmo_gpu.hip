@@ -637,6 +637,12 @@ int cg_set_tile_order(cg_ctx* c, int order) {
   return CG_OK;
 }
 
+int cg_set_fused_accumulator(cg_ctx* c, int keep) {
+  if (!c) return CG_ERR_ARG;
+  c->keep_accumulator = keep != 0;
+  return CG_OK;
+}
+
 int cg_set_dma(cg_ctx* c, int on) {
   if (!c) return CG_ERR_ARG;
   c->dma_allowed = on != 0;
@@ -657,6 +663,7 @@ StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int
   a.rstride = c->rstride;
   a.s_in = rin; a.s_p = rp; a.s_f = rf; a.s_out = rout;
   a.s_in_norm = -1; a.s_norm_out = -1;
+  a.incr = 0; a.s_a2 = -1;
   a.DIFFD = c->DIFFD;
   a.nx = c->p.nx; a.ny = c->p.ny; a.nz = c->p.nz;
   a.tile_order = c->tile_order;
@@ -892,8 +899,7 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
   // fixed physical slots for the fused path: P=0, F=1, X=2, Y=3
   c->sP = 0; c->sF = 1;
   const int X = 2, Y = 3;
-  const bool ko_norm = c->p.ko_coeff != 0.0 && c->p.normalize_metric &&
-                       (c->p.stencil_order == 2 || c->p.stencil_order == 8);
+  const bool ko_norm = ko_norm_path(c);
   // one fused stage: sources and RHS with the current FRW values, then the
   // FRW stage step (bssn.cc:269), whose phi weights the averages of the output
   const bool flrw = lapse_needs_kavg(c->p.lapse);  // averages needed inside the step
@@ -901,10 +907,16 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
   // dust: the sources of stage n use W, detg, g_ij of stage n-1's input
   int prev_in = 0, prev_norm = 0;
   double prev_phiF = 0.0;
-  auto stage = [&](int n, int rin, int rout, int normalize, int in_norm = -1, int norm_out = -1) -> int {
-    StageArgs a = make_args(c, n, rin, 0, 1, rout, normalize);
+  // Accumulator-free form (no _f sweeps; the KO path with normalisation and
+  // the dust fluid keep the reference's accumulator)
+  const bool incr = fused_incr(c) && c->incr_pick;
+  auto stage = [&](int n, int rin, int rout, int normalize, int in_norm = -1, int norm_out = -1, int sf = 1,
+                   int sa2 = -1) -> int {
+    StageArgs a = make_args(c, n, rin, 0, sf, rout, normalize);
     a.s_in_norm = in_norm;
     a.s_norm_out = norm_out;
+    a.incr = incr ? 1 : 0;
+    a.s_a2 = sa2;
     if (c->dreg) {
       if (int rc = dust_flux(c, rin, normalize)) return rc;
       if (n > 1) {
@@ -927,9 +939,34 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
   // state (stepInit's _a) into the dead slot Y, so that the next step's
   // stage 1 runs the DMA kernel instead of normalising halos on the fly.
   bool have_norm = false;
+  int norm_slot = Y;
   for (int s = 0; s < nsteps; ++s) {
     int rc;
     c->sP = 0; c->sF = 1;
+    const bool pre = !ko_norm && !c->dreg && c->p.normalize_metric && s + 1 < nsteps && c->var &&
+                     c->dma_allowed && c->dma_pick && dma_fits(c, *c->var);
+    if (incr) {
+      // a1, a2, a3 rotate over slots 1..3 so that the normalised copy of the
+      // last stage 4 (written over a1) is the next stage 1's input
+      if (flrw && !c->ext_fresh && (rc = ext_from_state(c))) return rc;
+      int s1, s2, s3;
+      incr_slots(have_norm ? norm_slot : 0, s1, s2, s3);
+      if ((rc = have_norm ? stage(1, 0, s1, 0, norm_slot) : stage(1, 0, s1, c->p.normalize_metric))) return rc;
+      if ((rc = stage(2, s1, s2, 0))) return rc;
+      if ((rc = stage(3, s2, s3, 0))) return rc;
+      if ((rc = stage(4, s3, 0, 0, -1, pre ? s1 : -1, s1, s2))) return rc;
+      have_norm = pre;
+      norm_slot = s1;
+      c->sA = 0; c->sC = s3; c->a_is_p = true;
+      c->ext_fresh = flrw;
+      if (!flrw) {
+        c->ext_lazy = true;
+        c->ext_lazy_in = s3;
+        c->ext_lazy_src = src4;
+        c->ext_lazy_phiF = c->frw.g_phi;
+      }
+      continue;
+    }
     if (ko_norm) {
       // KO needs normalised A_ij at halo points: materialise stepInit first
       c->a_is_p = false; c->sA = X; c->sC = Y;
@@ -945,8 +982,6 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
     }
     const int in3 = ko_norm ? X : Y, out3 = ko_norm ? Y : X;
     if ((rc = stage(3, in3, out3, 0))) return rc;
-    const bool pre = !ko_norm && !c->dreg && c->p.normalize_metric && s + 1 < nsteps && c->var &&
-                     c->dma_allowed && c->dma_pick && dma_fits(c, *c->var);
     if ((rc = stage(4, out3, 0, 0, -1, pre ? Y : -1))) return rc;
     have_norm = pre;
     c->sA = 0; c->sC = out3; c->a_is_p = true;
@@ -1269,8 +1304,15 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
   if (!c || reps < 1) return CG_ERR_ARG;
   if (!c->a_is_p) return fail(c, CG_ERR_STATE, "autotune between method-level stages would clobber _a/_c");
   if (int rc = ext_resolve(c)) return rc;
-  // Times real stage-2-shaped launches (stencils of _p, writes into the dead
-  // _f and spare registers), so the evolved state is untouched.
+  // Times stage-1, stage-2 and stage-4 shaped launches per candidate, in
+  // each RK4 form the fused step may use, so the evolved state is untouched:
+  // outputs into the spare slot 3, the NaN count into scratch, and dead
+  // slots as the other inputs (stage 1: stencils of _p; stages 2 and 4:
+  // stencils of slot 2, as their real input is not _p, whose point values
+  // would otherwise come from cache; stage 4 reads its a2 from slot 3 just
+  // before writing it).  The candidate's step
+  // estimate is t1 + 2 t2 + t4 (stage 3 has stage 2's shape); the forms load
+  // their stages differently, so the best form depends on the grid.
   std::string rep = "[";
   const StageVariant* best = nullptr;
   double best_ms = 1e30;
@@ -1280,10 +1322,13 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
   const bool timing = c->timing;
   c->timing = false;
   int best_order = 0;
-  bool best_dma = false;
+  bool best_dma = false, best_incr = false;
   if (c->dreg) {  // dust: the fluxes of the state the timed launches read
     if (int rc = dust_flux(c, 0, 0)) { c->timing = timing; return rc; }
   }
+  unsigned* nan_scratch = reinterpret_cast<unsigned*>(out_buf(c, 1));
+  if (!nan_scratch) { c->timing = timing; return CG_ERR_HIP; }
+  const bool can_incr = fused_incr(c);
   for (const auto& v : stage_variants()) {
     if (!variant_matches(c, v)) continue;
     c->var = &v;
@@ -1293,32 +1338,52 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
       for (int order = 0; order < 4; ++order) {
         if (order == 3 && !morton_fits((c->p.nx + v.ti - 1) / v.ti, (c->p.ny + v.tj - 1) / v.tj)) continue;
         c->tile_order = order;
-        StageArgs a = make_args(c, 2, 0, 0, 1, 3, 0);
-        a.rho_out = nullptr;  // dust: DIFFr_a keeps the last real stage's source
-        int rc = launch(c, a);
-        if (rc) { c->timing = timing; return rc; }
-        HIPCHK(c, hipEventRecord(e0, c->stream));
-        for (int r = 0; r < reps; ++r)
-          if ((rc = launch(c, a))) { c->timing = timing; return rc; }
-        HIPCHK(c, hipEventRecord(e1, c->stream));
-        HIPCHK(c, hipEventSynchronize(e1));
-        float ms = 0;
-        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-        const double avg = ms / reps;
-        int vg = 0, sc = 0, ld = 0;
-        (d ? v.info_dma : v.info)(&vg, &sc, &ld);
-        char buf[256];
-        snprintf(buf, sizeof buf,
-                 "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"dma\":%d,\"order\":%d,\"ms\":%.4f,\"vgprs\":%d,"
-                 "\"scratch\":%d}",
-                 rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, d, order, avg, vg, sc);
-        rep += buf;
-        if (avg < best_ms) { best_ms = avg; best = &v; best_order = order; best_dma = d == 1; }
+        for (int form = can_incr ? 1 : 0; form >= 0; --form) {
+          double tq[3] = {0.0, 0.0, 0.0};
+          for (int q = 0; q < 3; ++q) {
+            const int st = q == 0 ? 1 : q == 1 ? 2 : 4;
+            StageArgs a = make_args(c, st, st == 1 ? 0 : 2, 0, 1, 3, 0);
+            if (st == 1 && c->p.normalize_metric) {
+              // steady state of the fused step: the normalised copy with
+              // DMA staging (slot 2 stands in), on the fly without
+              if (d == 1 && !c->dreg && !ko_norm_path(c)) a.s_in_norm = 2;
+              else a.normalize = 1;
+            }
+            a.rho_out = nullptr;  // dust: DIFFr_a keeps the last real stage's source
+            a.incr = form;
+            a.s_a2 = 3;
+            a.nan_count = nan_scratch;
+            int rc = launch(c, a);
+            if (rc) { c->timing = timing; return rc; }
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+            for (int r = 0; r < reps; ++r)
+              if ((rc = launch(c, a))) { c->timing = timing; return rc; }
+            HIPCHK(c, hipEventRecord(e1, c->stream));
+            HIPCHK(c, hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+            tq[q] = ms / reps;
+          }
+          const double avg = 0.25 * (tq[0] + 2.0 * tq[1] + tq[2]);  // mean stage of a step
+          int vg = 0, sc = 0, ld = 0;
+          (d ? v.info_dma : v.info)(&vg, &sc, &ld);
+          char buf[320];
+          snprintf(buf, sizeof buf,
+                   "%s{\"tile\":[%d,%d,%d],\"wpe\":%d,\"dma\":%d,\"order\":%d,\"incr\":%d,\"ms\":%.4f,"
+                   "\"ms1\":%.4f,\"ms2\":%.4f,\"ms4\":%.4f,\"vgprs\":%d,\"scratch\":%d}",
+                   rep.size() > 1 ? "," : "", v.ti, v.tj, v.tk, v.wpe, d, order, form, avg, tq[0], tq[1], tq[2], vg,
+                   sc);
+          rep += buf;
+          if (avg < best_ms) {
+            best_ms = avg; best = &v; best_order = order; best_dma = d == 1; best_incr = form == 1;
+          }
+        }
       }
     }
   }
   c->tile_order = best_order;
   c->dma_pick = best_dma || !best;
+  c->incr_pick = best_incr || !best;
   rep += "]";
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

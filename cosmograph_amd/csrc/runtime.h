@@ -107,6 +107,8 @@ struct cg_ctx {
   bool dma_allowed = true, dma_pick = true;
   double gd_c = 1.0, dw_mu_l = 0.0, dw_p = 0.0;  // gauge parameters (BSSNGaugeHandler.h:139-149)
   bool dust_bg = false;                   // dust sources as perturbations of the FRW fluids (cg_set_dust_background)
+  bool keep_accumulator = false;          // fused steps with the reference's _f accumulator (cg_set_fused_accumulator)
+  bool incr_pick = true;                  // the autotuner's choice between the two RK4 forms
   bool alpha_nonzero = false;             // a non-zero DIFFalpha was uploaded
   double* cons = nullptr;                 // [CONS_N][pad] per-point constraint values (lazy)
   double* cons_part = nullptr;            // reduction partials
@@ -174,4 +176,18 @@ bool morton_fits(int ntx, int nty);
 bool alpha_live(const cg_ctx* c);
 StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int normalize);
 int ext_resolve(cg_ctx* c);
+// KO at orders 2/8 with metric normalisation materialises stepInit's _a
+inline bool ko_norm_path(const cg_ctx* c) {
+  return c->p.ko_coeff != 0.0 && c->p.normalize_metric && (c->p.stencil_order == 2 || c->p.stencil_order == 8);
+}
+// fused steps may run in the accumulator-free form (StageArgs::incr)
+inline bool fused_incr(const cg_ctx* c) { return !ko_norm_path(c) && !c->dreg && !c->keep_accumulator; }
+// stage slots of an accumulator-free fused step whose stage 1 reads the
+// normalised copy in slot n (1..3), or n = 0 with none: a1, a2, a3 follow n
+// in the rotation 1 -> 2 -> 3 -> 1, and stage 4 writes the next copy over a1
+inline void incr_slots(int n, int& s1, int& s2, int& s3) {
+  s1 = n % 3 + 1;
+  s2 = s1 % 3 + 1;
+  s3 = s2 % 3 + 1;
+}
 }  // namespace cg

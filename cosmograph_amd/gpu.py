@@ -114,6 +114,7 @@ def load_library():
         "cg_kernel_info": ([vp, ip, ip, ip, ip, ip, ip], C.c_int),
         "cg_set_timing": ([vp, C.c_int], C.c_int),
         "cg_set_dma": ([vp, C.c_int], C.c_int),
+        "cg_set_fused_accumulator": ([vp, C.c_int], C.c_int),
         "cg_ics_static_dust": ([vp, dp, C.c_size_t, C.c_double, C.c_double, C.c_int, dp, C.POINTER(C.c_int)], C.c_int),
         "cg_set_tile_order": ([vp, C.c_int], C.c_int),
         "cg_set_dust_background": ([vp, C.c_int], C.c_int),
@@ -309,6 +310,11 @@ class BSSN:
     def set_dust_background(self, on: bool):
         """Dust fluid as perturbations of the FRW fluids (cg_set_dust_background)."""
         self._chk(self.L.cg_set_dust_background(self.h, int(on)))
+
+    def set_fused_accumulator(self, keep: bool):
+        """True: fused steps keep the reference's _f accumulator (17 register
+        sweeps per step); False (default): the accumulator-free form (13)."""
+        self._chk(self.L.cg_set_fused_accumulator(self.h, 1 if keep else 0))
 
     def set_tile_order(self, order: int):
         """Tile traversal order of the stage kernel (0-3; the autotuner sets it)."""
