@@ -1121,10 +1121,19 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         }
         const double p = (double)ldo((const TS*)RD(q, a.s_p), idx);
         TS* __restrict__ fr = RD(q, a.s_f);
-        const double fo = st >= 2 ? (double)ldo((const TS*)fr, idx) : 0.0;
+        double fo = 0.0;
+        if (!a.incr) {
+          if (st >= 2) fo = (double)ldo((const TS*)fr, idx);
+        } else if (st == 4) {
+          // accumulator-free form: K3Finalize's accumulator from the three stage outputs
+          const double a1 = (double)ldo((const TS*)fr, idx);
+          const double a2 = (double)ldo((const TS*)RD(q, a.s_a2), idx);
+          const double a3 = (double)ldo((const TS*)RD(q, a.s_in), idx);
+          fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
+        }
         const double inc = a.cf * cc;
         sto(o, idx, st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc));
-        if (st != 4) sto(fr, idx, (TS)(fo + inc));
+        if (st != 4 && !a.incr) sto(fr, idx, (TS)(fo + inc));
       }
     }
   }  // valid

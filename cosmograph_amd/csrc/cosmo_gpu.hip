@@ -907,8 +907,8 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
   // dust: the sources of stage n use W, detg, g_ij of stage n-1's input
   int prev_in = 0, prev_norm = 0;
   double prev_phiF = 0.0;
-  // Accumulator-free form (no _f sweeps; the KO path with normalisation and
-  // the dust fluid keep the reference's accumulator)
+  // Accumulator-free form (no _f sweeps; the KO path with normalisation
+  // keeps the reference's accumulator)
   const bool incr = fused_incr(c) && c->incr_pick;
   auto stage = [&](int n, int rin, int rout, int normalize, int in_norm = -1, int norm_out = -1, int sf = 1,
                    int sa2 = -1) -> int {

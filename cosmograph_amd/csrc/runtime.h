@@ -181,7 +181,7 @@ inline bool ko_norm_path(const cg_ctx* c) {
   return c->p.ko_coeff != 0.0 && c->p.normalize_metric && (c->p.stencil_order == 2 || c->p.stencil_order == 8);
 }
 // fused steps may run in the accumulator-free form (StageArgs::incr)
-inline bool fused_incr(const cg_ctx* c) { return !ko_norm_path(c) && !c->dreg && !c->keep_accumulator; }
+inline bool fused_incr(const cg_ctx* c) { return !ko_norm_path(c) && !c->keep_accumulator; }
 // stage slots of an accumulator-free fused step whose stage 1 reads the
 // normalised copy in slot n (1..3), or n = 0 with none: a1, a2, a3 follow n
 // in the rotation 1 -> 2 -> 3 -> 1, and stage 4 writes the next copy over a1

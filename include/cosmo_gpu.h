@@ -203,13 +203,16 @@ int cg_autotune(cg_ctx* ctx, int reps, char* report, size_t report_len);
 int cg_set_dma(cg_ctx* ctx, int on);
 
 /* Tuning knob, no counterpart in the reference.  keep = 0 (default): fused
- * steps (cg_rk4_steps) of static-matter and vacuum contexts run RK4 without
+ * steps (cg_rk4_steps, and cg_batch_step of the members) may run RK4 without
  * the _f accumulator: stages 1-3 write only the next stage input and stage 4
  * rebuilds K3Finalize's accumulator from the three stage outputs
- * ((a1-p) + 2(a2-p) + (a3-p))/3, 13 register sweeps per step instead of the
- * 17 of RK4Register.h:118-176.  The result differs from the reference's
- * accumulation by rounding (about one unit in the last place of _p per
- * step).  keep = 1: the reference's accumulator. */
+ * ((a1-p) + 2(a2-p) + (a3-p))/3, 14 register sweeps per step instead of the
+ * 17 of RK4Register.h:118-176 (the dust fluid's registers included).  The
+ * result differs from the reference's accumulation by rounding (about one
+ * unit in the last place of _p per step).  The autotuner times both forms
+ * and keeps the faster; without it the accumulator-free form runs.  KO at
+ * orders 2 and 8 with metric normalisation always uses the reference's form.
+ * keep = 1: the reference's accumulator. */
 int cg_set_fused_accumulator(cg_ctx* ctx, int keep);
 
 /* Traversal order of the stage kernel's tiles (the autotuner sets it):
