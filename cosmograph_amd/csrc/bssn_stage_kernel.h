@@ -109,14 +109,19 @@ __device__ __host__ constexpr int stencil_seq(int s) {
 // Staging order of the DMA path in split precision.  Position s is staged in
 // LDS buffer s % 3, and buffer 1 is FP32-sized, so every FP64 (metric) field
 // sits at a position s % 3 != 1:
-// PHI, KK, G11, G12, GAM1, G13, G22, GAM2, G23, G33, GAM3, [ALPHA], A11..A33.
-template <bool LA>
+// PHI, KK, G11, G12, GAM1, G13, G22, GAM2, G23, G33, GAM3, [ALPHA],
+// [12 dust fluxes], A11..A33 (everything after the head is FP32).
+template <bool LA, bool DU = false>
 __device__ __host__ constexpr int stencil_seq_split(int s) {
   constexpr int head[11] = {PHI, KK, G11, G12, GAM1, G13, G22, GAM2, G23, G33, GAM3};
   if (s < 11) return head[s];
   if (LA) {
     if (s == 11) return ALPHA;
     s -= 1;
+  }
+  if (DU) {
+    if (s < 11 + NFLUX) return FLUX0 + (s - 11);
+    s -= NFLUX;
   }
   return A11 + (s - 11 < 5 ? s - 11 : 5);
 }
@@ -509,7 +514,6 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
 
   if constexpr (DMA) {
     static_assert(std::is_same<TS, TC>::value, "DMA staging differentiates the stored type");
-    static_assert(!(SPLIT && FLUXES), "split-precision dust uses register staging");
     static_assert(NT % 64 == 0 && HK % 2 == 0 && HK4 % 4 == 0 && TK % 4 == 0,
                   "whole waves, whole 16-byte groups per row");
     constexpr int MPR = (CPR + NW - 1) / NW, MQD = (CQD + NW - 1) / NW;  // chunks per wave
@@ -565,7 +569,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         for (int m = 0; m < MQD; ++m) dma16(src, qoff[m] * 4u, dst + qchunk(m) * 1024);
       }
     };
-    auto seq = [&](int s) { return SPLIT ? stencil_seq_split<LIVE_ALPHA>(s) : stencil_seq<LIVE_ALPHA, FLUXES>(s); };
+    auto seq = [&](int s) {
+      return SPLIT ? stencil_seq_split<LIVE_ALPHA, FLUXES>(s) : stencil_seq<LIVE_ALPHA, FLUXES>(s);
+    };
     dma(seq(0), 0);
     if (nseq > 1) dma(seq(1), 1);
 #pragma unroll

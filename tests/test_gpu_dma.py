@@ -122,3 +122,46 @@ def test_tile_orders_give_identical_results(gpu, prec):
     for order in range(1, 4):
         for f in FIELDS:
             assert np.array_equal(out[order][f], out[0][f]), (order, f)
+
+
+DUST_CASES = [
+    # precision, order, shape, lambda
+    ("fp32_split", 6, (24, 20, 32), 0.02),
+    ("fp32_split", 4, (20, 24, 16), 0.0),
+    ("fp64", 6, (20, 24, 28), 0.02),
+]
+
+
+@pytest.mark.parametrize("prec,order,shape,lam", DUST_CASES)
+def test_dust_dma_matches_register_staging(gpu, prec, order, shape, lam):
+    """The dust-fluid stage kernel's DMA twin stages the 12 fluxes through
+    LDS with the other FP32 (or FP64) fields."""
+    _say(f"case dust dma {prec} order={order} shape={shape} lam={lam}")
+    names = FIELDS + ["dust_D", "dust_S1", "dust_S2", "dust_S3"]
+    out, info = {}, {}
+    x = [np.arange(n) / n for n in shape]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    for dma in (False, True):
+        g = gpu.BSSN(gpu.GpuParams(n=shape, order=order, precision=prec, matter="dust", lambda_=lam))
+        g.set_dma(dma)
+        st = smooth_random_state(shape, seed=order + 7, with_alpha=False)
+        for f in FIELDS:
+            g.upload(f + "_p", st[f] - (0.5 if f == "DIFFK" else 0.0))
+        g.upload("dust_D_p", 0.05 * (1.0 + 0.02 * np.cos(2 * np.pi * (X + Y))))
+        g.upload("dust_S1_p", 1e-3 * np.sin(2 * np.pi * (Z - 2 * X)))
+        g.upload("dust_S3_p", 1e-3 * np.cos(2 * np.pi * (Y + Z)))
+        g.step(2)
+        out[dma] = {f: g.download(f + "_p") for f in names}
+        info[dma] = g.kernel_info()
+        del g
+    assert info[True]["lds_bytes"] != info[False]["lds_bytes"], info  # the DMA twin ran
+    worst = 0.0
+    for f in names:
+        ref = out[False][f]
+        if not np.any(ref):
+            assert not np.any(out[True][f]), f
+            continue
+        err = np.sqrt(np.sum((out[True][f] - ref) ** 2)) / np.sqrt(np.sum(ref ** 2))
+        worst = max(worst, err)
+    _say(f"  worst rel_l2 {worst:.3e}")
+    assert worst <= (1e-15 if prec == "fp64" else 1e-7)
