@@ -482,12 +482,12 @@ int cg_batch_autotune(cg_batch* b, int reps, char* report, size_t report_len) {
             ba.nreal = n;
             ba.total = b->total;
             ba.per_xcd = (b->total + 7) / 8;
-            auto launch_q = go;  // with DMA, stage 1 reads the stand-in copy instead of normalising
-            int rc = launch_q(ba, b->stream);  // warm-up
+            // with DMA, stage 1 reads the stand-in copy instead of normalising
+            int rc = go(ba, b->stream);  // warm-up
             if (rc) return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
             BCHK(b, hipEventRecord(b->e0, b->stream));
             for (int k = 0; k < reps; ++k)
-              if ((rc = launch_q(ba, b->stream)))
+              if ((rc = go(ba, b->stream)))
                 return bfail(b, CG_ERR_HIP, std::string("batch launch: ") + hipGetErrorString((hipError_t)rc));
             BCHK(b, hipEventRecord(b->e1, b->stream));
             BCHK(b, hipEventSynchronize(b->e1));
