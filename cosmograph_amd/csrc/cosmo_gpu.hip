@@ -1316,18 +1316,28 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
   std::string rep = "[";
   const StageVariant* best = nullptr;
   double best_ms = 1e30;
-  hipEvent_t e0, e1;
-  HIPCHK(c, hipEventCreate(&e0));
-  HIPCHK(c, hipEventCreate(&e1));
-  const bool timing = c->timing;
+  // events and the per-launch timing switch are restored on every return
+  struct Guard {
+    cg_ctx* c;
+    bool timing;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Guard() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      c->timing = timing;
+    }
+  } guard{c, c->timing};
+  HIPCHK(c, hipEventCreate(&guard.e0));
+  HIPCHK(c, hipEventCreate(&guard.e1));
+  hipEvent_t e0 = guard.e0, e1 = guard.e1;
   c->timing = false;
   int best_order = 0;
   bool best_dma = false, best_incr = false;
   if (c->dreg) {  // dust: the fluxes of the state the timed launches read
-    if (int rc = dust_flux(c, 0, 0)) { c->timing = timing; return rc; }
+    if (int rc = dust_flux(c, 0, 0)) return rc;
   }
   unsigned* nan_scratch = reinterpret_cast<unsigned*>(out_buf(c, 1));
-  if (!nan_scratch) { c->timing = timing; return CG_ERR_HIP; }
+  if (!nan_scratch) return CG_ERR_HIP;
   const bool can_incr = fused_incr(c);
   for (const auto& v : stage_variants()) {
     if (!variant_matches(c, v)) continue;
@@ -1354,10 +1364,10 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
             a.s_a2 = 3;
             a.nan_count = nan_scratch;
             int rc = launch(c, a);
-            if (rc) { c->timing = timing; return rc; }
+            if (rc) return rc;
             HIPCHK(c, hipEventRecord(e0, c->stream));
             for (int r = 0; r < reps; ++r)
-              if ((rc = launch(c, a))) { c->timing = timing; return rc; }
+              if ((rc = launch(c, a))) return rc;
             HIPCHK(c, hipEventRecord(e1, c->stream));
             HIPCHK(c, hipEventSynchronize(e1));
             float ms = 0;
@@ -1385,9 +1395,6 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
   c->dma_pick = best_dma || !best;
   c->incr_pick = best_incr || !best;
   rep += "]";
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  c->timing = timing;
   c->var = best;
   if (report && report_len) {
     strncpy(report, rep.c_str(), report_len - 1);
