@@ -241,8 +241,9 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU (slab decomposition with "
-              "halo exchange) is a design only, see DESIGN.md.", file=sys.stderr)
+        print("bench.py: measures one GPU only (--gpus 1); sharding a box over several GPUs "
+              "(slab decomposition along i with a halo exchange of r planes per stage) is not built.",
+              file=sys.stderr)
         sys.exit(2)
 
     import cosmograph_amd as cg

@@ -2,7 +2,7 @@
 
 Every case creates the oracle and the GPU context from identical data and
 compares every field register.  Bars: FP64 state <= 1e-12 relative L2
-single right-hand sides <= 1e-10 relative L2 (a RHS
+(the drop-in's parity bar), single right-hand sides <= 1e-10 relative L2 (a RHS
 carries the cancellation of the stencils; the state it feeds does not);
 FP32 split <= 1e-5 relative L2 of the state.
 """
