@@ -113,7 +113,10 @@ def ensemble_leg(cg, steps, warmup, log):
     """Batched ensemble: 48 x 48^3 + 24 x 96^3 + 8 x 128^3 = 80 realisations
     (43,319,296 points per step), BSSN + static dust, order 4, FP32 split,
     P(k)~k^-2 with rms drawn uniformly in [1e-4, 1e-2], seeds 1..80; one
-    batched launch per RK4 stage, replayed from a HIP graph."""
+    batched launch per RK4 stage, replayed from a HIP graph.  Afterwards the
+    same ensemble runs in FP64 (the drop-in path, which the tests hold to the
+    oracle at 1e-12) for the per-realisation constraint norms: H and M L2 of
+    every realisation in both precisions and their largest relative gap."""
     rng = np.random.default_rng(80)
     params, spec = [], []
     seed = 0
@@ -139,7 +142,26 @@ def ensemble_leg(cg, steps, warmup, log):
            "realisations": len(spec), "points_per_step": pts, "steps": steps,
            "value": pts * steps / wall, "unit": "point-updates/s", "ms_per_step": 1e3 * wall / steps,
            "gpu_ms_per_step": gpu_ms / steps, "kernel": b.kernel_info(), "autotune": tune, "nan_points": nan}
+    c32 = b.setConstraintCalcs()
     del b
+    # the same realisations in FP64, same number of steps
+    log("[bench] ensemble: FP64 run for the per-realisation constraint norms")
+    b = cg.Batch([cg.GpuParams(n=n, order=4, precision="fp64", matter="static") for n, _, _ in spec])
+    for m, (n, rms, sd) in zip(b.members, spec):
+        m.ics_static_dust(gaussian_modes(n, -2.0, sd), rms)
+    b.step(warmup)
+    b.step(steps)
+    c64 = b.setConstraintCalcs()
+    del b
+    gaps = {}
+    for c in ("H", "M"):
+        a32, a64 = np.asarray(c32[c])[:, 7], np.asarray(c64[c])[:, 7]
+        gaps[c] = float(np.max(np.abs(a32 - a64) / np.abs(a64)))
+        out[f"{c}_L2_fp32"] = [float(f"{x:.6e}") for x in a32]
+        out[f"{c}_L2_fp64"] = [float(f"{x:.6e}") for x in a64]
+    out["constraint_L2_max_rel_gap_fp32_vs_fp64"] = gaps
+    log(f"[bench] ensemble: largest relative gap of a realisation's H / M L2, FP32 vs FP64: "
+        f"{gaps['H']:.2e} / {gaps['M']:.2e}")
     return out
 
 
