@@ -1066,7 +1066,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
           } else if (st == 4) {
             const double a1 = PF ? lpf(f, 1) : (double)ldo((const TS*)fr, idx);
             const double a2 = (double)ldo((const TS*)RS(f, a.s_a2), idx);
-            const double a3 = (double)ldo(INS(f), idx);
+            // the stored value: v holds it exactly when storage and compute types agree
+            const double a3 = std::is_same<TS, TC>::value ? (double)v[f] : (double)ldo(INS(f), idx);
             fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
           }
           const double inc = a.cf * cc;
