@@ -103,3 +103,27 @@ def test_field_stats_match_oracle(oracle_mod, gpu, prec):
         assert abs(got[0] - want[0]) <= tol * scale, (name, got, want)
         assert abs(got[1] - want[1]) <= tol * max(want[1], 1e-3 * scale), (name, got, want)
         assert abs(got[2] - want[2]) <= 1e-12 * want[2], (name, got, want)
+
+
+def test_output_readers_reject_bad_requests(gpu):
+    """Error paths of the readers: a strip must hold the grid's extent along
+    its axis, the axis is 1..3, names must exist, and the sources of a
+    static-matter context exist only after the constraint pass."""
+    import ctypes as C
+    shape = (16, 16, 20)
+    _say("case output reader errors")
+    g = gpu.BSSN(gpu.GpuParams(n=shape, order=4))
+    L = g.L
+    buf = np.zeros(64)
+    ptr = buf.ctypes.data_as(C.POINTER(C.c_double))
+    assert L.cg_field_strip(g.h, b"DIFFphi_p", 1, 0, 0, ptr, 15) == -1          # CG_ERR_ARG: NX is 16
+    assert L.cg_field_strip(g.h, b"DIFFphi_p", 4, 0, 0, ptr, 16) == -1          # no axis 4
+    assert L.cg_field_strip(g.h, b"nosuch_p", 1, 0, 0, ptr, 16) == -1
+    assert L.cg_field_strip(g.h, b"DIFFr_a", 1, 0, 0, ptr, 16) == -3            # CG_ERR_STATE before the pass
+    assert L.cg_constraint_strip(g.h, 3, 0, 0, ptr, ptr, 16) == -1              # NZ is 20
+    out3 = (C.c_double * 3)()
+    assert L.cg_field_stats(g.h, b"nosuch_a", out3) == -1
+    assert L.cg_field_stats(g.h, b"ricci_a", out3) == -3
+    g.setConstraintCalcs()
+    assert L.cg_field_strip(g.h, b"DIFFr_a", 3, 0, 0, ptr, 20) == 0
+    assert L.cg_field_stats(g.h, b"ricci_a", out3) == 0
