@@ -557,6 +557,12 @@ static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err, size
   return nullptr;
 }
 
+namespace cg {
+const void* field_ptr(cg_ctx* c, const char* name, size_t* esz, int* err) {
+  return resolve(c, name, false, err, esz);
+}
+}  // namespace cg
+
 int cg_field_upload(cg_ctx* c, const char* name, const double* host, size_t n) {
   if (!c || !host) return CG_ERR_ARG;
   if ((long)n != c->npts) return fail(c, CG_ERR_ARG, "size mismatch");

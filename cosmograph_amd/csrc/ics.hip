@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -310,3 +311,100 @@ int cg_ics_static_dust(cg_ctx* c, const double* modes, size_t n_modes, double rm
   if (iters) *iters = it;
   return CG_OK;
 }
+
+// ------------------------------------------------------ power spectrum --
+namespace {
+
+template <typename T>
+__global__ void k_to_double(const T* __restrict__ f, long n, double* __restrict__ out) {
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x)
+    out[x] = (double)f[x];
+}
+
+// Fourier::powerDump's angular binning (utils/Fourier.h:159-207) of the r2c
+// half spectrum [nx][ny][nz/2+1]: |F|^2 of each mode into bin
+// floor(|p|), counted twice for 0 < pz < nz/2 (the mode and its conjugate)
+// and once at pz = 0 and pz = nz/2.  One row of nbins sums and counts per
+// block (LDS atomics), summed on the host in block order.
+constexpr int kPowThreads = 256;
+__global__ void __launch_bounds__(kPowThreads) k_power_bins(const double2* __restrict__ spec, int nx, int ny, int nz,
+                                                           int nbins, double* __restrict__ part_f,
+                                                           unsigned* __restrict__ part_n) {
+  extern __shared__ double sh[];
+  double* f2 = sh;
+  unsigned* cnt = reinterpret_cast<unsigned*>(sh + nbins);
+  for (int q = threadIdx.x; q < nbins; q += kPowThreads) { f2[q] = 0.0; cnt[q] = 0u; }
+  __syncthreads();
+  const int nzc = nz / 2 + 1;
+  const long total = (long)nx * ny * nzc;
+  for (long x = (long)blockIdx.x * kPowThreads + threadIdx.x; x < total; x += (long)gridDim.x * kPowThreads) {
+    const int k = (int)(x % nzc), j = (int)((x / nzc) % ny), i = (int)(x / ((long)nzc * ny));
+    const int px = i <= nx / 2 ? i : i - nx, py = j <= ny / 2 ? j : j - ny, pz = k;
+    const int bin = (int)sqrt((double)(px * px + py * py + pz * pz));
+    const double2 v = spec[x];
+    const double fp2 = v.x * v.x + v.y * v.y;
+    const bool twice = k >= 1 && k < nz / 2;
+    if (bin < nbins) {
+      atomicAdd(&f2[bin], twice ? 2.0 * fp2 : fp2);
+      atomicAdd(&cnt[bin], twice ? 2u : 1u);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nbins; q += kPowThreads) {
+    part_f[(long)blockIdx.x * nbins + q] = f2[q];
+    part_n[(long)blockIdx.x * nbins + q] = cnt[q];
+  }
+}
+
+}  // namespace
+
+int cg_power_spectrum(cg_ctx* c, const char* name, double* out, int nbins) {
+  if (!c || !out) return CG_ERR_ARG;
+  const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+  const int want = (int)(sqrt((double)nx * nx + (double)ny * ny + (double)nz * nz) / 2.0) + 1;
+  if (nbins != want) return fail(c, CG_ERR_ARG, "nbins must be (int)(sqrt(NX^2+NY^2+NZ^2)/2) + 1 = " + std::to_string(want));
+  if (nz % 2) return fail(c, CG_ERR_ARG, "the power spectrum needs an even nz");
+  size_t esz;
+  int e;
+  const void* d = field_ptr(c, name, &esz, &e);
+  if (!d) return fail(c, e == 3 ? CG_ERR_STATE : CG_ERR_ARG,
+                      e == 3 ? std::string(name) + " is filled by cg_constraints" : std::string("unknown field ") + name);
+  const long n = c->npts, nspec = (long)nx * ny * (nz / 2 + 1);
+  constexpr int kBlocks = 512;
+  hipStream_t s = c->stream;
+  Scratch w;
+  HIPCHK(c, hipMalloc((void**)&w.work, n * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&w.spec, nspec * sizeof(double2)));
+  HIPCHK(c, hipMalloc((void**)&w.part, (size_t)kBlocks * nbins * (sizeof(double) + sizeof(unsigned))));
+  FFTCHK(c, hipfftPlan3d(&w.r2c, nx, ny, nz, HIPFFT_D2Z));
+  w.have_r2c = true;
+  FFTCHK(c, hipfftSetStream(w.r2c, s));
+  const int nb = (int)std::min<long>(4096, (n + 255) / 256);
+  if (esz == 8)
+    hipLaunchKernelGGL(k_to_double<double>, dim3(nb), dim3(256), 0, s, (const double*)d, n, w.work);
+  else
+    hipLaunchKernelGGL(k_to_double<float>, dim3(nb), dim3(256), 0, s, (const float*)d, n, w.work);
+  HIPCHK(c, hipGetLastError());
+  FFTCHK(c, hipfftExecD2Z(w.r2c, w.work, (hipfftDoubleComplex*)w.spec));
+  double* pf = w.part;
+  unsigned* pn = reinterpret_cast<unsigned*>(w.part + (size_t)kBlocks * nbins);
+  const size_t shmem = (size_t)nbins * (sizeof(double) + sizeof(unsigned));
+  hipLaunchKernelGGL(k_power_bins, dim3(kBlocks), dim3(kPowThreads), shmem, s, w.spec, nx, ny, nz, nbins, pf, pn);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> hf((size_t)kBlocks * nbins);
+  std::vector<unsigned> hn((size_t)kBlocks * nbins);
+  HIPCHK(c, hipMemcpyAsync(hf.data(), pf, hf.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(hn.data(), pn, hn.size() * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  for (int q = 0; q < nbins; ++q) {
+    double f = 0.0;
+    unsigned long long cnt = 0;
+    for (int b = 0; b < kBlocks; ++b) {
+      f += hf[(size_t)b * nbins + q];
+      cnt += hn[(size_t)b * nbins + q];
+    }
+    out[q] = cnt ? f / (double)cnt : 0.0;  // utils/Fourier.h:209-220
+  }
+  return CG_OK;
+}
+

@@ -176,6 +176,8 @@ bool morton_fits(int ntx, int nty);
 bool alpha_live(const cg_ctx* c);
 StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int normalize);
 int ext_resolve(cg_ctx* c);
+// device array of a field-map name for reading (err 1: unknown, 3: filled by cg_constraints)
+const void* field_ptr(cg_ctx* c, const char* name, size_t* esz, int* err);
 // KO at orders 2/8 with metric normalisation materialises stepInit's _a
 inline bool ko_norm_path(const cg_ctx* c) {
   return c->p.ko_coeff != 0.0 && c->p.normalize_metric && (c->p.stencil_order == 2 || c->p.stencil_order == 8);

@@ -123,6 +123,7 @@ def load_library():
         "cg_constraint_strip": ([vp, C.c_int, C.c_long, C.c_long, dp, dp, C.c_size_t], C.c_int),
         "cg_field_strip": ([vp, C.c_char_p, C.c_int, C.c_long, C.c_long, dp, C.c_size_t], C.c_int),
         "cg_field_stats": ([vp, C.c_char_p, dp], C.c_int),
+        "cg_power_spectrum": ([vp, C.c_char_p, dp, C.c_int], C.c_int),
         "cg_stage_time_ms": ([vp, dp, ip], C.c_int),
         "cg_extra_field_data": ([vp, dp], C.c_int),
         "cg_batch_create": ([C.POINTER(_Params), C.c_int, C.POINTER(vp)], C.c_int),
@@ -260,6 +261,14 @@ class BSSN:
         out = (C.c_double * 3)()
         self._chk(self.L.cg_field_stats(self.h, name.encode(), out))
         return tuple(out)
+
+    def power_spectrum(self, name: str) -> np.ndarray:
+        """Fourier::powerDump of a field-map array: |F_k|^2 averaged in bins of floor(|k|)."""
+        nx, ny, nz = self.params.shape()
+        nbins = int(np.sqrt(float(nx * nx + ny * ny + nz * nz)) / 2.0) + 1
+        out = np.zeros(nbins)
+        self._chk(self.L.cg_power_spectrum(self.h, name.encode(), _dptr(out), nbins))
+        return out
 
     def extra_field_data(self):
         """BSSN::setExtraFieldData results: K_min, K_avg, avg_vol, rho_avg."""

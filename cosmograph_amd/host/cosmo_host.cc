@@ -196,6 +196,13 @@ bool BSSN::strip(const std::string& name, int axis, long n1, long n2, std::vecto
 
 void BSSN::fieldStats(const std::string& name, double out[3]) { check(cg_field_stats(ctx, name.c_str(), out)); }
 
+std::vector<double> BSSN::powerSpectrum(const std::string& name) {
+  const int nbins = (int)(std::sqrt((double)p.nx * p.nx + (double)p.ny * p.ny + (double)p.nz * p.nz) / 2.0) + 1;
+  std::vector<double> out((size_t)nbins);
+  check(cg_power_spectrum(ctx, name.c_str(), out.data(), nbins));
+  return out;
+}
+
 void BSSN::setExtraFieldData() {
   double e[4];
   check(cg_extra_field_data(ctx, e));
@@ -355,13 +362,14 @@ void CosmoSim::outputStep() {
   const bool slices2 = due(cfg, "IO_2D_grid_interval", step);
   const bool slices3 = due(cfg, "IO_3D_grid_interval", step);
   const bool stats = due(cfg, "IO_bssnstats_interval", step);
+  const bool power = due(cfg, "IO_powerspec_interval", step);
   const bool cons = due(cfg, "IO_constraint_interval", step);
   const bool cons_snap = std::stol(cfg("IO_constraint_interval", "0")) > 0 &&
                          step % std::stol(cfg("IO_constraint_snapshot_interval", "999999999")) == 0;
   double H[8] = {0}, M[8] = {0}, G[7] = {0}, A[7] = {0}, S[7] = {0};
   // prepBSSNOutput (sims/sim.cc:214-231): ricci_a, AijAij_a and the sources
   // of this state come from the constraint pass
-  if (strips || slices2 || slices3 || stats || cons || cons_snap) bssnSim->setConstraintCalcs(H, M, G, A, S);
+  if (strips || slices2 || slices3 || stats || power || cons || cons_snap) bssnSim->setConstraintCalcs(H, M, G, A, S);
   const std::string step_str = std::to_string(step);
 
   // io_bssn_fields_snapshot (IO/io.cc:106-158): config keys IO_<d>D_<name>
@@ -406,6 +414,27 @@ void CosmoSim::outputStep() {
         continue;
       }
       dump_line(iodata, v, "1D_" + name + ".strip.dat.gz");
+    }
+  }
+
+  // io_bssn_fields_powerdump (IO/io.cc:165-178): one line per field in
+  // spec.dat.gz, "%g" values as Fourier::powerDump writes them
+  if (power) {
+    for (const char* name : {"DIFFphi_a", "DIFFr_a"}) {
+      const std::vector<double> v = bssnSim->powerSpectrum(name);
+      const std::string fn = iodata->dir() + "spec.dat.gz";
+      gzFile f = gzopen(fn.c_str(), "ab");
+      if (!f) {
+        iodata->log("Error opening file: " + fn);
+        break;
+      }
+      char data[20];
+      for (double x : v) {
+        std::snprintf(data, sizeof data, "%g\t", x);
+        gzwrite(f, data, (unsigned)std::strlen(data));
+      }
+      gzwrite(f, "\n", 1);
+      gzclose(f);
     }
   }
 

@@ -89,6 +89,8 @@ class BSSN {
   // conformal_average, conformal_standard_deviation of a field and the
   // volume_average of DIFFphi_a (IO/io.cc:8-14, utils/math.h:1382-1470)
   void fieldStats(const std::string& name, double out[3]);
+  // Fourier::powerDump's bin averages (utils/Fourier.h:147-220)
+  std::vector<double> powerSpectrum(const std::string& name);
   // refreshes K_min, K_avg, avg_vol, rho_avg (bssn.cc:165-170)
   void setExtraFieldData();
   double K_min = 0, K_avg = 0, avg_vol = 1, rho_avg = 0;

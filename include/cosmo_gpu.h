@@ -131,6 +131,15 @@ int cg_field_strip(cg_ctx* ctx, const char* name, int axis, long n1, long n2, do
  * sums, reduced in a fixed order (deterministic). */
 int cg_field_stats(cg_ctx* ctx, const char* name, double out[3]);
 
+/* Fourier::powerDump (utils/Fourier.h:147-240), written by
+ * io_bssn_fields_powerdump (IO/io.cc:165-178) for DIFFphi_a and DIFFr_a:
+ * the r2c transform of a field-map array (hipFFT, FP64), |F_k|^2 averaged in
+ * bins of floor(|k|) (k in lattice units, each interior kz mode counted for
+ * its conjugate too).  out[nbins], nbins = (int)(sqrt(NX^2+NY^2+NZ^2)/2) + 1;
+ * empty bins are 0.  The per-bin sums use LDS atomics, so their last bits
+ * may differ between runs.  Needs an even NZ. */
+int cg_power_spectrum(cg_ctx* ctx, const char* name, double* out, int nbins);
+
 /* BSSN::setExtraFieldData results (bssn.cc:165-170; the public members
  * K_min, K_avg, avg_vol, rho_avg of bssn.h:48): out = {K_min, K_avg,
  * avg_vol, rho_avg} as of the last stepInit / K{n}Finalize, i.e. after a

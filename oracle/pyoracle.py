@@ -199,6 +199,25 @@ class Sim:
         return tuple(self.L.or_reduction(self.h, k, f, ph, phiF) for k in (2, 6, 1))
 
 
+def power_dump(field: np.ndarray) -> np.ndarray:
+    """Fourier::powerDump (utils/Fourier.h:147-220): the r2c transform of the
+    field, |F|^2 binned by floor(|p|) with p in lattice units, each mode with
+    0 < pz < NZ/2 counted twice (it stands for its conjugate too) and pz = 0,
+    NZ/2 once; the bin averages.  numpy's rfftn stands in for FFTW's r2c."""
+    nx, ny, nz = field.shape
+    nbins = int(np.sqrt(float(nx * nx + ny * ny + nz * nz)) / 2.0) + 1
+    F = np.fft.rfftn(np.asarray(field, dtype=np.float64))
+    px = np.where(np.arange(nx) <= nx // 2, np.arange(nx), np.arange(nx) - nx)
+    py = np.where(np.arange(ny) <= ny // 2, np.arange(ny), np.arange(ny) - ny)
+    pz = np.arange(nz // 2 + 1)
+    PX, PY, PZ = np.meshgrid(px, py, pz, indexing="ij")
+    bins = np.sqrt((PX * PX + PY * PY + PZ * PZ).astype(np.float64)).astype(np.int64)
+    w = np.where((PZ >= 1) & (PZ < nz // 2), 2, 1)
+    f2 = np.bincount(bins.ravel(), weights=(w * np.abs(F) ** 2).ravel(), minlength=nbins)
+    cnt = np.bincount(bins.ravel(), weights=w.ravel(), minlength=nbins)
+    return np.where(cnt > 0, f2 / np.maximum(cnt, 1), 0.0)
+
+
 def point_compute(params: Params, pt: Point) -> Point:
     lib().or_point_compute(C.byref(params), C.byref(pt))
     return pt

@@ -85,6 +85,7 @@ IO_1D_DIFFr_a = 1
 IO_1D_A11_a = 0
 IO_3D_grid_interval = 2
 IO_3D_DIFFK_a = 1
+IO_powerspec_interval = 2
 """)
     r = run_driver(cfg)
     assert "RK steps: 3" in r.stdout                        # steps = 2 runs 3 RK4 steps
@@ -97,7 +98,7 @@ IO_3D_DIFFK_a = 1
     o.frw_set(frw["phi"], frw["K"])
     for rho, w in frw["fluids"]:
         o.frw_add_fluid(rho, w)
-    want_H, want_strips, want_stats, want_3d = [], [], [], {}
+    want_H, want_strips, want_stats, want_3d, want_spec = [], [], [], {}, []
     for s in range(steps + 1):
         want_H.append(oracle_output_constraints(oracle_mod, o, kind == "dust")["H"])
         want_strips.append({"DIFFphi_a": o.field("DIFFphi_a")[:, 0, 0].copy(),
@@ -110,6 +111,8 @@ IO_3D_DIFFK_a = 1
             want_1d_H, want_1d_M = o.constraint_strip(1, 0, 0)
         if s % 2 == 0:
             want_3d[s] = o.field("DIFFK_a").copy()
+            want_spec.append(oracle_mod.power_dump(o.field("DIFFphi_a")))
+            want_spec.append(oracle_mod.power_dump(o.field("DIFFr_a")))
         o.run_steps(1)
     names = (out / "final_fields.txt").read_text().split()
     assert len(names) == (22 if kind == "dust" else 18)
@@ -158,3 +161,8 @@ IO_3D_DIFFK_a = 1
     assert len(h1) == 1 and len(m1) == 1 and len(h1[0]) == n
     np.testing.assert_allclose(h1[0], want_1d_H, rtol=0, atol=1e-10 * np.abs(want_1d_H).max())
     np.testing.assert_allclose(m1[0], want_1d_M, rtol=0, atol=1e-10 * np.abs(want_1d_M).max())
+    # io_bssn_fields_powerdump: DIFFphi_a then DIFFr_a at every second step, "%g" text
+    spec = rows(out / "spec.dat.gz")
+    assert len(spec) == len(want_spec)
+    for got, w in zip(spec, want_spec):
+        np.testing.assert_allclose(got, w, rtol=2e-5, atol=1e-12 * max(w.max(), 1e-300))

@@ -127,3 +127,17 @@ def test_output_readers_reject_bad_requests(gpu):
     g.setConstraintCalcs()
     assert L.cg_field_strip(g.h, b"DIFFr_a", 3, 0, 0, ptr, 20) == 0
     assert L.cg_field_stats(g.h, b"ricci_a", out3) == 0
+
+
+@pytest.mark.parametrize("prec,shape", [("fp64", (16, 16, 20)), ("fp32_split", (20, 12, 16))])
+def test_power_spectrum_matches_oracle(oracle_mod, gpu, prec, shape):
+    _say(f"case power spectrum {prec} shape={shape}")
+    st = smooth_random_state(shape, seed=41, kmax=4)
+    g = gpu.BSSN(gpu.GpuParams(n=shape, order=4, precision=prec))
+    for f in FIELDS:
+        g.upload(f + "_p", st[f])
+    for name in ("DIFFphi_p", "DIFFK_p", "A13_p"):
+        want = oracle_mod.power_dump(g.download(name))
+        got = g.power_spectrum(name)
+        assert got.shape == want.shape
+        np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12 * want.max(), err_msg=name)

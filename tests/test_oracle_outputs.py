@@ -70,3 +70,34 @@ def test_constraint_strip_runs_along_i_on_every_axis(oracle_mod):
     np.testing.assert_array_equal(h2[:8], h1)
     np.testing.assert_array_equal(h2[8:], h1[:2])
     np.testing.assert_array_equal(h3, np.concatenate([h1, h1[:4]]))
+
+
+def test_power_dump_weights_satisfy_parseval(oracle_mod):
+    """The bins' counts times their averages sum to sum_k |F_k|^2 over the full
+    spectrum, which Parseval fixes at N^3 sum_x f^2: a wrong weight for the
+    pz = 0 / NZ/2 planes or for the conjugate modes breaks it."""
+    shape = (12, 10, 14)
+    f = np.random.default_rng(5).standard_normal(shape)
+    P = oracle_mod.power_dump(f)
+    nx, ny, nz = shape
+    px = np.where(np.arange(nx) <= nx // 2, np.arange(nx), np.arange(nx) - nx)
+    py = np.where(np.arange(ny) <= ny // 2, np.arange(ny), np.arange(ny) - ny)
+    PX, PY, PZ = np.meshgrid(px, py, np.arange(nz // 2 + 1), indexing="ij")
+    bins = np.sqrt((PX * PX + PY * PY + PZ * PZ).astype(float)).astype(int)
+    w = np.where((PZ >= 1) & (PZ < nz // 2), 2, 1)
+    cnt = np.bincount(bins.ravel(), weights=w.ravel(), minlength=P.size)
+    assert cnt.sum() == f.size                      # every mode of the full spectrum once
+    total = float(np.sum(cnt * P))
+    assert total == pytest.approx(f.size * float(np.sum(f * f)), rel=1e-12)
+
+
+def test_power_dump_of_a_single_mode(oracle_mod):
+    """cos(2 pi (3x + 4y)/N) has all its power in the bin |p| = 5."""
+    n = 16
+    x = np.arange(n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    f = np.cos(2 * np.pi * (3 * X + 4 * Y) / n)
+    P = oracle_mod.power_dump(f)
+    assert np.argmax(P) == 5
+    others = np.delete(P, 5)
+    assert np.max(others) <= 1e-20 * P[5]
