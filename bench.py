@@ -299,10 +299,21 @@ def main():
             nb = float(np.sqrt(np.sum(b * b)))
             d = float(np.sqrt(np.sum((a - b) ** 2)))
             errs[f] = d / nb if nb > 0 else d
+        # Hamiltonian / momentum constraint of the final states (L2 = H[7],
+        # L-infinity = H[2]); the FP64 run stands for the oracle (held to it at
+        # 1e-12 by the tests)
+        c32, c64 = g32.setConstraintCalcs(), g64.setConstraintCalcs()
+        cons = {f"{c}_{k}_{p}": float(cc[c][q]) for c in ("H", "M") for k, q in (("L2", 7), ("Linf", 2))
+                for p, cc in (("fp32", c32), ("fp64", c64))}
+        cons["H_L2_fp32_over_fp64"] = float(c32["H"][7] / c64["H"][7]) if c64["H"][7] else None
+        cons["H_Linf_fp32_over_fp64"] = float(c32["H"][2] / c64["H"][2]) if c64["H"][2] else None
+        log(f"[bench] H L2 / Linf: FP32 {c32['H'][7]:.6e} / {c32['H'][2]:.6e}, "
+            f"FP64 {c64['H'][7]:.6e} / {c64['H'][2]:.6e}")
         res = {"fp64_value": rate64, "fp64_ms_per_step": 1e3 * t64 / args.steps,
                "fp64_stage_kernel_ms": k64_ms, "fp32_over_fp64": rate32 / rate64,
                "max_rel_l2_fp32_vs_fp64": max(errs.values()),
                "rel_l2_fp32_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+               "constraints": cons,
                "fp64_kernel": g64.kernel_info(), "fp64_autotune": tune64}
         del g64
 
