@@ -446,7 +446,7 @@ int cg_batch_autotune(cg_batch* b, int reps, char* report, size_t report_len) {
     for (int d = 0; d < 2; ++d) {
       b->dma_pick = d == 1;
       if (d == 1 && !batch_dma_fits(b, v)) continue;
-      for (int order = 0; order < 4; ++order) {
+      for (int order = 0; order < 6; ++order) {
         if (order == 3) {
           bool ok = true;
           for (auto* c : b->m) ok = ok && morton_fits((c->p.nx + v.ti - 1) / v.ti, (c->p.ny + v.tj - 1) / v.tj);

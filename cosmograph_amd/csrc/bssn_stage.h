@@ -68,7 +68,7 @@ struct StageArgs {
   int nx, ny, nz;
   int ntx, nty, ntz;      // tile counts per axis
   int per_xcd;            // tiles handled per XCD (XCD-aware block remap)
-  int tile_order;         // traversal order of the tiles (0, 1, 2: j, k, i slowest; 3: Morton columns)
+  int tile_order;         // traversal order of the tiles (0, 1, 2: j, k, i slowest; 3: Morton columns; 4, 5: j strips)
   int stage;              // 0: RHS only into out; 1..4: fused K{n}Finalize
   int normalize;          // stage 1: det(gamma)=1 rescale, trace-free A on the fly
   int matter;

@@ -260,6 +260,15 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     tx = t % a.ntx; ty = (t / a.ntx) % a.nty; tz = t / (a.ntx * a.nty);
   } else if (a.tile_order == 2) {
     tz = t % a.ntz; ty = (t / a.ntz) % a.nty; tx = t / (a.ntz * a.nty);
+  } else if (a.tile_order >= 4) {
+    // strips of W = 2 (order 4) or 4 (order 5) tile rows along j: k fastest,
+    // then j inside the strip, then i, then the strips.  The j halos inside
+    // a strip come from tiles in flight; only the strip edges re-read theirs
+    const int W = a.tile_order == 4 ? 2 : 4;
+    tz = t % a.ntz;
+    const int r = t / a.ntz, s = r / (W * a.ntx), w = r - s * W * a.ntx;
+    const int wc = a.nty - s * W < W ? a.nty - s * W : W;
+    ty = s * W + w % wc; tx = w / wc;
   } else {
     tz = t % a.ntz;
     morton_xy(t / a.ntz, __builtin_ctz((unsigned)a.ntx), __builtin_ctz((unsigned)a.nty), tx, ty);

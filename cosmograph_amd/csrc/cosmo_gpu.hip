@@ -638,7 +638,7 @@ int cg_set_dust_background(cg_ctx* c, int on) {
 }
 
 int cg_set_tile_order(cg_ctx* c, int order) {
-  if (!c || order < 0 || order > 3) return CG_ERR_ARG;
+  if (!c || order < 0 || order > 5) return CG_ERR_ARG;
   c->tile_order = order;
   return CG_OK;
 }
@@ -1351,7 +1351,7 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
     for (int d = 0; d < 2; ++d) {
       if (d == 1 && !(c->dma_allowed && dma_fits(c, v))) continue;
       c->dma_pick = d == 1;
-      for (int order = 0; order < 4; ++order) {
+      for (int order = 0; order < 6; ++order) {
         if (order == 3 && !morton_fits((c->p.nx + v.ti - 1) / v.ti, (c->p.ny + v.tj - 1) / v.tj)) continue;
         c->tile_order = order;
         for (int form = can_incr ? 1 : 0; form >= 0; --form) {

@@ -326,7 +326,7 @@ class BSSN:
         self._chk(self.L.cg_set_fused_accumulator(self.h, 1 if keep else 0))
 
     def set_tile_order(self, order: int):
-        """Tile traversal order of the stage kernel (0-3; the autotuner sets it)."""
+        """Tile traversal order of the stage kernel (0-5; the autotuner sets it)."""
         self._chk(self.L.cg_set_tile_order(self.h, int(order)))
 
     def stage_time_ms(self):

@@ -227,7 +227,8 @@ int cg_set_fused_accumulator(cg_ctx* ctx, int keep);
 /* Traversal order of the stage kernel's tiles (the autotuner sets it):
  * 0, 1, 2 = the j, k, i axis slowest; 3 = k fastest, then the (i, j) tile
  * columns in Morton order (where the tile counts along i and j are powers
- * of two; order 0 otherwise).  Results do not depend on it.  No
+ * of two; order 0 otherwise); 4, 5 = k fastest, then strips of 2 or 4 tile
+ * rows along j, traversed along i.  Results do not depend on it.  No
  * counterpart in the reference. */
 int cg_set_tile_order(cg_ctx* ctx, int order);
 

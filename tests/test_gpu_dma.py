@@ -105,21 +105,23 @@ def test_batch_dma_matches_register_staging(gpu):
         _compare(out[True][r], out[False][r], "fp32_split")
 
 
-@pytest.mark.parametrize("prec", ["fp32_split", "fp64"])
-def test_tile_orders_give_identical_results(gpu, prec):
+@pytest.mark.parametrize("prec,shape", [("fp32_split", (32, 32, 48)), ("fp64", (32, 32, 48)),
+                                        ("fp32_split", (32, 40, 48))])
+def test_tile_orders_give_identical_results(gpu, prec, shape):
     """Every tile is computed by the same code whatever the traversal order
-    (0-2: one axis slowest; 3: Morton columns, here 8 x 4 tile columns)."""
-    shape = (32, 32, 48)
+    (0-2: one axis slowest; 3: Morton columns, here 8 x 4 tile columns; 4, 5:
+    strips of 2 or 4 tile rows along j, the last one partial when the 5 tile
+    rows of the second shape do not fill it)."""
     _say(f"case tile orders {prec} shape={shape}")
     out = []
-    for order in range(4):
+    for order in range(6):
         g = gpu.BSSN(gpu.GpuParams(n=shape, order=6, precision=prec, tile=(4, 8, 8)))
         g.set_tile_order(order)
         _load(g, shape, seed=5, lapse="Static")
         g.step(1)
         out.append({f: g.download(f + "_p") for f in FIELDS})
         del g
-    for order in range(1, 4):
+    for order in range(1, 6):
         for f in FIELDS:
             assert np.array_equal(out[order][f], out[0][f]), (order, f)
 
