@@ -60,7 +60,8 @@ typedef struct {
 } cg_params;
 
 /* BSSN::BSSN(ConfigParser*, Fourier*) (components/bssn/bssn.cc:13-44):
- * allocate all registers, zero-filled. */
+ * allocate all registers, zero-filled.  CG_ERR_ARG for a grid of 2^31 points
+ * or more (the kernels index points with 32-bit integers). */
 int cg_create(const cg_params* params, cg_ctx** out);
 /* BSSN::~BSSN (bssn.cc:46-54) */
 int cg_destroy(cg_ctx* ctx);
