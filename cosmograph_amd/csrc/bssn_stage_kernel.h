@@ -1186,14 +1186,29 @@ stage_kernel_batch(const BatchArgs ba) {
   const int b = blockIdx.x;
   const int g = (b & 7) * ba.per_xcd + (b >> 3);
   if (g >= ba.total) return;
+  // The tables are read through the constant address space: the kernel never
+  // writes them, so the reads are scalar loads, as for kernel arguments.  (As
+  // generic loads they would be vector loads made uniform, reissued after
+  // every asm wait of the DMA staging, which clobbers memory.)  The host
+  // compilation pass only checks this body and has no such address space.
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(4))) const int CInt;
+  typedef __attribute__((address_space(4))) const double CDouble;
+  typedef __attribute__((address_space(4))) const StageArgs CArgs;
+#else
+  typedef const int CInt;
+  typedef const double CDouble;
+  typedef const StageArgs CArgs;
+#endif
+  CInt* const ts = (CInt*)ba.tstart;
   int r = 0, n = ba.nreal;  // last grid whose first tile is <= g
   while (n > 1) {
     const int h = n >> 1;
-    if (ba.tstart[r + h] <= g) { r += h; n -= h; } else { n = h; }
+    if (ts[r + h] <= g) { r += h; n -= h; } else { n = h; }
   }
-  const double* f = ba.frw + (long)r * FRW_N;
-  stage_body<TS, TC, R, H, TI, TJ, TK, LIVE_ALPHA, 0, DMA>(ba.tab[r], g - ba.tstart[r],
-                                                      FrwV{f[0], f[1], f[2], f[3], f[4], f[5]});
+  CDouble* const f = (CDouble*)ba.frw + (long)r * FRW_N;
+  const StageArgs a = ((CArgs*)ba.tab)[r];
+  stage_body<TS, TC, R, H, TI, TJ, TK, LIVE_ALPHA, 0, DMA>(a, g - ts[r], FrwV{f[0], f[1], f[2], f[3], f[4], f[5]});
 }
 
 // ------------------------------------------------------------ variants
