@@ -106,6 +106,25 @@ def test_batch_matches_single_grid(gpu):
             assert rel_l2(b.members[r].download(f + "_p"), g.download(f + "_p")) <= 1e-14, (r, f)
 
 
+def test_batch_tile_orders_give_identical_results(gpu):
+    """The strip tile orders (4, 5) in the batched kernel, with partial strips
+    (2 and 3 tile rows along j), equal the default order bit for bit."""
+    shapes = [(16, 16, 16), (20, 16, 24), (24, 24, 24)]
+    _say(f"case batch tile orders shapes={shapes}")
+    out = {}
+    for order in (0, 4, 5):
+        b, _ = _make_batch(gpu, shapes, "fp32_split")
+        for m in b.members:
+            m.set_tile_order(order)
+        b.step(2)
+        out[order] = [{f: m.download(f + "_p") for f in FIELDS} for m in b.members]
+        del b
+    for order in (4, 5):
+        for r in range(len(shapes)):
+            for f in FIELDS:
+                assert np.array_equal(out[order][r][f], out[0][r][f]), (order, r, f)
+
+
 def test_batch_autotune_and_constraints(oracle_mod, gpu):
     _say("case batch autotune + constraints")
     b, states = _make_batch(gpu, SHAPES, "fp64")
