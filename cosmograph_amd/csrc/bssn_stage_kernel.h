@@ -1030,8 +1030,30 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     // one loop per RK4 form, each with only its own loads and stores
     auto epilogue = [&](auto incr_tag) {
       constexpr bool INC = decltype(incr_tag)::value;
+      // Stage 4 of the accumulator-free form reads a2 (and a3 of the FP64
+      // metric) of every field from memory.  The compiler cannot move a load
+      // above a store into the same register allocation, so loads issued
+      // field by field would each wait out a memory latency; they are issued
+      // AHEAD fields before their use instead (a few registers per field).
+      constexpr int AHEAD = 4;
+      double a2v[NF], a3m[7];
+      auto pre = [&](int g) {
+        if constexpr (INC) {
+          if (g < NF && st == 4 && !(g == ALPHA && !LIVE_ALPHA)) {
+            if (metric_field(g)) {
+              a2v[g] = ldo((const double*)RM(g, a.s_a2), idx);
+              a3m[g] = ldo(INM(g), idx);
+            } else {
+              a2v[g] = (double)ldo((const TS*)RS(g, a.s_a2), idx);
+            }
+          }
+        }
+      };
+  #pragma unroll
+      for (int g = 0; g < AHEAD; ++g) pre(g);
   #pragma unroll
       for (int f = 0; f < NF; ++f) {
+        pre(f + AHEAD);
         if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
         const double cc = (double)rhs[f];
         if (metric_field(f)) {
@@ -1047,9 +1069,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
             if (st >= 2) fo = PF ? lpf(f, 1) : ldo((const double*)fr, idx);
           } else if (st == 4) {
             const double a1 = PF ? lpf(f, 1) : ldo((const double*)fr, idx);
-            const double a2 = ldo((const double*)RM(f, a.s_a2), idx);
-            const double a3 = ldo(INM(f), idx);
-            fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
+            fo = ((a1 - p) + 2.0 * (a2v[f] - p) + (a3m[f] - p)) * (1.0 / 3.0);
           }
           const double inc = a.cf * cc;
           const double nv = st == 4 ? fo + (inc + p) : p + a.ca * cc;
@@ -1074,7 +1094,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
             if (st >= 2) fo = PF ? lpf(f, 1) : (double)ldo((const TS*)fr, idx);
           } else if (st == 4) {
             const double a1 = PF ? lpf(f, 1) : (double)ldo((const TS*)fr, idx);
-            const double a2 = (double)ldo((const TS*)RS(f, a.s_a2), idx);
+            const double a2 = a2v[f];
             // the stored value: v holds it exactly when storage and compute types agree
             const double a3 = std::is_same<TS, TC>::value ? (double)v[f] : (double)ldo(INS(f), idx);
             fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
