@@ -1030,16 +1030,24 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     // one loop per RK4 form, each with only its own loads and stores
     auto epilogue = [&](auto incr_tag) {
       constexpr bool INC = decltype(incr_tag)::value;
-      // Stage 4 of the accumulator-free form reads a2 (and a3 of the FP64
-      // metric) of every field from memory.  The compiler cannot move a load
-      // above a store into the same register allocation, so loads issued
-      // field by field would each wait out a memory latency; they are issued
-      // AHEAD fields before their use instead (a few registers per field).
-      constexpr int AHEAD = 4;
-      double a2v[NF], a3m[7];
+      // The epilogue's loads from memory (stage 4 of the accumulator-free
+      // form: a2 and a3 of the FP64 metric; without the LDS prefetch also p
+      // and f / a1).  The compiler cannot move a load above a store into the
+      // same register allocation, so loads issued field by field would each
+      // wait out a memory latency; they are issued AHEAD fields before their
+      // use instead (a few registers per field).
+      constexpr int AHEAD = PF ? 4 : 2;
+      double a2v[NF], a3m[7], pv[NF], qv[NF];
       auto pre = [&](int g) {
+        if (g >= NF || st == 0 || (g == ALPHA && !LIVE_ALPHA)) return;
+        if constexpr (!PF) {
+          const bool m = metric_field(g);
+          pv[g] = m ? ldo((const double*)RM(g, a.s_p), idx) : (double)ldo((const TS*)RS(g, a.s_p), idx);
+          if (INC ? st == 4 : st >= 2)
+            qv[g] = m ? ldo((const double*)RM(g, a.s_f), idx) : (double)ldo((const TS*)RS(g, a.s_f), idx);
+        }
         if constexpr (INC) {
-          if (g < NF && st == 4 && !(g == ALPHA && !LIVE_ALPHA)) {
+          if (st == 4) {
             if (metric_field(g)) {
               a2v[g] = ldo((const double*)RM(g, a.s_a2), idx);
               a3m[g] = ldo(INM(g), idx);
@@ -1062,13 +1070,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
             sto(o, idx, cc);
             continue;
           }
-          const double p = PF ? lpf(f, 0) : ldo((const double*)RM(f, a.s_p), idx);
+          const double p = PF ? lpf(f, 0) : pv[f];
           double* __restrict__ fr = RM(f, a.s_f);
           double fo = 0.0;
           if constexpr (!INC) {
-            if (st >= 2) fo = PF ? lpf(f, 1) : ldo((const double*)fr, idx);
+            if (st >= 2) fo = PF ? lpf(f, 1) : qv[f];
           } else if (st == 4) {
-            const double a1 = PF ? lpf(f, 1) : ldo((const double*)fr, idx);
+            const double a1 = PF ? lpf(f, 1) : qv[f];
             fo = ((a1 - p) + 2.0 * (a2v[f] - p) + (a3m[f] - p)) * (1.0 / 3.0);
           }
           const double inc = a.cf * cc;
@@ -1087,13 +1095,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
             sto(o, idx, (TS)cc);
             continue;
           }
-          const double p = PF ? lpf(f, 0) : (double)ldo((const TS*)RS(f, a.s_p), idx);
+          const double p = PF ? lpf(f, 0) : pv[f];
           TS* __restrict__ fr = RS(f, a.s_f);
           double fo = 0.0;
           if constexpr (!INC) {
-            if (st >= 2) fo = PF ? lpf(f, 1) : (double)ldo((const TS*)fr, idx);
+            if (st >= 2) fo = PF ? lpf(f, 1) : qv[f];
           } else if (st == 4) {
-            const double a1 = PF ? lpf(f, 1) : (double)ldo((const TS*)fr, idx);
+            const double a1 = PF ? lpf(f, 1) : qv[f];
             const double a2 = a2v[f];
             // the stored value: v holds it exactly when storage and compute types agree
             const double a3 = std::is_same<TS, TC>::value ? (double)v[f] : (double)ldo(INS(f), idx);
