@@ -67,26 +67,31 @@ def time_steps(g, warmup, steps, autotune=True):
     return t1 - t0, avg_ms, nlaunch, tune
 
 
-def cpu_baseline(n_sample, order, spectral_index, rms, seed):
-    """The CPU oracle (oracle/liboracle.so, OpenMP) on a bounded sample: one
-    RK4 step of the same physics on an n_sample^3 grid."""
+def cpu_baseline(n_sample, steps, order, spectral_index, rms, seed):
+    """The CPU oracle on a bounded sample: `steps` RK4 steps of the same
+    physics on an n_sample^3 grid.  Timed with the oracle's -O3 -ffast-math
+    build (oracle/liboracle_perf.so, the reference's optimisation flags,
+    OpenMP); the reference's own BSSN translation unit cannot be built
+    without FFTW, so this is the restatement ("port")."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from cosmograph_amd.ics import static_dust_ics
     fields, frw = static_dust_ics(n_sample, spectral_index, rms, seed)
-    s = pyoracle.Sim(pyoracle.make_params(n_sample, order=order))
+    s = pyoracle.Sim(pyoracle.make_params(n_sample, order=order), perf=True)
     for name, arr in fields.items():
         s.field(name)[:] = arr
+    del fields
     s.frw_set(frw["phi"], frw["K"])
     for rho, w in frw["fluids"]:
         s.frw_add_fluid(rho, w)
     t0 = time.perf_counter()
-    s.run_steps(1)
+    s.run_steps(steps)
     dt = time.perf_counter() - t0
-    threads = int(pyoracle.lib().or_num_threads())
-    return {"value": n_sample ** 3 / dt, "unit": "point-updates/s", "cores": threads, "kind": "port",
-            "sample": f"1 RK4 step of the oracle (C restatement, OpenMP, {threads} threads) on a "
-                      f"{n_sample}^3 grid, same physics/order; {dt:.1f} s"}
+    threads = int(s.L.or_num_threads())
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"value": n_sample ** 3 * steps / dt, "unit": "point-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} RK4 steps of the oracle (C restatement, -O3 -ffast-math, OpenMP {threads} threads; "
+                      f"{ncpu} host CPUs available) on a {n_sample}^3 grid, same physics/order; {dt:.1f} s"}
 
 
 def copy_probe(gib):
@@ -212,7 +217,7 @@ def c5_leg(cg, args, log):
     b_alg = (17 * 22) * 4   # FP32 words of the dust-fluid step (F = 22, G = 0)
     achieved = b_alg / 4.0 * n ** 3 / (runs["fp32_split"]["stage_kernel_ms"] * 1e-3) / 1e9
     r32, r64 = runs["fp32_split"], runs["fp64"]
-    print(json.dumps({
+    emit({
         "metric": f"BSSN + dust fluid + Lambda RK4 point-updates/s at {n}^3, FP32 split",
         "value": r32["value"], "unit": "point-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
         "ms_per_step": r32["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -223,7 +228,7 @@ def c5_leg(cg, args, log):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "b_alg_bytes_per_point_update": b_alg,
         "fp32": r32, "fp64": r64, "fp32_over_fp64": r32["value"] / r64["value"],
-        "rel_l2_fp32_vs_fp64": runs["rel_l2_fp32_vs_fp64"]}))
+        "rel_l2_fp32_vs_fp64": runs["rel_l2_fp32_vs_fp64"]}, {}, args.detail_file, log)
 
 
 def load_traffic(workload):
@@ -246,7 +251,9 @@ def main():
     ap.add_argument("--seed", type=int, default=9)
     ap.add_argument("--rms", type=float, default=None, help="rms of delta (default 1e-2 for c3, 3e-2 for c5)")
     ap.add_argument("--spectral-index", type=float, default=-3.0)
-    ap.add_argument("--cpu-sample", type=int, default=128, help="grid side of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=256, help="grid side of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="RK4 steps of the CPU-baseline sample")
+    ap.add_argument("--detail-file", default=None, help="also write the line and the detail tables to this file")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="batched-ensemble leg (0 = skip)")
@@ -331,8 +338,8 @@ def main():
 
     cpu = None
     if args.cpu_sample > 0:
-        log(f"[bench] CPU baseline sample {args.cpu_sample}^3")
-        cpu = cpu_baseline(args.cpu_sample, order, args.spectral_index, args.rms, args.seed)
+        log(f"[bench] CPU baseline sample {args.cpu_sample}^3, {args.cpu_steps} steps")
+        cpu = cpu_baseline(args.cpu_sample, args.cpu_steps, order, args.spectral_index, args.rms, args.seed)
 
     line = {
         "metric": "BSSN RK4 point-updates/s at 256^3, FP32-split vs FP64 GPU at <=1e-5 rel L2 err",
@@ -345,7 +352,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 storage+stencils, f64 RK4 accumulation/FRW",
+        "dtype": "fp32 split: f32 storage+stencils of A_ij, K, Gamma^i, alpha, D; FP64 metric "
+                 "(DIFFgamma_ij, DIFFphi), FP64 RK4 accumulation/FRW/reductions",
         "data": "synthetic: seeded Gaussian random field ICs, constraint-solved",
         "config": {"workload": f"BSSN + static dust, {n}^3 periodic, order {order}, "
                                f"P(k)~k^{args.spectral_index:g}, rms {args.rms:g}, dt=0.1dx, FP32 split",
@@ -356,12 +364,33 @@ def main():
         "b_alg_bytes_per_point_update": bytes_pt_upd,
         "stage_kernel_avg_ms": k32_ms,
         "stage_kernel_launches_timed": nl32,
-        "fp32_kernel": info32,
-        "fp32_autotune": tune32,
+        "kernel": info32,
     }
-    line.update(res)
-    line["ensemble"] = ens
-    print(json.dumps(line))
+    detail = {"fp32_autotune": tune32}
+    if res:
+        detail["fp64_autotune"] = res.pop("fp64_autotune")
+        line.update(res)
+    if ens is not None:
+        detail["ensemble"] = dict(ens)
+        line["ensemble"] = {k: ens[k] for k in ("workload", "realisations", "points_per_step", "steps", "value",
+                                                 "unit", "ms_per_step", "gpu_ms_per_step", "nan_points",
+                                                 "constraint_L2_max_rel_gap_fp32_vs_fp64")}
+    emit(line, detail, args.detail_file, log)
+
+
+def emit(line, detail, detail_file, log):
+    """The result line on stdout (compact: the autotune tables and the
+    per-realisation arrays go to stderr and, with --detail-file, to a file)."""
+    text = json.dumps(line)
+    if len(text) > 4096:
+        log(f"[bench] warning: result line is {len(text)} bytes")
+    log("[bench] detail " + json.dumps(detail))
+    if detail_file:
+        d = os.path.dirname(os.path.abspath(detail_file))
+        os.makedirs(d, exist_ok=True)
+        with open(detail_file, "w") as fh:
+            json.dump({"line": line, "detail": detail}, fh, indent=1)
+    print(text, flush=True)
 
 
 if __name__ == "__main__":

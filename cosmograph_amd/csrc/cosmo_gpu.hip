@@ -378,9 +378,6 @@ bool variant_fits(const cg_ctx* c, const StageVariant& v) {
 bool dma_fits(const cg_ctx* c, const StageVariant& v) {
   if (!v.launch_dma) return false;
   const int kh = (v.halo + 3) / 4 * 4;
-  // the halo DMA addresses each array with a 32-bit byte offset (FP64
-  // elements at most) from its base
-  if ((unsigned long long)c->npts * 8ull > 0xFFFFFFFFull) return false;
   return c->p.nz % 4 == 0 && c->p.nz >= kh && c->p.nz > v.tk + kh - 2;
 }
 
@@ -432,7 +429,11 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   *out = nullptr;
   const cg_params& p = *params;
   if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return CG_ERR_ARG;
-  if ((long long)p.nx * p.ny * p.nz >= (1ll << 31)) return CG_ERR_ARG;  // 32-bit point indices in the kernels
+  // The stage kernels address every array as a uniform base plus a 32-bit
+  // byte offset (element offset * element size, unsigned), and the FP64
+  // registers (the conformal metric in every precision) have 8-byte elements:
+  // the largest element offset times 8 must stay below 2^32.
+  if ((long long)p.nx * p.ny * p.nz > (1ll << 29)) return CG_ERR_ARG;
   if (p.stencil_order != 2 && p.stencil_order != 4 && p.stencil_order != 6 && p.stencil_order != 8) return CG_ERR_ARG;
   if (p.precision != CG_FP64 && p.precision != CG_FP32_SPLIT) return CG_ERR_ARG;
   if (p.matter != CG_MATTER_NONE && p.matter != CG_MATTER_STATIC && p.matter != CG_MATTER_DUST) return CG_ERR_ARG;

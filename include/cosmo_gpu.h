@@ -60,8 +60,9 @@ typedef struct {
 } cg_params;
 
 /* BSSN::BSSN(ConfigParser*, Fourier*) (components/bssn/bssn.cc:13-44):
- * allocate all registers, zero-filled.  CG_ERR_ARG for a grid of 2^31 points
- * or more (the kernels index points with 32-bit integers). */
+ * allocate all registers, zero-filled.  CG_ERR_ARG for a grid of more than
+ * 2^29 points (about 812^3): the kernels address each array with a 32-bit
+ * byte offset, and the FP64 registers have 8-byte elements. */
 int cg_create(const cg_params* params, cg_ctx** out);
 /* BSSN::~BSSN (bssn.cc:46-54) */
 int cg_destroy(cg_ctx* ctx);

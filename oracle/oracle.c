@@ -1102,6 +1102,34 @@ static void point_constraints(const or_point *bd, cstat out[5]) {
   for (int c = 0; c < 5; ++c) out[c].scaled = out[c].val / out[c].scale;
 }
 
+/* Test access to the per-point values setConstraintCalcs reduces
+ * (bssn.cc:965-1071): {val, scale, scaled} of H, M, G, A, S, 15 doubles.
+ * or_point_constraints: one point filled by or_point_compute (with dA);
+ * or_constraint_points: every grid point of _a, out[15 * idx + q]. */
+void or_point_constraints(const or_point *bd, double out[15]) {
+  cstat c[5];
+  point_constraints(bd, c);
+  for (int q = 0; q < 5; ++q) {
+    out[3 * q] = c[q].val;
+    out[3 * q + 1] = c[q].scale;
+    out[3 * q + 2] = c[q].scaled;
+  }
+}
+
+void or_constraint_points(or_sim *s, double *out) {
+  const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nx; ++i) {
+    or_point bd;
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        long idx = ((long)i * ny + j) * nz + k;
+        fill_point(s, i, j, k, &bd, 0, 1);
+        or_point_constraints(&bd, out + 15 * idx);
+      }
+  }
+}
+
 void or_bssn_constraints(or_sim *s, double H[8], double M[8], double G[7], double A[7], double S[7]) {
   const long n = s->npts;
   const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;

@@ -137,6 +137,10 @@ void or_rk4_demo(const double *x0, long n, double h, double c0, double c1, int n
 /* point algebra on caller-filled inputs (for brute-force identity tests) */
 void or_point_compute(const or_params *p, or_point *bd);
 void or_point_momentum(const or_point *bd, double M[3]);
+/* {val, scale, scaled} of H, M, G, A, S (setConstraintCalcs' per-point
+ * values, bssn.cc:965-1071) at one point / at every point of _a */
+void or_point_constraints(const or_point *bd, double out[15]);
+void or_constraint_points(or_sim *s, double *out);
 /* grid stencil probes (for bitwise comparison with utils/math.h) */
 double or_stencil(or_sim *s, const double *f, int kind, int d1, int d2, long i, long j, long k);
 

@@ -18,6 +18,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# the same source built with the reference's optimisation flags (-O3
+# -ffast-math, cmake/optimization.cmake:15-18): used only to time the CPU
+# baseline, never as the checker
+PERF_LIB_PATH = os.path.join(HERE, "liboracle_perf.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
 NF, NSRC = 18, 11
@@ -58,19 +62,22 @@ class Point(C.Structure):
     _fields_ = _point_fields()
 
 
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.check_call(["make", "-s", "-C", HERE, "all"])
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(perf: bool = False):
+    """The checker library, or with perf=True its -O3 -ffast-math build
+    (timing only)."""
+    key = "perf" if perf else "check"
+    if key not in _libs:
+        path = PERF_LIB_PATH if perf else LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         vp, dp = C.c_void_p, C.POINTER(C.c_double)
         L.or_create.restype = vp
         L.or_create.argtypes = [C.POINTER(Params)]
@@ -99,14 +106,16 @@ def lib():
         L.or_num_nans.restype = C.c_long
         L.or_point_compute.argtypes = [C.POINTER(Params), C.POINTER(Point)]
         L.or_point_momentum.argtypes = [C.POINTER(Point), dp]
+        L.or_point_constraints.argtypes = [C.POINTER(Point), dp]
+        L.or_constraint_points.argtypes = [vp, dp]
         L.or_stencil.argtypes = [vp, dp, C.c_int, C.c_int, C.c_int, C.c_long, C.c_long, C.c_long]
         L.or_stencil.restype = C.c_double
         L.or_reduction.argtypes = [vp, C.c_int, dp, dp, C.c_double]
         L.or_reduction.restype = C.c_double
         L.or_constraint_strip.argtypes = [vp, C.c_int, C.c_long, C.c_long, dp, dp]
         L.or_rk4_demo.argtypes = [dp, C.c_long, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]
-        _lib = L
-    return _lib
+        _libs[key] = L
+    return _libs[key]
 
 
 def make_params(n, order=4, dx=None, dt_frac=0.1, ko=0.0, a_adj=0.0, k_damp=0.0,
@@ -121,9 +130,9 @@ def make_params(n, order=4, dx=None, dt_frac=0.1, ko=0.0, a_adj=0.0, k_damp=0.0,
 class Sim:
     """Oracle simulation state; method names follow components/bssn/bssn.h."""
 
-    def __init__(self, params: Params):
+    def __init__(self, params: Params, perf: bool = False):
         self.p = params
-        self.L = lib()
+        self.L = lib(perf)
         self.h = self.L.or_create(C.byref(params))
         self.shape = (params.nx, params.ny, params.nz)
 
@@ -182,6 +191,13 @@ class Sim:
         self.L.or_bssn_constraints(self.h, H, M, G, A, S)
         return {"H": list(H), "M": list(M), "G": list(G), "A": list(A), "S": list(S)}
 
+    def constraint_points(self) -> np.ndarray:
+        """[nx, ny, nz, 5, 3]: {val, scale, scaled} of H, M, G, A, S at every
+        point of _a (what setConstraintCalcs reduces)."""
+        out = np.zeros(self.shape + (5, 3))
+        self.L.or_constraint_points(self.h, dptr(out))
+        return out
+
     def num_nans(self, name="DIFFphi_a"):
         return self.L.or_num_nans(self.h, name.encode())
 
@@ -227,6 +243,14 @@ def point_momentum(pt: Point):
     out = (C.c_double * 3)()
     lib().or_point_momentum(C.byref(pt), out)
     return np.array(out[:])
+
+
+def point_constraints(pt: Point) -> np.ndarray:
+    """[5, 3]: {val, scale, scaled} of H, M, G, A, S at a point filled by
+    point_compute."""
+    out = (C.c_double * 15)()
+    lib().or_point_constraints(C.byref(pt), out)
+    return np.array(out[:]).reshape(5, 3)
 
 
 # --------------------------------------------------------------- reference
