@@ -90,7 +90,24 @@ struct StageArgs {
   unsigned int* nan_count;
   double* cons;           // constraint mode: 17 per-point FP64 arrays (see CONS_*)
   long cons_stride;
+  // Constraint mode without per-point arrays: with cons_rows set, every
+  // block reduces its points into one row of cons_rows (row = tile -
+  // tile_base; pass 0: CONS_ROW_W sums and maxima, pass 1: the 10 squared
+  // deviations from cons_means).  tile_base/tile_count select a chunk of the
+  // grid's tiles (tile_count 0: all of them).
+  double* cons_rows;
+  const double* cons_means;
+  int cons_pass;
+  int tile_base, tile_count;
 };
+
+// row of the reduced constraint mode, per constraint q (H, M, G, A, S):
+// 6q + {sum val, sum scale, sum scaled, sum val^2, max|val|, max|scaled|};
+// pass 1: 2q + {sum (val - mean)^2, sum (scaled - mean scaled)^2}
+constexpr int CONS_ROW_W = 30;
+constexpr int CONS_ROW_W1 = 10;
+// the 5 constraints' value, scale, scaled of one point
+constexpr int CONS_PT = 15;
 
 // FRW values one stage sees (utils/FRW.h getters), the split-precision
 // background part of H, and the FRW phi after the stage's FRW step
@@ -157,6 +174,52 @@ __device__ __forceinline__ void block_ext_put(double* part, long row, bool on, d
     double acc = sh[0][q];
     for (int k = 1; k < nw; ++k) acc = q == EXT_KMIN ? (sh[k][q] < acc ? sh[k][q] : acc) : acc + sh[k][q];
     part[row * EXT_W + q] = acc;
+  }
+}
+
+// deterministic wavefront max (fixed butterfly order) of non-NaN values
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double o = __shfl_xor(v, m, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+// One row of the reduced constraint mode per workgroup (fixed order:
+// butterfly within each wave, waves in index order).  cv holds the point's
+// value, scale, scaled of H, M, G, A, S; `on` is false for threads outside
+// the grid.  Every thread of the block must call this: it holds barriers.
+__device__ __forceinline__ void block_cons_put(double* row, int pass, const double* means, bool on,
+                                               const double cv[CONS_PT]) {
+  __shared__ double sh[16][CONS_ROW_W];
+  const int wv = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+  const int w = pass == 0 ? CONS_ROW_W : CONS_ROW_W1;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const double v = on ? cv[3 * q] : 0.0, sc = on ? cv[3 * q + 1] : 0.0, sd = on ? cv[3 * q + 2] : 0.0;
+    if (pass == 0) {
+      const double r0 = wave_sum(v), r1 = wave_sum(sc), r2 = wave_sum(sd), r3 = wave_sum(v * v);
+      // the reference's `if (fabs(C) > max)` from 0: NaN never wins
+      const double r4 = wave_max(on && fabs(v) > 0.0 ? fabs(v) : 0.0);
+      const double r5 = wave_max(on && fabs(sd) > 0.0 ? fabs(sd) : 0.0);
+      if ((threadIdx.x & 63) == 0) {
+        sh[wv][6 * q] = r0; sh[wv][6 * q + 1] = r1; sh[wv][6 * q + 2] = r2;
+        sh[wv][6 * q + 3] = r3; sh[wv][6 * q + 4] = r4; sh[wv][6 * q + 5] = r5;
+      }
+    } else {
+      const double d0 = on ? v - means[2 * q] : 0.0, d1 = on ? sd - means[2 * q + 1] : 0.0;
+      const double r0 = wave_sum(d0 * d0), r1 = wave_sum(d1 * d1);
+      if ((threadIdx.x & 63) == 0) { sh[wv][2 * q] = r0; sh[wv][2 * q + 1] = r1; }
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < w) {
+    const int col = threadIdx.x;
+    const bool ismax = pass == 0 && (col % 6) >= 4;
+    double acc = sh[0][col];
+    for (int k = 1; k < nw; ++k) acc = ismax ? (sh[k][col] > acc ? sh[k][col] : acc) : acc + sh[k][col];
+    row[col] = acc;
   }
 }
 

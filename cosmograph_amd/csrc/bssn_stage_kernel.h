@@ -662,6 +662,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   // Threads outside the grid (ragged edge tiles) skip the algebra but stay
   // alive for the block-wide setExtraFieldData reduction at the end.
   double phi_new = 0.0, K_new = 0.0, rho_new = 0.0;  // stage output and source seen by the reduction
+  double cv[CONS_PT];  // constraint mode: value, scale, scaled of H, M, G, A, S at this point
+#pragma unroll
+  for (int q = 0; q < CONS_PT; ++q) cv[q] = 0.0;
   if (valid) {
     // ---------------------------------------------------------- algebra
     const TC phiF = (TC)fv.phi, KF = (TC)fv.K, rhoF = (TC)fv.rho, SF = (TC)fv.S;
@@ -882,7 +885,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       // setConstraintCalcs (bssn.cc:965-1071) per-point quantities, FP64
       double* __restrict__ co = a.cons;
       const long cs = a.cons_stride;
-      auto put = [&](int q, double val) { co[(long)q * cs + idx] = val; };
+      auto put = [&](int q, double val) {
+        if (q < CONS_PT) cv[q] = val;
+        if (!a.cons_rows) co[(long)q * cs + idx] = val;
+      };
       const double rr = (double)DIFFr + fv.rho;
       const double e5 = exp(5.0 * (double)phi), e6 = exp(6.0 * (double)phi);
       // hamiltonianConstraintScale (bssn.cc:1086-1100)
@@ -951,236 +957,241 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       put(CONS_AIJAIJ, AijAij);
       put(CONS_RHO, DIFFr);   // the sources the output step sees (DIFFr_a, DIFFS_a)
       put(CONS_SRCS, DIFFS);
-      return;
-    }
+    } else {
 
-    // ------------------------------------------------ right-hand sides
-    TC rhs[NF];
-    const TC dt = (TC)a.dt;
-    const TC adj = (TC)a.a_adj;
-  #pragma unroll
-    for (int s = 0; s < 6; ++s)
-      rhs[G11 + s] = TC(-2) * alpha * v[A11 + s] + TC(0.5) * adj * dt * Hc * v[G11 + s] - ko[G11 + s];
-    // bssn_macros.h:309-338
-  #pragma unroll
-    for (int I = 0; I < 3; ++I)
-  #pragma unroll
-      for (int J = I; J < 3; ++J) {
-        const int s = sy(I, J);
-        TC AA = TC(0);
-  #pragma unroll
-        for (int m = 0; m < 3; ++m)
-          AA += AM(m, I) * (GI(m, 0) * AM(0, J) + GI(m, 1) * AM(1, J) + GI(m, 2) * AM(2, J));
-        const TC ricTF = ric[s] - TC(1.0 / 3.0) * e4phi * GM(I, J) * ricci;
-        rhs[A11 + s] = em4phi * (alpha * (ricTF - TC(8) * PI * STF[s]) - DDa[s]) +
-                       alpha * (K * v[A11 + s] - TC(2) * AA) - adj * dt * v[A11 + s] * Hc - ko[A11 + s];
-      }
-    // bssn_macros.h:340-361
-  #pragma unroll
-    for (int I = 0; I < 3; ++I) {
-      const TC second = G[I][0] * Ac[0] + G[I][3] * Ac[3] + G[I][5] * Ac[5] +
-                        TC(2) * (G[I][1] * Ac[1] + G[I][2] * Ac[2] + G[I][4] * Ac[4]) +
-                        TC(6) * (Ac[sy(I, 0)] * dphi[0] + Ac[sy(I, 1)] * dphi[1] + Ac[sy(I, 2)] * dphi[2]);
-      rhs[GAM1 + I] = TC(-2) * (Ac[sy(I, 0)] * dal[0] + Ac[sy(I, 1)] * dal[1] + Ac[sy(I, 2)] * dal[2]) +
-                      TC(2) * alpha *
-                          (second - TC(2.0 / 3.0) * (GI(I, 0) * dK[0] + GI(I, 1) * dK[1] + GI(I, 2) * dK[2]) -
-                           TC(8) * PI * (GI(I, 0) * Si[0] + GI(I, 1) * Si[1] + GI(I, 2) * Si[2])) -
-                      ko[GAM1 + I];
-    }
-    // bssn.cc:766-795
-    rhs[KK] = -DDaTR + alpha * (TC(1.0 / 3.0) * v[KK] * (v[KK] + TC(2) * KF) + AijAij) +
-              TC(4) * PI * alpha * (DIFFr + DIFFS) + TC(4) * PI * v[ALPHA] * (rhoF + SF) -
-              (TC)a.k_damp * Hc * exp(TC(-5) * phi) - ko[KK];
-    // bssn.cc:797-820
-    rhs[PHI] = TC(0.1) * adj * dt * Hc - TC(1.0 / 6.0) * (alpha * v[KK] + v[ALPHA] * KF) - ko[PHI];
-    // BSSNGaugeHandler.cc (the shift-free gauges; beta = 0)
-    TC lapse = TC(0);
-    switch (a.lapse) {
-      case LAPSE_HARMONIC: lapse = -alpha * K; break;                                          // :32-35
-      case LAPSE_CONFORMAL_FLRW: lapse = TC(-1.0 / 3.0) * alpha * (TC)a.ext_in[EXTV_KAVG]; break;  // :24-27
-      case LAPSE_ANHARMONIC: lapse = alpha * alpha * (K - (TC)a.ext_in[EXTV_KAVG]); break;     // :50-58
-      case LAPSE_ONE_PLUS_LOG:                                                                 // :64-68
-        lapse = TC(-2) * alpha * (K - (TC)a.ext_in[EXTV_KAVG]) * (TC)a.gd_c;
-        break;
-      case LAPSE_CONFORMAL_SYNC: lapse = TC(-1.0 / 3.0) * alpha * KF; break;                   // :84-87
-      case LAPSE_AWA_GAUGE_WAVE: lapse = -alpha * alpha * v[KK]; break;                        // :209-212
-      case LAPSE_AWA_SHIFTED_WAVE: lapse = -v[KK]; break;                                      // :217-220
-      case LAPSE_DAMPED_WAVE:                                                                  // :151-155
-        lapse = alpha * alpha * ((TC)a.dw_mu_l * (TC(12) * phi * (TC)a.dw_p - log(alpha)) - K);
-        break;
-      default: break;  // Static, TestAijDriverLapse
-    }
-    rhs[ALPHA] = lapse - ko[ALPHA];
-
-    // ------------------------------------------------ RK4Register epilogue
-    // K{n}Finalize (RK4Register.h:129-176): f += cf*c ; a' = p + ca*c, and at
-    // stage 4 p' = f + (cf*c + p).  Accumulation in FP64 for every precision.
-    const int st = a.stage;
-    rho_new = (double)DIFFr;
-    double gn[6], An[6];  // stage 4: the new DIFFgamma_ij and A_ij as stored
-    // DMA kernels: _p and _f of this point from the wave's LDS prefetch
-    const char* const pfr = ldma + (tid >> 6) * PFR;
-    const int pq = tid & 63;
-    auto lpf = [&](int f, int kind) -> double {
-      if (!SPLIT || f < 7) return *reinterpret_cast<const double*>(pfr + f * 1024 + kind * 512 + pq * 8);
-      const int n = f - 7;
-      return (double)*reinterpret_cast<const TS*>(pfr + (N64 + (n >> 1)) * 1024 + (n & 1) * 512 + kind * 256 + pq * 4);
-    };
-    if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    // one loop per RK4 form, each with only its own loads and stores
-    auto epilogue = [&](auto incr_tag) {
-      constexpr bool INC = decltype(incr_tag)::value;
-      // The epilogue's loads from memory (stage 4 of the accumulator-free
-      // form: a2 and a3 of the FP64 metric; without the LDS prefetch also p
-      // and f / a1).  The compiler cannot move a load above a store into the
-      // same register allocation, so loads issued field by field would each
-      // wait out a memory latency; they are issued AHEAD fields before their
-      // use instead (a few registers per field).
-      constexpr int AHEAD = PF ? 4 : 2;
-      double a2v[NF], a3m[7], pv[NF], qv[NF];
-      auto pre = [&](int g) {
-        if (g >= NF || st == 0 || (g == ALPHA && !LIVE_ALPHA)) return;
-        if constexpr (!PF) {
-          const bool m = metric_field(g);
-          pv[g] = m ? ldo((const double*)RM(g, a.s_p), idx) : (double)ldo((const TS*)RS(g, a.s_p), idx);
-          if (INC ? st == 4 : st >= 2)
-            qv[g] = m ? ldo((const double*)RM(g, a.s_f), idx) : (double)ldo((const TS*)RS(g, a.s_f), idx);
+      // ------------------------------------------------ right-hand sides
+      TC rhs[NF];
+      const TC dt = (TC)a.dt;
+      const TC adj = (TC)a.a_adj;
+    #pragma unroll
+      for (int s = 0; s < 6; ++s)
+        rhs[G11 + s] = TC(-2) * alpha * v[A11 + s] + TC(0.5) * adj * dt * Hc * v[G11 + s] - ko[G11 + s];
+      // bssn_macros.h:309-338
+    #pragma unroll
+      for (int I = 0; I < 3; ++I)
+    #pragma unroll
+        for (int J = I; J < 3; ++J) {
+          const int s = sy(I, J);
+          TC AA = TC(0);
+    #pragma unroll
+          for (int m = 0; m < 3; ++m)
+            AA += AM(m, I) * (GI(m, 0) * AM(0, J) + GI(m, 1) * AM(1, J) + GI(m, 2) * AM(2, J));
+          const TC ricTF = ric[s] - TC(1.0 / 3.0) * e4phi * GM(I, J) * ricci;
+          rhs[A11 + s] = em4phi * (alpha * (ricTF - TC(8) * PI * STF[s]) - DDa[s]) +
+                         alpha * (K * v[A11 + s] - TC(2) * AA) - adj * dt * v[A11 + s] * Hc - ko[A11 + s];
         }
-        if constexpr (INC) {
-          if (st == 4) {
-            if (metric_field(g)) {
-              a2v[g] = ldo((const double*)RM(g, a.s_a2), idx);
-              a3m[g] = ldo(INM(g), idx);
-            } else {
-              a2v[g] = (double)ldo((const TS*)RS(g, a.s_a2), idx);
+      // bssn_macros.h:340-361
+    #pragma unroll
+      for (int I = 0; I < 3; ++I) {
+        const TC second = G[I][0] * Ac[0] + G[I][3] * Ac[3] + G[I][5] * Ac[5] +
+                          TC(2) * (G[I][1] * Ac[1] + G[I][2] * Ac[2] + G[I][4] * Ac[4]) +
+                          TC(6) * (Ac[sy(I, 0)] * dphi[0] + Ac[sy(I, 1)] * dphi[1] + Ac[sy(I, 2)] * dphi[2]);
+        rhs[GAM1 + I] = TC(-2) * (Ac[sy(I, 0)] * dal[0] + Ac[sy(I, 1)] * dal[1] + Ac[sy(I, 2)] * dal[2]) +
+                        TC(2) * alpha *
+                            (second - TC(2.0 / 3.0) * (GI(I, 0) * dK[0] + GI(I, 1) * dK[1] + GI(I, 2) * dK[2]) -
+                             TC(8) * PI * (GI(I, 0) * Si[0] + GI(I, 1) * Si[1] + GI(I, 2) * Si[2])) -
+                        ko[GAM1 + I];
+      }
+      // bssn.cc:766-795
+      rhs[KK] = -DDaTR + alpha * (TC(1.0 / 3.0) * v[KK] * (v[KK] + TC(2) * KF) + AijAij) +
+                TC(4) * PI * alpha * (DIFFr + DIFFS) + TC(4) * PI * v[ALPHA] * (rhoF + SF) -
+                (TC)a.k_damp * Hc * exp(TC(-5) * phi) - ko[KK];
+      // bssn.cc:797-820
+      rhs[PHI] = TC(0.1) * adj * dt * Hc - TC(1.0 / 6.0) * (alpha * v[KK] + v[ALPHA] * KF) - ko[PHI];
+      // BSSNGaugeHandler.cc (the shift-free gauges; beta = 0)
+      TC lapse = TC(0);
+      switch (a.lapse) {
+        case LAPSE_HARMONIC: lapse = -alpha * K; break;                                          // :32-35
+        case LAPSE_CONFORMAL_FLRW: lapse = TC(-1.0 / 3.0) * alpha * (TC)a.ext_in[EXTV_KAVG]; break;  // :24-27
+        case LAPSE_ANHARMONIC: lapse = alpha * alpha * (K - (TC)a.ext_in[EXTV_KAVG]); break;     // :50-58
+        case LAPSE_ONE_PLUS_LOG:                                                                 // :64-68
+          lapse = TC(-2) * alpha * (K - (TC)a.ext_in[EXTV_KAVG]) * (TC)a.gd_c;
+          break;
+        case LAPSE_CONFORMAL_SYNC: lapse = TC(-1.0 / 3.0) * alpha * KF; break;                   // :84-87
+        case LAPSE_AWA_GAUGE_WAVE: lapse = -alpha * alpha * v[KK]; break;                        // :209-212
+        case LAPSE_AWA_SHIFTED_WAVE: lapse = -v[KK]; break;                                      // :217-220
+        case LAPSE_DAMPED_WAVE:                                                                  // :151-155
+          lapse = alpha * alpha * ((TC)a.dw_mu_l * (TC(12) * phi * (TC)a.dw_p - log(alpha)) - K);
+          break;
+        default: break;  // Static, TestAijDriverLapse
+      }
+      rhs[ALPHA] = lapse - ko[ALPHA];
+
+      // ------------------------------------------------ RK4Register epilogue
+      // K{n}Finalize (RK4Register.h:129-176): f += cf*c ; a' = p + ca*c, and at
+      // stage 4 p' = f + (cf*c + p).  Accumulation in FP64 for every precision.
+      const int st = a.stage;
+      rho_new = (double)DIFFr;
+      double gn[6], An[6];  // stage 4: the new DIFFgamma_ij and A_ij as stored
+      // DMA kernels: _p and _f of this point from the wave's LDS prefetch
+      const char* const pfr = ldma + (tid >> 6) * PFR;
+      const int pq = tid & 63;
+      auto lpf = [&](int f, int kind) -> double {
+        if (!SPLIT || f < 7) return *reinterpret_cast<const double*>(pfr + f * 1024 + kind * 512 + pq * 8);
+        const int n = f - 7;
+        return (double)*reinterpret_cast<const TS*>(pfr + (N64 + (n >> 1)) * 1024 + (n & 1) * 512 + kind * 256 + pq * 4);
+      };
+      if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      // one loop per RK4 form, each with only its own loads and stores
+      auto epilogue = [&](auto incr_tag) {
+        constexpr bool INC = decltype(incr_tag)::value;
+        // The epilogue's loads from memory (stage 4 of the accumulator-free
+        // form: a2 and a3 of the FP64 metric; without the LDS prefetch also p
+        // and f / a1).  The compiler cannot move a load above a store into the
+        // same register allocation, so loads issued field by field would each
+        // wait out a memory latency; they are issued AHEAD fields before their
+        // use instead (a few registers per field).
+        constexpr int AHEAD = PF ? 4 : 2;
+        double a2v[NF], a3m[7], pv[NF], qv[NF];
+        auto pre = [&](int g) {
+          if (g >= NF || st == 0 || (g == ALPHA && !LIVE_ALPHA)) return;
+          if constexpr (!PF) {
+            const bool m = metric_field(g);
+            pv[g] = m ? ldo((const double*)RM(g, a.s_p), idx) : (double)ldo((const TS*)RS(g, a.s_p), idx);
+            if (INC ? st == 4 : st >= 2)
+              qv[g] = m ? ldo((const double*)RM(g, a.s_f), idx) : (double)ldo((const TS*)RS(g, a.s_f), idx);
+          }
+          if constexpr (INC) {
+            if (st == 4) {
+              if (metric_field(g)) {
+                a2v[g] = ldo((const double*)RM(g, a.s_a2), idx);
+                a3m[g] = ldo(INM(g), idx);
+              } else {
+                a2v[g] = (double)ldo((const TS*)RS(g, a.s_a2), idx);
+              }
             }
+          }
+        };
+    #pragma unroll
+        for (int g = 0; g < AHEAD; ++g) pre(g);
+    #pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          pre(f + AHEAD);
+          if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
+          const double cc = (double)rhs[f];
+          if (metric_field(f)) {
+            double* __restrict__ o = RM(f, a.s_out);
+            if (st == 0) {
+              sto(o, idx, cc);
+              continue;
+            }
+            const double p = PF ? lpf(f, 0) : pv[f];
+            double* __restrict__ fr = RM(f, a.s_f);
+            double fo = 0.0;
+            if constexpr (!INC) {
+              if (st >= 2) fo = PF ? lpf(f, 1) : qv[f];
+            } else if (st == 4) {
+              const double a1 = PF ? lpf(f, 1) : qv[f];
+              fo = ((a1 - p) + 2.0 * (a2v[f] - p) + (a3m[f] - p)) * (1.0 / 3.0);
+            }
+            const double inc = a.cf * cc;
+            const double nv = st == 4 ? fo + (inc + p) : p + a.ca * cc;
+            if (f == PHI) phi_new = nv;
+            if (f <= G33) gn[f] = nv;
+            sto(o, idx, nv);
+            if (st == 4) {
+              if (f == PHI && nan_as_float(nv)) atomicAdd(a.nan_count, 1u);
+            } else if (!INC) {
+              sto(fr, idx, fo + inc);
+            }
+          } else {
+            TS* __restrict__ o = RS(f, a.s_out);
+            if (st == 0) {
+              sto(o, idx, (TS)cc);
+              continue;
+            }
+            const double p = PF ? lpf(f, 0) : pv[f];
+            TS* __restrict__ fr = RS(f, a.s_f);
+            double fo = 0.0;
+            if constexpr (!INC) {
+              if (st >= 2) fo = PF ? lpf(f, 1) : qv[f];
+            } else if (st == 4) {
+              const double a1 = PF ? lpf(f, 1) : qv[f];
+              const double a2 = a2v[f];
+              // the stored value: v holds it exactly when storage and compute types agree
+              const double a3 = std::is_same<TS, TC>::value ? (double)v[f] : (double)ldo(INS(f), idx);
+              fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
+            }
+            const double inc = a.cf * cc;
+            const TS nv = st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc);
+            if (f == KK) K_new = (double)nv;
+            if (f >= A11 && f <= A33) An[f - A11] = (double)nv;
+            sto(o, idx, nv);
+            if (st != 4 && !INC) sto(fr, idx, (TS)(fo + inc));
           }
         }
       };
+      if (a.incr) epilogue(std::true_type{});
+      else epilogue(std::false_type{});
+      if (DMA && st == 4 && a.s_norm_out >= 0) {
+        // set_DIFFgamma_Aij_norm (bssn.cc:61-119) of the new state, as the
+        // next step's stepInit applies it to _a (DMA kernels only: the
+        // launcher pairs the copy with DMA staging)
+        const double tn = norm_factor<double>(gn[0], gn[1], gn[2], gn[3], gn[4], gn[5]);
+        double g[6];
+    #pragma unroll
+        for (int s = 0; s < 6; ++s) g[s] = (gn[s] + (diag6(s) ? tn : 0.0)) / (1.0 - tn);
+        const double i11 = 1.0 + g[3] + g[5] - g[4] * g[4] + g[3] * g[5];
+        const double i22 = 1.0 + g[0] + g[5] - g[2] * g[2] + g[0] * g[5];
+        const double i33 = 1.0 + g[0] + g[3] - g[1] * g[1] + g[0] * g[3];
+        const double i12 = g[2] * g[4] - g[1] * (1.0 + g[5]);
+        const double i13 = g[1] * g[4] - g[2] * (1.0 + g[3]);
+        const double i23 = g[1] * g[2] - g[4] * (1.0 + g[0]);
+        const double trA = i11 * An[0] + i22 * An[3] + i33 * An[5] + 2.0 * (i12 * An[1] + i13 * An[2] + i23 * An[4]);
+    #pragma unroll
+        for (int s = 0; s < 6; ++s) {
+          sto(RM(G11 + s, a.s_norm_out), idx, g[s]);
+          sto(RS(A11 + s, a.s_norm_out), idx,
+              (TS)((An[s] - (1.0 / 3.0) * ((diag6(s) ? 1.0 : 0.0) + g[s]) * trA) / (1.0 - tn)));
+        }
+      }
+      if constexpr (DUST) {
+        // dust right-hand sides (dust.cc:203-224): flux divergence plus the
+        // momentum source of populateDerivedFields (dust.cc:139-150)
+        TC drhs[NDUST];
+        drhs[0] = ddust[0];
+        const TC pre = TC(0.5) * alpha * Wc * Dc * exp(TC(4) * phi);
   #pragma unroll
-      for (int g = 0; g < AHEAD; ++g) pre(g);
+        for (int k = 0; k < 3; ++k) {
+          const TC dp4 = TC(4) * dphi[k];
+          const TC acc = vc[0] * vc[0] * (dp4 * GM(0, 0) + dg[k][0]) + vc[1] * vc[1] * (dp4 * GM(1, 1) + dg[k][3]) +
+                         vc[2] * vc[2] * (dp4 * GM(2, 2) + dg[k][5]) +
+                         TC(2) * (vc[0] * vc[1] * (dp4 * GM(0, 1) + dg[k][1]) + vc[0] * vc[2] * (dp4 * GM(0, 2) + dg[k][2]) +
+                                  vc[1] * vc[2] * (dp4 * GM(1, 2) + dg[k][4]));
+          drhs[1 + k] = ddust[1 + k] + (pre * acc - Wc * Dc * dal[k]);
+        }
   #pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        pre(f + AHEAD);
-        if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
-        const double cc = (double)rhs[f];
-        if (metric_field(f)) {
-          double* __restrict__ o = RM(f, a.s_out);
-          if (st == 0) {
-            sto(o, idx, cc);
-            continue;
-          }
-          const double p = PF ? lpf(f, 0) : pv[f];
-          double* __restrict__ fr = RM(f, a.s_f);
-          double fo = 0.0;
-          if constexpr (!INC) {
-            if (st >= 2) fo = PF ? lpf(f, 1) : qv[f];
-          } else if (st == 4) {
-            const double a1 = PF ? lpf(f, 1) : qv[f];
-            fo = ((a1 - p) + 2.0 * (a2v[f] - p) + (a3m[f] - p)) * (1.0 / 3.0);
-          }
-          const double inc = a.cf * cc;
-          const double nv = st == 4 ? fo + (inc + p) : p + a.ca * cc;
-          if (f == PHI) phi_new = nv;
-          if (f <= G33) gn[f] = nv;
-          sto(o, idx, nv);
-          if (st == 4) {
-            if (f == PHI && nan_as_float(nv)) atomicAdd(a.nan_count, 1u);
-          } else if (!INC) {
-            sto(fr, idx, fo + inc);
-          }
-        } else {
-          TS* __restrict__ o = RS(f, a.s_out);
+        for (int q = 0; q < NDUST; ++q) {
+          const double cc = (double)drhs[q];
+          TS* __restrict__ o = RD(q, a.s_out);
           if (st == 0) {
             sto(o, idx, (TS)cc);
             continue;
           }
-          const double p = PF ? lpf(f, 0) : pv[f];
-          TS* __restrict__ fr = RS(f, a.s_f);
+          const double p = (double)ldo((const TS*)RD(q, a.s_p), idx);
+          TS* __restrict__ fr = RD(q, a.s_f);
           double fo = 0.0;
-          if constexpr (!INC) {
-            if (st >= 2) fo = PF ? lpf(f, 1) : qv[f];
+          if (!a.incr) {
+            if (st >= 2) fo = (double)ldo((const TS*)fr, idx);
           } else if (st == 4) {
-            const double a1 = PF ? lpf(f, 1) : qv[f];
-            const double a2 = a2v[f];
-            // the stored value: v holds it exactly when storage and compute types agree
-            const double a3 = std::is_same<TS, TC>::value ? (double)v[f] : (double)ldo(INS(f), idx);
+            // accumulator-free form: K3Finalize's accumulator from the three stage outputs
+            const double a1 = (double)ldo((const TS*)fr, idx);
+            const double a2 = (double)ldo((const TS*)RD(q, a.s_a2), idx);
+            const double a3 = (double)ldo((const TS*)RD(q, a.s_in), idx);
             fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
           }
           const double inc = a.cf * cc;
-          const TS nv = st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc);
-          if (f == KK) K_new = (double)nv;
-          if (f >= A11 && f <= A33) An[f - A11] = (double)nv;
-          sto(o, idx, nv);
-          if (st != 4 && !INC) sto(fr, idx, (TS)(fo + inc));
+          sto(o, idx, st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc));
+          if (st != 4 && !a.incr) sto(fr, idx, (TS)(fo + inc));
         }
       }
-    };
-    if (a.incr) epilogue(std::true_type{});
-    else epilogue(std::false_type{});
-    if (DMA && st == 4 && a.s_norm_out >= 0) {
-      // set_DIFFgamma_Aij_norm (bssn.cc:61-119) of the new state, as the
-      // next step's stepInit applies it to _a (DMA kernels only: the
-      // launcher pairs the copy with DMA staging)
-      const double tn = norm_factor<double>(gn[0], gn[1], gn[2], gn[3], gn[4], gn[5]);
-      double g[6];
-  #pragma unroll
-      for (int s = 0; s < 6; ++s) g[s] = (gn[s] + (diag6(s) ? tn : 0.0)) / (1.0 - tn);
-      const double i11 = 1.0 + g[3] + g[5] - g[4] * g[4] + g[3] * g[5];
-      const double i22 = 1.0 + g[0] + g[5] - g[2] * g[2] + g[0] * g[5];
-      const double i33 = 1.0 + g[0] + g[3] - g[1] * g[1] + g[0] * g[3];
-      const double i12 = g[2] * g[4] - g[1] * (1.0 + g[5]);
-      const double i13 = g[1] * g[4] - g[2] * (1.0 + g[3]);
-      const double i23 = g[1] * g[2] - g[4] * (1.0 + g[0]);
-      const double trA = i11 * An[0] + i22 * An[3] + i33 * An[5] + 2.0 * (i12 * An[1] + i13 * An[2] + i23 * An[4]);
-  #pragma unroll
-      for (int s = 0; s < 6; ++s) {
-        sto(RM(G11 + s, a.s_norm_out), idx, g[s]);
-        sto(RS(A11 + s, a.s_norm_out), idx,
-            (TS)((An[s] - (1.0 / 3.0) * ((diag6(s) ? 1.0 : 0.0) + g[s]) * trA) / (1.0 - tn)));
-      }
-    }
-    if constexpr (DUST) {
-      // dust right-hand sides (dust.cc:203-224): flux divergence plus the
-      // momentum source of populateDerivedFields (dust.cc:139-150)
-      TC drhs[NDUST];
-      drhs[0] = ddust[0];
-      const TC pre = TC(0.5) * alpha * Wc * Dc * exp(TC(4) * phi);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const TC dp4 = TC(4) * dphi[k];
-        const TC acc = vc[0] * vc[0] * (dp4 * GM(0, 0) + dg[k][0]) + vc[1] * vc[1] * (dp4 * GM(1, 1) + dg[k][3]) +
-                       vc[2] * vc[2] * (dp4 * GM(2, 2) + dg[k][5]) +
-                       TC(2) * (vc[0] * vc[1] * (dp4 * GM(0, 1) + dg[k][1]) + vc[0] * vc[2] * (dp4 * GM(0, 2) + dg[k][2]) +
-                                vc[1] * vc[2] * (dp4 * GM(1, 2) + dg[k][4]));
-        drhs[1 + k] = ddust[1 + k] + (pre * acc - Wc * Dc * dal[k]);
-      }
-#pragma unroll
-      for (int q = 0; q < NDUST; ++q) {
-        const double cc = (double)drhs[q];
-        TS* __restrict__ o = RD(q, a.s_out);
-        if (st == 0) {
-          sto(o, idx, (TS)cc);
-          continue;
-        }
-        const double p = (double)ldo((const TS*)RD(q, a.s_p), idx);
-        TS* __restrict__ fr = RD(q, a.s_f);
-        double fo = 0.0;
-        if (!a.incr) {
-          if (st >= 2) fo = (double)ldo((const TS*)fr, idx);
-        } else if (st == 4) {
-          // accumulator-free form: K3Finalize's accumulator from the three stage outputs
-          const double a1 = (double)ldo((const TS*)fr, idx);
-          const double a2 = (double)ldo((const TS*)RD(q, a.s_a2), idx);
-          const double a3 = (double)ldo((const TS*)RD(q, a.s_in), idx);
-          fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
-        }
-        const double inc = a.cf * cc;
-        sto(o, idx, st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc));
-        if (st != 4 && !a.incr) sto(fr, idx, (TS)(fo + inc));
-      }
-    }
+    }  // constraint mode / stage
   }  // valid
+  if constexpr (CONS) {
+    if (a.cons_rows)
+      block_cons_put(a.cons_rows + (long)(t - a.tile_base) * CONS_ROW_W, a.cons_pass, a.cons_means, valid, cv);
+    return;
+  }
 
   // setExtraFieldData after K{n}Finalize (bssn.cc:267-305): the averages of
   // the new _a weighted with the new FRW phi; DIFFr is this stage's source.
@@ -1198,9 +1209,9 @@ __global__ void __launch_bounds__(TI* TJ* TK) __attribute__((amdgpu_waves_per_eu
 stage_kernel(const StageArgs a) {
   const int b = blockIdx.x;
   const int t = (b & 7) * a.per_xcd + (b >> 3);  // consecutive tiles stay on one XCD
-  if (t >= a.ntx * a.nty * a.ntz) return;
+  if (t >= (a.tile_count ? a.tile_count : a.ntx * a.nty * a.ntz)) return;
   stage_body<TS, TC, R, H, TI, TJ, TK, LIVE_ALPHA, MODE, DMA>(
-      a, t, FrwV{a.phi_F, a.K_F, a.rho_F, a.S_F, a.H_frw, a.phi_F_next});
+      a, a.tile_base + t, FrwV{a.phi_F, a.K_F, a.rho_F, a.S_F, a.H_frw, a.phi_F_next});
 }
 
 // Batched ensemble: one launch over the tiles of many grids.  Tiles are

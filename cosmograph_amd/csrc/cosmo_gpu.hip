@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -348,7 +349,59 @@ __global__ void k_combine(const double* __restrict__ part, int nb, int w, unsign
   }
 }
 
+// ------------------------------------- constraints without allocation --
+// Scratch of cg_constraints_noalloc: the rows of one chunk of kNaRows tiles
+// (one row per workgroup of the reduced constraint mode), then the running
+// totals of pass 0 (CONS_ROW_W) and pass 1 (CONS_ROW_W1) and the 10 means.
+// One copy per device, held until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returned for that device.
+constexpr int kNaRows = 4096;
+constexpr long kNaTot = (long)kNaRows * CONS_ROW_W;
+constexpr long kNaScratch = kNaTot + 64;  // doubles: 0.98 MB
+static_assert(CONS_ROW_W == 5 * kStats && CONS_ROW_W + 2 * CONS_ROW_W1 <= 64, "scratch layout");
+constexpr int kNaDevices = 64;
+double* g_scratch_dev[kNaDevices];  // per device, set once at context creation
+std::mutex g_scratch_mu;
+
+constexpr int kFoldThreads = 256;
+// acc[col] = (first ? 0 : acc[col]) (+ or max) the rows[r * stride + col],
+// r < n, in a fixed order: thread t takes rows t, t + 256, ..., then a fixed
+// tree.  Columns 6q + 4, 6q + 5 of pass 0 are maxima of non-negative values.
+__global__ void __launch_bounds__(kFoldThreads) k_rows_fold(const double* __restrict__ rows, int n, int stride,
+                                                           int width, int pass, int first, double* __restrict__ acc) {
+  __shared__ double sh[kFoldThreads];
+  for (int col = 0; col < width; ++col) {
+    const bool ismax = pass == 0 && (col % 6) >= 4;
+    double a = 0.0;
+    for (int r = threadIdx.x; r < n; r += kFoldThreads) {
+      const double v = rows[(long)r * stride + col];
+      a = ismax ? red_max(a, v) : a + v;
+    }
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = kFoldThreads / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w)
+        sh[threadIdx.x] = ismax ? red_max(sh[threadIdx.x], sh[threadIdx.x + w]) : sh[threadIdx.x] + sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[col] = first ? sh[0] : (ismax ? red_max(acc[col], sh[0]) : acc[col] + sh[0]);
+    __syncthreads();
+  }
+}
+
+// means of value and scaled per constraint for the second pass
+__global__ void k_cons_means(const double* __restrict__ acc0, double n, double* __restrict__ means) {
+  const int q = threadIdx.x;
+  if (q < 5) {
+    means[2 * q] = acc0[6 * q] / n;
+    means[2 * q + 1] = acc0[6 * q + 2] / n;
+  }
+}
+
 }  // namespace
+
+// the scratch array itself (file scope; see kNaScratch above)
+static __device__ __align__(256) double scratch[kNaScratch];
 
 namespace cg {
 
@@ -453,6 +506,15 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   c->fstride = 4 * pad;
   *out = c;
   HIPCHK(c, hipSetDevice(p.device));
+  if (p.device >= 0 && p.device < kNaDevices) {
+    // the device address of this device's copy of `scratch`, once per device
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[p.device]) {
+      void* ptr = nullptr;
+      HIPCHK(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      g_scratch_dev[p.device] = (double*)ptr;
+    }
+  }
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(c, hipMalloc(&c->regs, (size_t)(NF - 7) * 4 * pad * c->ts));
   HIPCHK(c, hipMemsetAsync(c->regs, 0, (size_t)(NF - 7) * 4 * pad * c->ts, c->stream));
@@ -1063,20 +1125,16 @@ int cg_kernel_info(const cg_ctx* cc, int* vgprs, int* scratch, int* lds, int* ti
 // the per-point constraint pass (prepBSSNOutput's set_bd_values on every
 // point, sims/sim.cc:214-231, with the constraint values of
 // setConstraintCalcs) into c->cons
-static int cons_eval(cg_ctx* c) {
+// The register the constraint pass reads: _a as the reference's output step
+// sees it, after stepInit, i.e. the normalised _p when the last call was a
+// fused step.  The momentum constraint differentiates A_ij, so the
+// normalised register is materialised (into the spare stage register and the
+// dead _f) rather than normalised on the fly.  Allocates nothing.
+static int cons_input(cg_ctx* c, int* rin_out) {
   // the pending K4Finalize averages read the dust source this pass rewrites
   if (int rc = ext_resolve(c)) return rc;
-  const long pad = c->rstride;
-  const int nb = 1024;
-  if (!c->cons) {
-    HIPCHK(c, hipMalloc((void**)&c->cons, (size_t)CONS_N * pad * sizeof(double)));
-    HIPCHK(c, hipMalloc((void**)&c->cons_part, (size_t)nb * 5 * kStats * sizeof(double) + 64 * sizeof(double)));
-  }
-  // evaluated on _a as the reference's output step sees it: after stepInit,
-  // i.e. the normalised _p when the last call was a fused step.  The momentum
-  // constraint differentiates A_ij, so the normalised register is
-  // materialised (into the spare stage register and the dead _f) rather than
-  // normalised on the fly.
+  if (c->p.nx <= 4 + c->R - 2 || c->p.ny <= 8 + c->R - 2 || c->p.nz <= 8 + c->R - 2)
+    return fail(c, CG_ERR_ARG, "grid too small for the constraint kernel tile");
   int rin = c->sA;
   if (c->a_is_p) {
     rin = (c->sC == 3) ? 2 : 3;  // the spare of the two stage registers
@@ -1091,20 +1149,60 @@ static int cons_eval(cg_ctx* c) {
     HIPCHK(c, hipGetLastError());
     if (c->dreg && dust_reg_init(c, c->sP, rin, c->sF)) return CG_ERR_HIP;
   }
-  // dust: sources from this state itself (initDustStep's populate, sims/dust.cc:53-63)
+  *rin_out = rin;
+  return CG_OK;
+}
+
+// arguments of a constraint pass over register rin with the 4x8x8 tile of
+// the constraint kernel; dust: sources from this state itself
+// (initDustStep's populate, sims/dust.cc:53-63)
+static StageArgs cons_args(cg_ctx* c, int rin) {
   StageArgs a = make_args(c, 0, rin, c->sP, c->sF, c->sC, 0);
-  a.cons = c->cons;
-  a.cons_stride = pad;
   a.ntx = (c->p.nx + 3) / 4;
   a.nty = (c->p.ny + 7) / 8;
   a.ntz = (c->p.nz + 7) / 8;
   a.per_xcd = (a.ntx * a.nty * a.ntz + 7) / 8;
-  if (c->p.nx <= 4 + c->R - 2 || c->p.ny <= 8 + c->R - 2 || c->p.nz <= 8 + c->R - 2)
-    return fail(c, CG_ERR_ARG, "grid too small for the constraint kernel tile");
+  return a;
+}
+
+static int cons_eval(cg_ctx* c) {
+  int rin;
+  if (int rc = cons_input(c, &rin)) return rc;
+  const long pad = c->rstride;
+  const int nb = 1024;
+  if (!c->cons) {
+    HIPCHK(c, hipMalloc((void**)&c->cons, (size_t)CONS_N * pad * sizeof(double)));
+    HIPCHK(c, hipMalloc((void**)&c->cons_part, (size_t)nb * 5 * kStats * sizeof(double) + 64 * sizeof(double)));
+  }
+  StageArgs a = cons_args(c, rin);
+  a.cons = c->cons;
+  a.cons_stride = pad;
   int rc = launch_constraints(c->prec, c->p.stencil_order, alpha_live(c), c->dreg != nullptr, a, c->stream);
   if (rc) return fail(c, CG_ERR_HIP, std::string("constraint launch: ") + hipGetErrorString((hipError_t)rc));
   c->cons_valid = true;
   return CG_OK;
+}
+
+// setConstraintCalcs' outputs from the totals: tot[6q + {0..5}] = sum val,
+// sum scale, sum scaled, sum val^2, max|val|, max|scaled|; means[2q + {0, 1}]
+// of val and scaled; var[2q + {0, 1}] their summed squared deviations
+static void cons_stats(const cg_ctx* c, const double* tot, const double* means, const double* var, double* H,
+                       double* M, double* G, double* A, double* S) {
+  const double N = (double)c->npts;
+  double* outs[5] = {H, M, G, A, S};
+  const double dx3 = c->p.dx * c->p.dx * c->p.dx;
+  for (int q = 0; q < 5; ++q) {
+    double* o = outs[q];
+    if (!o) continue;
+    o[0] = means[2 * q];
+    o[1] = sqrt(var[2 * q] / (N - 1.0));           // BSSN_NORMALIZE_STDEV
+    o[2] = tot[q * kStats + 4];
+    o[3] = tot[q * kStats + 1];                    // mean_scale is left as a sum
+    o[4] = means[2 * q + 1];
+    o[5] = sqrt(var[2 * q + 1] / (N - 1.0));
+    o[6] = tot[q * kStats + 5];
+    if (q < 2) o[7] = sqrt(tot[q * kStats + 3] * dx3);  // H_L2, M_L2 (bssn.cc:989, 994, 1067)
+  }
 }
 
 int cg_constraints(cg_ctx* c, double H[8], double M[8], double G[7], double A[7], double S[7]) {
@@ -1133,20 +1231,46 @@ int cg_constraints(cg_ctx* c, double H[8], double M[8], double G[7], double A[7]
   double var[10];
   HIPCHK(c, hipMemcpyAsync(var, fin + 40, sizeof var, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  double* outs[5] = {H, M, G, A, S};
-  const double dx3 = c->p.dx * c->p.dx * c->p.dx;
-  for (int q = 0; q < 5; ++q) {
-    double* o = outs[q];
-    if (!o) continue;
-    o[0] = means[2 * q];
-    o[1] = sqrt(var[2 * q] / (N - 1.0));           // BSSN_NORMALIZE_STDEV
-    o[2] = tot[q * kStats + 4];
-    o[3] = tot[q * kStats + 1];                    // mean_scale is left as a sum
-    o[4] = means[2 * q + 1];
-    o[5] = sqrt(var[2 * q + 1] / (N - 1.0));
-    o[6] = tot[q * kStats + 5];
-    if (q < 2) o[7] = sqrt(tot[q * kStats + 3] * dx3);  // H_L2, M_L2 (bssn.cc:989, 994, 1067)
+  cons_stats(c, tot, means, var, H, M, G, A, S);
+  return CG_OK;
+}
+
+int cg_constraints_noalloc(cg_ctx* c, double H[8], double M[8], double G[7], double A[7], double S[7]) {
+  if (!c) return CG_ERR_ARG;
+  double* sc = (c->p.device >= 0 && c->p.device < kNaDevices) ? g_scratch_dev[c->p.device] : nullptr;
+  if (!sc) return fail(c, CG_ERR_STATE, "no constraint scratch on this device (device ordinal >= 64)");
+  int rin;
+  if (int rc = cons_input(c, &rin)) return rc;
+  StageArgs a = cons_args(c, rin);
+  double* acc0 = sc + kNaTot;             // pass-0 totals [CONS_ROW_W]
+  double* acc1 = acc0 + CONS_ROW_W;       // pass-1 totals [CONS_ROW_W1]
+  double* dmeans = acc1 + CONS_ROW_W1;    // [10]
+  a.cons = nullptr;
+  a.cons_rows = sc;
+  a.cons_means = dmeans;
+  const int total = a.ntx * a.nty * a.ntz;
+  for (int pass = 0; pass < 2; ++pass) {
+    a.cons_pass = pass;
+    for (int base = 0; base < total; base += kNaRows) {
+      a.tile_base = base;
+      a.tile_count = std::min(kNaRows, total - base);
+      a.per_xcd = (a.tile_count + 7) / 8;
+      int rc = launch_constraints(c->prec, c->p.stencil_order, alpha_live(c), c->dreg != nullptr, a, c->stream);
+      if (rc) return fail(c, CG_ERR_HIP, std::string("constraint launch: ") + hipGetErrorString((hipError_t)rc));
+      hipLaunchKernelGGL(k_rows_fold, dim3(1), dim3(kFoldThreads), 0, c->stream, (const double*)sc, a.tile_count,
+                         CONS_ROW_W, pass == 0 ? CONS_ROW_W : CONS_ROW_W1, pass, base == 0 ? 1 : 0,
+                         pass == 0 ? acc0 : acc1);
+      HIPCHK(c, hipGetLastError());
+    }
+    if (pass == 0) {
+      hipLaunchKernelGGL(k_cons_means, dim3(1), dim3(64), 0, c->stream, (const double*)acc0, (double)c->npts, dmeans);
+      HIPCHK(c, hipGetLastError());
+    }
   }
+  double res[CONS_ROW_W + 2 * CONS_ROW_W1];
+  HIPCHK(c, hipMemcpyAsync(res, acc0, sizeof res, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  cons_stats(c, res, res + CONS_ROW_W + CONS_ROW_W1, res + CONS_ROW_W, H, M, G, A, S);
   return CG_OK;
 }
 

@@ -120,6 +120,7 @@ def load_library():
         "cg_set_dust_background": ([vp, C.c_int], C.c_int),
         "cg_autotune": ([vp, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
         "cg_constraints": ([vp, dp, dp, dp, dp, dp], C.c_int),
+        "cg_constraints_noalloc": ([vp, dp, dp, dp, dp, dp], C.c_int),
         "cg_constraint_strip": ([vp, C.c_int, C.c_long, C.c_long, dp, dp, C.c_size_t], C.c_int),
         "cg_field_strip": ([vp, C.c_char_p, C.c_int, C.c_long, C.c_long, dp, C.c_size_t], C.c_int),
         "cg_field_stats": ([vp, C.c_char_p, dp], C.c_int),
@@ -233,11 +234,14 @@ class BSSN:
         """nsteps whole RK4 steps (stepInit + 4 fused stages each)."""
         self._chk(self.L.cg_rk4_steps(self.h, nsteps))
 
-    def setConstraintCalcs(self):
-        """BSSN::setConstraintCalcs: dict of the H, M, G, A, S statistic arrays."""
+    def setConstraintCalcs(self, noalloc: bool = False):
+        """BSSN::setConstraintCalcs: dict of the H, M, G, A, S statistic arrays.
+        noalloc=True: the same statistics through cg_constraints_noalloc
+        (no device allocation; per-point arrays are not kept)."""
         H, M = (C.c_double * 8)(), (C.c_double * 8)()
         G, A, S = (C.c_double * 7)(), (C.c_double * 7)(), (C.c_double * 7)()
-        self._chk(self.L.cg_constraints(self.h, H, M, G, A, S))
+        fn = self.L.cg_constraints_noalloc if noalloc else self.L.cg_constraints
+        self._chk(fn(self.h, H, M, G, A, S))
         return {"H": list(H), "M": list(M), "G": list(G), "A": list(A), "S": list(S)}
 
     def set1DConstraintOutput(self, axis: int = 1, n1: int = 0, n2: int = 0):

@@ -112,6 +112,22 @@ int cg_rk4_steps(cg_ctx* ctx, int nsteps);
  * stage's source). */
 int cg_constraints(cg_ctx* ctx, double H[8], double M[8], double G[7], double A[7], double S[7]);
 
+/* cg_constraints without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * constraint kernel reduces each workgroup's points into one row of a
+ * fixed device array (0.98 MB per device, 4096 rows: the grid's tiles are
+ * evaluated in chunks of 4096, each folded into running totals in a fixed
+ * order), in two passes (sums and maxima, then the squared deviations from
+ * the means for the N-1 standard deviations), instead of the 22 per-point
+ * FP64 arrays cg_constraints allocates on first use (2.9 GB at 256^3).  Any
+ * grid cg_create accepts fits.  Same outputs as cg_constraints to FP64
+ * rounding (different summation order; maxima equal); "ricci_a",
+ * "AijAij_a", "DIFFS_a" and the constraint strips are not filled.  The array
+ * is shared by every context on a device: calls that overlap on one device
+ * race. */
+int cg_constraints_noalloc(cg_ctx* ctx, double H[8], double M[8], double G[7], double A[7], double S[7]);
+
 /* BSSN::set1DConstraintOutput (bssn.cc:903-938), read by the
  * 1D_hamiltonian_constraints / 1D_momentum_constraints strips of
  * io_bssn_constraint_violation (IO/io.cc:186-235): H and the momentum
