@@ -1,28 +1,33 @@
 """The 128^3 FP64 static-dust configuration at its full size against the
 oracle: 200 RK4 steps of order-4 BSSN + static dust from the seeded
 constraint-solved ICs (P(k) ~ k^-2, rms 1e-3, seed 9).  The oracle's run is
-held in tests/golden/c2_digest.json (tests/golden/make_c2_digest.py): the H
-and M statistics at the output point of every 10th step, and per field the
-sum, L2 norm, largest magnitude and 64 probe values of _p after 200 steps.
+held in tests/golden/c2_digest.json and c2_sample.npz
+(tests/golden/make_digest.py): the H and M statistics at the output point of
+every 10th step; per field the L2 norm and largest magnitude of _p after 200
+steps, and _p on the strided 32^3 sub-lattice.
 
-Bars: field L2 norms and largest magnitudes <= 1e-12 relative, sums within
-1e-12 of the L2 bound sqrt(N^3)*L2, probe values within 1e-11 of the field's
-largest magnitude, every constraint statistic within 1e-9 (relative, floored
-at 1e-3 of the largest |H| or |M|; the constraints differentiate the state
-twice), the FRW background bit for bit.  The GPU path runs its default
-(accumulator-free) RK4 form; the oracle runs the reference's."""
-import json
-import os
+Bars (the FP64 drop-in's 1e-12):
+  * every field: relative L2 of the difference on the 32^3 sample, and the
+    full field's L2 norm and largest magnitude, <= 1e-12 (fields the oracle
+    holds at zero: exactly zero);
+  * constraint statistics: the sum of scales to 1e-12 relative; the value
+    statistics (mean, stdev, max, L2) to 1e-12 of max(|value|, s), s the
+    mean scale (H[3]/N, M[3]/N), i.e. of the magnitude of the terms whose
+    difference the constraint is (H = R + 2/3 K^2 - A.A - 16 pi rho is a
+    residual ~1e-5 of terms ~1 at the first steps); the scaled statistics to
+    1e-12 of max(|value|, 1) for the same reason;
+  * the FRW background bit for bit.
+The GPU path runs its default (accumulator-free) RK4 form; the oracle runs
+the reference's."""
 import sys
 
 import numpy as np
 import pytest
 
+import digests
 from common import FIELDS
 
 pytestmark = pytest.mark.gpu
-
-HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 @pytest.fixture(scope="module")
@@ -37,48 +42,36 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _f(h):
-    return float.fromhex(h)
+def check_constraints(got, rec, npts, worst):
+    for c in ("H", "M"):
+        gaps, want = digests.constraint_gap(got[c], rec[c])
+        s = abs(want[3]) / npts
+        for q, w in enumerate(want):
+            worst[c][q] = max(worst[c][q], gaps[q])
+            if np.isnan(w):
+                assert np.isnan(got[c][q]), (rec["step"], c, q, got[c][q])
+                continue
+            floor = {3: 0.0, 4: 1.0, 5: 1.0, 6: 1.0}.get(q, s)
+            assert abs(got[c][q] - w) <= 1e-12 * max(abs(w), floor), (rec["step"], c, q, got[c][q], w)
 
 
 def test_c2_full_size_parity(gpu):
     from cosmograph_amd.ics import static_dust_ics
-    D = json.load(open(os.path.join(HERE, "golden", "c2_digest.json")))
-    cfg = D["config"]
+    d, sample = digests.load("c2")
+    cfg = d["config"]
     n = cfg["n"]
     _say(f"case c2 full size n={n} order={cfg['order']} steps={cfg['steps']}")
-    fields, frw = static_dust_ics(n, cfg["spectral_index"], cfg["rms"], cfg["seed"])
-    g = gpu.BSSN(gpu.GpuParams(n=n, order=cfg["order"], precision="fp64"))
-    for name, arr in fields.items():
-        g.upload(name, arr)
-    g.frw_set(frw["phi"], frw["K"])
-    for rho, w in frw["fluids"]:
-        g.frw_add_fluid(rho, w)
-    for rec in D["constraints"]:
+    g = digests.gpu_context(gpu, d, "fp64", static_dust_ics)
+    worst = {"H": [0.0] * 8, "M": [0.0] * 8}
+    for rec in d["constraints"]:
         if rec["step"]:
             g.step(cfg["every"])
-        got = g.setConstraintCalcs()
-        for c in ("H", "M"):
-            want = [_f(x) for x in rec[c]]
-            for q in range(8):
-                w = want[q]
-                if np.isnan(w):  # the scaled statistics of a constraint that vanishes (M of the ICs: 0/0)
-                    assert np.isnan(got[c][q]), (rec["step"], c, q, got[c][q])
-                    continue
-                assert abs(got[c][q] - w) <= 1e-9 * max(abs(w), 1e-3 * abs(want[2])), (rec["step"], c, q, got[c][q], w)
-    assert list(g.frw_get()) == [_f(x) for x in D["frw"]]
-    pts = D["probe_points"]
-    npts = float(n) ** 3
+        check_constraints(g.setConstraintCalcs(), rec, float(n) ** 3, worst)
+    _say(f"c2 worst relative constraint gaps: H {['%.1e' % x for x in worst['H']]} "
+         f"M {['%.1e' % x for x in worst['M']]}")
+    assert list(g.frw_get()) == [digests.fhex(x) for x in d["frw"]]
+    errs = digests.sample_errors(g, d, sample)
+    _say("c2 field gaps (sample rel L2): " + " ".join(f"{f} {errs[f]['sample']:.1e}" for f in FIELDS))
     for f in FIELDS:
-        a = g.download(f + "_p")
-        d = D["fields"][f]
-        l2, mx = _f(d["l2"]), _f(d["max_abs"])
-        if l2 == 0.0:
-            assert not np.any(a), f
-            continue
-        assert abs(np.sqrt(np.sum(a * a)) - l2) <= 1e-12 * l2, f
-        assert abs(np.abs(a).max() - mx) <= 1e-12 * mx, f
-        assert abs(a.sum() - _f(d["sum"])) <= 1e-12 * np.sqrt(npts) * l2, f
-        probe = np.array([a[i, j, k] for i, j, k in pts])
-        want = np.array([_f(x) for x in d["probes"]])
-        assert np.max(np.abs(probe - want)) <= 1e-11 * mx, (f, np.max(np.abs(probe - want)) / mx)
+        for k in ("sample", "l2", "max"):
+            assert errs[f][k] <= 1e-12, (f, k, errs[f][k])
