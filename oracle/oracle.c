@@ -34,6 +34,13 @@ static const char *SRC_NAMES[OR_NSRC] = {"DIFFr", "DIFFS", "S1",    "S2",
                                          "S3",    "STF11", "STF12", "STF13",
                                          "STF22", "STF23", "STF33"};
 static const char *DUST_NAMES[OR_NDUST] = {"D", "S1", "S2", "S3"};
+/* shift-build fields, bssn_macros.h:18-47 (order of BSSN_APPLY_TO_FIELDS) */
+static const char *X_NAMES[OR_NX] = {"beta1", "beta2", "beta3", "expN", "auxB1", "auxB2", "auxB3"};
+/* extra fields a build carries: none by default, beta and expN with
+ * USE_BSSN_SHIFT, plus auxB with USE_GAMMA_DRIVER */
+static int n_extra(const or_params *p) {
+  return p->shift == OR_SHIFT_NONE ? 0 : p->shift == OR_SHIFT_GAMMA_DRIVER ? OR_NX : OR_X_AUXB1;
+}
 /* Dust derived arrays, components/dust_fluid/dust.h:37-46 */
 enum {
   DD_ADV1 = 0, DD_AS1V1 = 3, DD_S1SRC = 12, DD_DETG = 15, DD_G11 = 16,
@@ -115,6 +122,7 @@ struct or_sim {
   long npts;
   long *xo, *yo, *zo; /* wrapped offsets, index range [-HALO, n+HALO) */
   double *reg[OR_NF][4];
+  double *xreg[OR_NX][4]; /* shift-build registers (n_extra of them allocated) */
   double *src[OR_NSRC];
   double *ricci, *AijAij, *K0;
   double *DIFFD;
@@ -145,6 +153,8 @@ or_sim *or_create(const or_params *p) {
   s->zo = mk_wrap(p->nz, 1);
   for (int f = 0; f < OR_NF; ++f)
     for (int r = 0; r < 4; ++r) s->reg[f][r] = zalloc(s->npts);
+  for (int f = 0; f < n_extra(p); ++f)
+    for (int r = 0; r < 4; ++r) s->xreg[f][r] = zalloc(s->npts);
   for (int f = 0; f < OR_NSRC; ++f) s->src[f] = zalloc(s->npts);
   s->ricci = zalloc(s->npts);
   s->AijAij = zalloc(s->npts);
@@ -164,6 +174,8 @@ void or_destroy(or_sim *s) {
   if (!s) return;
   for (int f = 0; f < OR_NF; ++f)
     for (int r = 0; r < 4; ++r) free(s->reg[f][r]);
+  for (int f = 0; f < OR_NX; ++f)
+    for (int r = 0; r < 4; ++r) free(s->xreg[f][r]);
   for (int f = 0; f < OR_NSRC; ++f) free(s->src[f]);
   free(s->ricci); free(s->AijAij); free(s->K0); free(s->DIFFD);
   for (int f = 0; f < OR_NDUST; ++f)
@@ -209,6 +221,8 @@ double *or_field(or_sim *s, const char *name) {
   }
   for (int f = 0; f < OR_NF; ++f)
     if (!strcmp(base, FIELD_NAMES[f])) return s->reg[f][r];
+  for (int f = 0; f < n_extra(&s->p); ++f)
+    if (!strcmp(base, X_NAMES[f])) return s->xreg[f][r];
   if (r != OR_A) return NULL;
   for (int f = 0; f < OR_NSRC; ++f)
     if (!strcmp(base, SRC_NAMES[f])) return s->src[f];
@@ -326,6 +340,50 @@ static double ko_q(const or_sim *s, const double *f, long i, long j, long k) {
   return 0.0;
 }
 
+/* lop_forward_derivative / lop_backward_derivative (math.h:1220-1232): the
+ * lopsided first derivative of the stencil order plus its forward/backward
+ * dissipation stencil.  Order 2: math.h:166-230 and 1087-1130 (the second
+ * derivative stencils 888-956); order 4: math.h:265-337 (no dissipation,
+ * 1132-1146); orders 6 and 8 are stubs that return 0 (math.h:539-549,
+ * 661-673, 1148-1178). */
+static double lop(const or_sim *s, const double *f, long i, long j, long k, int d, int fwd) {
+  const double dx = s->p.dx;
+#define P(t) at1(s, f, i, j, k, d, (t))
+  if (s->p.order == 2) {
+    if (fwd)
+      return ((-1.0 / 2.0 * P(2) + 2.0 * P(1) - 3.0 / 2.0 * P(0)) / dx) +
+             -1.0 / 2.0 * dx * ((2.0 * P(0) - 5.0 * P(1) + 4.0 * P(2) - 1.0 * P(3)) / dx / dx);
+    return ((+1.0 / 2.0 * P(-2) - 2.0 * P(-1) + 3.0 / 2.0 * P(0)) / dx) +
+           +1.0 / 2.0 * dx * ((2.0 * P(0) - 5.0 * P(-1) + 4.0 * P(-2) - 1.0 * P(-3)) / dx / dx);
+  }
+  if (s->p.order == 4) {
+    if (fwd)
+      return (+1.0 / 12.0 * P(3) - 1.0 / 2.0 * P(2) + 3.0 / 2.0 * P(1) - 5.0 / 6.0 * P(0) - 1.0 / 4.0 * P(-1)) / dx +
+             0.0;
+    return (-1.0 / 12.0 * P(-3) + 1.0 / 2.0 * P(-2) - 3.0 / 2.0 * P(-1) + 5.0 / 6.0 * P(0) + 1.0 / 4.0 * P(1)) / dx +
+           0.0;
+  }
+  return 0.0 + 0.0;
+#undef P
+}
+
+/* upwind_derivative, math.h:1247-1252 */
+static double upwind(const or_sim *s, const double *f, long i, long j, long k, int d, double c) {
+  if (c > 0) return c * lop(s, f, i, j, k, d, 1);
+  return c * lop(s, f, i, j, k, d, 0);
+}
+
+void or_upwind(or_sim *s, const double *f, const double *c, double *out) {
+  const long P = s->npts;
+  for (int d = 1; d <= 3; ++d)
+    for (long i = 0; i < s->p.nx; ++i)
+      for (long j = 0; j < s->p.ny; ++j)
+        for (long k = 0; k < s->p.nz; ++k) {
+          long idx = (i * s->p.ny + j) * s->p.nz + k;
+          out[(d - 1) * P + idx] = upwind(s, f, i, j, k, d, c[idx]);
+        }
+}
+
 double or_stencil(or_sim *s, const double *f, int kind, int d1, int d2, long i, long j, long k) {
   switch (kind) {
   case 0: return deriv(s, f, i, j, k, d1);
@@ -336,8 +394,12 @@ double or_stencil(or_sim *s, const double *f, int kind, int d1, int d2, long i, 
 
 /* ------------------------------------------------------- point algebra --- */
 /* Restates BSSN::set_bd_values (bssn.cc:361-421) after the stencil loads,
- * then the ev_* functions (bssn.cc:748-831).  theta = 0, beta = 0 (Z4c and
- * shift compiled out by default, cosmo_macros.h:43-45, 88-90). */
+ * then the ev_* functions (bssn.cc:748-891).  theta = 0 (Z4c compiled out by
+ * default, cosmo_macros.h:88-90).  beta = 0 unless p->shift selects the
+ * USE_BSSN_SHIFT build (cosmo_macros.h:43-44): then the shift terms of
+ * bssn_macros.h:292-385 and bssn.cc:786-828 are added, with the advection
+ * terms (upwind derivatives) supplied in bd->adv, and the extra fields'
+ * right-hand sides (bssn.cc:852-891) are formed in bd->rhsx. */
 void or_point_compute(const or_params *p, or_point *bd) {
   const double *v = bd->v;
   const double theta = 0.0;
@@ -482,8 +544,21 @@ void or_point_compute(const or_params *p, or_point *bd) {
 
   /* ---- ev_* : bssn.cc:748-831, bssn_macros.h:304-385 ---- */
   const double dt = p->dt, alpha = bd->alpha, H = bd->H;
-  for (int a = 0; a < 6; ++a)
-    bd->rhs[a] = (-2.0 * alpha * v[F_A11 + a]) + 0.5 * p->a_adj_amp * dt * H * v[a] - bd->ko[a];
+  const int shift = p->shift != OR_SHIFT_NONE;
+  const double *b = bd->xv; /* beta^1..3 */
+#define DB(j, i) bd->dbeta[j][i]
+  const double divb = DB(0, 0) + DB(1, 1) + DB(2, 2);
+  /* BSSN_DT_DIFFGAMMAIJ, bssn_macros.h:292-307 */
+  for (int I = 0; I < 3; ++I)
+    for (int J = I; J < 3; ++J) {
+      int a = SY[I][J];
+      double dtg = -2.0 * alpha * v[F_A11 + a];
+      if (shift)
+        dtg = -2.0 * alpha * v[F_A11 + a] + bd->adv[a] + GG(I, 0) * DB(J, 0) + GG(I, 1) * DB(J, 1) +
+              GG(I, 2) * DB(J, 2) + GG(J, 0) * DB(I, 0) + GG(J, 1) * DB(I, 1) + GG(J, 2) * DB(I, 2) -
+              (2.0 / 3.0) * GG(I, J) * (divb);
+      bd->rhs[a] = (dtg) + 0.5 * p->a_adj_amp * dt * H * v[a] - bd->ko[a];
+    }
   for (int I = 0; I < 3; ++I)
     for (int J = I; J < 3; ++J) {
       int a = SY[I][J];
@@ -497,6 +572,10 @@ void or_point_compute(const or_params *p, or_point *bd) {
         }
       double dtA = (exp(-4.0 * bd->phi) * (alpha * (bd->ricciTF[a] - 8.0 * PI * bd->src[5 + a]) - bd->DDaTF[a]) +
                     alpha * ((bd->K + 2.0 * theta) * v[F_A11 + a] - 2.0 * (AA2)));
+      if (shift) /* bssn_macros.h:319-333 */
+        dtA = dtA + bd->adv[F_A11 + a] + AA(I, 0) * DB(J, 0) + AA(I, 1) * DB(J, 1) + AA(I, 2) * DB(J, 2) +
+              AA(J, 0) * DB(I, 0) + AA(J, 1) * DB(I, 1) + AA(J, 2) * DB(I, 2) -
+              (2.0 / 3.0) * AA(I, J) * (divb);
       bd->rhs[F_A11 + a] = dtA - 1.0 * p->a_adj_amp * dt * v[F_A11 + a] * H - bd->ko[F_A11 + a];
     }
   for (int I = 0; I < 3; ++I) {
@@ -511,31 +590,90 @@ void or_point_compute(const or_params *p, or_point *bd) {
               (1.0 / 3.0) * (2.0 * (GI(I, 0) * bd->dK[0] + GI(I, 1) * bd->dK[1] + GI(I, 2) * bd->dK[2]) + 0.0) -
               2.0 * alpha * 0.0 * (v[F_GAM1 + I] - bd->Gammad[I]) -
               8.0 * PI * (GI(I, 0) * bd->src[2] + GI(I, 1) * bd->src[3] + GI(I, 2) * bd->src[4])));
-    bd->rhs[F_GAM1 + I] = (noshift + 0.0) - bd->ko[F_GAM1 + I];
+    double shiftG = 0.0;
+    if (shift) { /* BSSN_DT_GAMMAI_SHIFT, bssn_macros.h:363-382 */
+      const double(*ddb)[6] = bd->ddbeta;
+#define DDB(k, l, m) ddb[m][SY[k][l]] /* d_k d_l beta^m */
+      shiftG = bd->adv[F_GAM1 + I] - bd->Gammad[0] * DB(0, I) - bd->Gammad[1] * DB(1, I) -
+               bd->Gammad[2] * DB(2, I) + (2.0 / 3.0) * bd->Gammad[I] * (divb) +
+               (1.0 / 3.0) * (GI(I, 0) * DDB(0, 0, 0) + GI(I, 0) * DDB(1, 0, 1) + GI(I, 0) * DDB(2, 0, 2) +
+                              GI(I, 1) * DDB(0, 1, 0) + GI(I, 1) * DDB(1, 1, 1) + GI(I, 1) * DDB(2, 1, 2) +
+                              GI(I, 2) * DDB(0, 2, 0) + GI(I, 2) * DDB(1, 2, 1) + GI(I, 2) * DDB(2, 2, 2)) +
+               (GI(0, 0) * DDB(0, 0, I) + GI(1, 1) * DDB(1, 1, I) + GI(2, 2) * DDB(2, 2, I) +
+                2.0 * (GI(0, 1) * DDB(0, 1, I) + GI(0, 2) * DDB(0, 2, I) + GI(1, 2) * DDB(1, 2, I)));
+#undef DDB
+    }
+    bd->rhs[F_GAM1 + I] = (noshift + shiftG) - bd->ko[F_GAM1 + I];
   }
   bd->rhs[F_K] = (-bd->DDaTR +
                   alpha * (1.0 / 3.0 * (v[F_K] + 2.0 * theta) * (v[F_K] + 2.0 * theta + 2.0 * bd->K_F) + bd->AijAij) +
-                  4.0 * PI * alpha * (bd->src[0] + bd->src[1]) + 4.0 * PI * v[F_ALPHA] * (bd->rho_F + bd->S_F) -
-                  1.0 * p->k_damping_amp * H * exp(-5.0 * bd->phi) + 0.0 * (1.0 - 0.0) * theta - bd->ko[F_K]);
+                  4.0 * PI * alpha * (bd->src[0] + bd->src[1]) + 4.0 * PI * v[F_ALPHA] * (bd->rho_F + bd->S_F) +
+                  bd->adv[F_K] - 1.0 * p->k_damping_amp * H * exp(-5.0 * bd->phi) + 0.0 * (1.0 - 0.0) * theta -
+                  bd->ko[F_K]);
   bd->rhs[F_PHI] = (0.1 * p->a_adj_amp * dt * H -
-                    1.0 / 6.0 * (alpha * (v[F_K] + 2.0 * theta) + v[F_ALPHA] * bd->K_F - (0.0 + 0.0 + 0.0)) -
-                    bd->ko[F_PHI]);
+                    1.0 / 6.0 * (alpha * (v[F_K] + 2.0 * theta) + v[F_ALPHA] * bd->K_F - (DB(0, 0) + DB(1, 1) + DB(2, 2))) +
+                    bd->adv[F_PHI] - bd->ko[F_PHI]);
   /* gauge: BSSNGaugeHandler.cc:16-35 */
   double lapse = 0.0;
   switch (p->lapse) {
   case OR_LAPSE_HARMONIC: lapse = -1.0 * alpha * (bd->K); break;                        /* :32-35 */
   case OR_LAPSE_CONFORMAL_FLRW: lapse = -1.0 / 3.0 * alpha * bd->K_avg; break;          /* :24-27 */
   case OR_LAPSE_ANHARMONIC: lapse = 1.0 * PW2(alpha) * (bd->K - bd->K_avg); break;      /* :50-58 */
-  case OR_LAPSE_ONE_PLUS_LOG: lapse = -2.0 * alpha * (bd->K - bd->K_avg) * p->gd_c; break; /* :64-68, beta = 0 */
+  case OR_LAPSE_ONE_PLUS_LOG:                                                           /* :64-68 */
+    lapse = -2.0 * alpha * (bd->K - bd->K_avg) * p->gd_c + b[0] * bd->dalpha[0] + b[1] * bd->dalpha[1] +
+            b[2] * bd->dalpha[2];
+    break;
   case OR_LAPSE_CONFORMAL_SYNC: lapse = -1.0 / 3.0 * alpha * bd->K_F; break;            /* :84-87 */
   case OR_LAPSE_AWA_GAUGE_WAVE: lapse = -1.0 * PW2(alpha) * v[F_K]; break;              /* :209-212 */
   case OR_LAPSE_AWA_SHIFTED_WAVE: lapse = -1.0 * v[F_K]; break;                         /* :217-220 */
-  case OR_LAPSE_DAMPED_WAVE:                                                            /* :151-155, beta = 0 */
-    lapse = PW2(alpha) * (p->dw_mu_l * (12.0 * bd->phi * p->dw_p - log(alpha)) - bd->K);
+  case OR_LAPSE_DAMPED_WAVE:                                                            /* :151-155 */
+    lapse = PW2(alpha) * (p->dw_mu_l * (12.0 * bd->phi * p->dw_p - log(alpha)) - bd->K) + b[0] * bd->dalpha[0] +
+            b[1] * bd->dalpha[1] + b[2] * bd->dalpha[2];
+    break;
+  case OR_LAPSE_TEST_K_DRIVER: /* :93-102; ev_DIFFK is the complete right-hand side above */
+    lapse = -1.0 * p->k_driver_coeff *
+            ((bd->rhs[F_K] - (PW2(bd->K_avg) / 3.0 + 4.0 * PI * bd->rho_avg)) + (bd->K - bd->K_avg));
     break;
   default: lapse = 0.0; break;                                  /* Static, TestAijDriverLapse */
   }
-  bd->rhs[F_ALPHA] = lapse - bd->ko[F_ALPHA];
+  bd->rhs[F_ALPHA] = lapse + bd->adv[F_ALPHA] - bd->ko[F_ALPHA];
+  if (shift) {
+    /* ev_beta^i = shift gauge - KO (bssn.cc:852-869, BSSNGaugeHandler.cc:113-273) */
+    for (int I = 0; I < 3; ++I) {
+      double sh = 0.0;
+      switch (p->shift) {
+      case OR_SHIFT_GAMMA_DRIVER: sh = bd->xv[OR_X_AUXB1 + I]; break; /* :113-143 */
+      case OR_SHIFT_DAMPED_WAVE: {                                     /* :157-207 */
+        /* the y-component reads Gamma3 where Gamma2 is meant (:185), kept */
+        const double Gq = v[F_GAM1 + (I == 0 ? 0 : 2)];
+        sh = b[0] * DB(0, I) + b[1] * DB(1, I) + b[2] * DB(2, I) - p->dw_mu_s * alpha * b[I] +
+             alpha * (-p->dw_mu_l * (12.0 * bd->phi * p->dw_p - log(alpha)) * b[I] +
+                      exp(-4.0 * bd->phi) *
+                          (-GI(I, 0) * bd->dalpha[0] - GI(I, 1) * bd->dalpha[1] - GI(I, 2) * bd->dalpha[2] +
+                           alpha * (Gq - 2.0 * (GI(I, 0) * bd->dphi[0] + GI(I, 1) * bd->dphi[1] +
+                                                GI(I, 2) * bd->dphi[2]))));
+        break;
+      }
+      case OR_SHIFT_AWA_SHIFTED_WAVE: /* :225-253 */
+        sh = (p->gauge_wave_dir == I + 1) ? -2.0 * bd->K * alpha : 0.0;
+        break;
+      case OR_SHIFT_AIJ_DRIVER: /* :262-273: |ev_A_Ij + A_Ij| over j */
+        sh = -p->k_driver_coeff * (fabs(bd->rhs[F_A11 + SY[I][0]] + AA(I, 0)) +
+                                   fabs(bd->rhs[F_A11 + SY[I][1]] + AA(I, 1)) +
+                                   fabs(bd->rhs[F_A11 + SY[I][2]] + AA(I, 2)));
+        break;
+      default: sh = 0.0; break; /* Static */
+      }
+      bd->rhsx[OR_X_BETA1 + I] = sh - bd->kox[OR_X_BETA1 + I];
+    }
+    /* ev_expN, bssn.cc:871-875 */
+    bd->rhsx[OR_X_EXPN] = b[0] * bd->dexpN[0] + b[1] * bd->dexpN[1] + b[2] * bd->dexpN[2] - alpha * bd->K / 3.0;
+    /* ev_auxB^i, bssn.cc:878-891 */
+    if (p->shift == OR_SHIFT_GAMMA_DRIVER)
+      for (int I = 0; I < 3; ++I)
+        bd->rhsx[OR_X_AUXB1 + I] = 0.75 * bd->rhs[F_GAM1 + I] - p->gd_eta * bd->xv[OR_X_AUXB1 + I];
+  }
+#undef DB
 #undef GI
 #undef GG
 #undef AA
@@ -616,6 +754,26 @@ static void fill_point(or_sim *s, long i, long j, long k, or_point *bd, int with
       for (int jj = 0; jj < 3; ++jj) bd->dA[jj][a] = deriv(s, s->reg[F_A11 + a][OR_A], i, j, k, jj + 1);
   if (with_ko)
     for (int f = 0; f < OR_NF; ++f) bd->ko[f] = ko_q(s, s->reg[f][OR_A], i, j, k);
+  if (s->p.shift != OR_SHIFT_NONE) {
+    /* calculate_dbeta, calculate_dexpN (bssn.cc:547-566); the double
+     * derivatives of beta that BSSN_DT_GAMMAI_SHIFT evaluates in place;
+     * the upwind advection terms of every field (bssn_macros.h:296-370,
+     * bssn.cc:786-828) */
+    for (int f = 0; f < n_extra(&s->p); ++f) bd->xv[f] = s->xreg[f][OR_A][idx];
+    for (int m = 0; m < 3; ++m) {
+      const double *bm = s->xreg[OR_X_BETA1 + m][OR_A];
+      for (int jj = 0; jj < 3; ++jj) bd->dbeta[jj][m] = deriv(s, bm, i, j, k, jj + 1);
+      for (int I = 0; I < 3; ++I)
+        for (int J = I; J < 3; ++J) bd->ddbeta[m][SY[I][J]] = dderiv(s, bm, i, j, k, I + 1, J + 1);
+      if (with_ko) bd->kox[OR_X_BETA1 + m] = ko_q(s, bm, i, j, k);
+    }
+    for (int d = 0; d < 3; ++d) bd->dexpN[d] = deriv(s, s->xreg[OR_X_EXPN][OR_A], i, j, k, d + 1);
+    for (int f = 0; f < OR_NF; ++f) {
+      const double *fa = s->reg[f][OR_A];
+      bd->adv[f] = upwind(s, fa, i, j, k, 1, bd->xv[0]) + upwind(s, fa, i, j, k, 2, bd->xv[1]) +
+                   upwind(s, fa, i, j, k, 3, bd->xv[2]);
+    }
+  }
   or_point_compute(&s->p, bd);
   /* bssn.cc:469-471, 663 */
   s->AijAij[idx] = bd->AijAij;
@@ -633,6 +791,7 @@ void or_bssn_rk_evolve(or_sim *s) {
         fill_point(s, i, j, k, &bd, 1, 0);
         long idx = ((long)i * ny + j) * nz + k;
         for (int f = 0; f < OR_NF; ++f) s->reg[f][OR_C][idx] = bd.rhs[f];
+        for (int f = 0; f < n_extra(&s->p); ++f) s->xreg[f][OR_C][idx] = bd.rhsx[f];
       }
   }
 }
@@ -845,6 +1004,7 @@ void or_bssn_normalize(or_sim *s) {
 /* BSSN::stepInit, bssn.cc:219-226 */
 void or_bssn_step_init(or_sim *s) {
   for (int f = 0; f < OR_NF; ++f) reg_step_init(s->reg[f], s->npts);
+  for (int f = 0; f < n_extra(&s->p); ++f) reg_step_init(s->xreg[f], s->npts);
   or_bssn_extra_field_data(s);
   if (s->p.normalize_metric) or_bssn_normalize(s);
 }
@@ -853,6 +1013,7 @@ void or_bssn_step_init(or_sim *s) {
 void or_bssn_k_finalize(or_sim *s, int n) {
   frw_stage(&s->frw, n, s->p.dt);
   for (int f = 0; f < OR_NF; ++f) reg_finalize(s->reg[f], s->npts, n, s->p.dt);
+  for (int f = 0; f < n_extra(&s->p); ++f) reg_finalize(s->xreg[f], s->npts, n, s->p.dt);
   or_bssn_extra_field_data(s);
 }
 

@@ -31,6 +31,8 @@ extern "C" {
 #define OR_NF 18     /* evolved BSSN fields, reference order bssn_macros.h:77-99 */
 #define OR_NSRC 11   /* source arrays bssn_macros.h:101-112 */
 #define OR_NDUST 4   /* D, S1, S2, S3 (components/dust_fluid/dust.h:32-35) */
+#define OR_NX 7      /* shift build: beta1..3, expN (USE_BSSN_SHIFT), auxB1..3 (USE_GAMMA_DRIVER),
+                        bssn_macros.h:18-47 */
 
 enum { OR_P = 0, OR_A = 1, OR_C = 2, OR_F = 3 };
 enum { OR_MATTER_NONE = 0, OR_MATTER_STATIC = 1, OR_MATTER_DUST = 2 };
@@ -39,8 +41,18 @@ enum { OR_MATTER_NONE = 0, OR_MATTER_STATIC = 1, OR_MATTER_DUST = 2 };
 enum {
   OR_LAPSE_STATIC = 0, OR_LAPSE_HARMONIC = 1, OR_LAPSE_CONFORMAL_FLRW = 2, OR_LAPSE_ANHARMONIC = 3,
   OR_LAPSE_ONE_PLUS_LOG = 4, OR_LAPSE_CONFORMAL_SYNC = 5, OR_LAPSE_AWA_GAUGE_WAVE = 6,
-  OR_LAPSE_AWA_SHIFTED_WAVE = 7, OR_LAPSE_DAMPED_WAVE = 8, OR_LAPSE_TEST_AIJ_DRIVER = 9
+  OR_LAPSE_AWA_SHIFTED_WAVE = 7, OR_LAPSE_DAMPED_WAVE = 8, OR_LAPSE_TEST_AIJ_DRIVER = 9,
+  OR_LAPSE_TEST_K_DRIVER = 10
 };
+/* shift gauges (BSSNGaugeHandler.h:113-135).  0 = the default build without
+ * USE_BSSN_SHIFT (beta = 0, no extra fields); any other value is the
+ * USE_BSSN_SHIFT build with that shift, GammaDriver also with
+ * USE_GAMMA_DRIVER (the auxB fields). */
+enum {
+  OR_SHIFT_NONE = 0, OR_SHIFT_STATIC = 1, OR_SHIFT_GAMMA_DRIVER = 2, OR_SHIFT_DAMPED_WAVE = 3,
+  OR_SHIFT_AWA_SHIFTED_WAVE = 4, OR_SHIFT_AIJ_DRIVER = 5
+};
+enum { OR_X_BETA1 = 0, OR_X_EXPN = 3, OR_X_AUXB1 = 4 };
 
 typedef struct {
   int nx, ny, nz;
@@ -55,6 +67,11 @@ typedef struct {
   double lambda;          /* Lambda::lambda (rho_Lambda) */
   double gd_c;            /* gauge parameters (BSSNGaugeHandler.h:139-149) */
   double dw_mu_l, dw_p;
+  int shift;              /* OR_SHIFT_* */
+  int gauge_wave_dir;     /* AwA shifted wave direction 1..3 (BSSNGaugeHandler.h:71) */
+  double gd_eta;          /* Gamma-driver damping (bssn.cc:18) */
+  double dw_mu_s;         /* damped-wave shift parameter (BSSNGaugeHandler.h:67) */
+  double k_driver_coeff;  /* TestKDriverLapse / AijDriverShift (BSSNGaugeHandler.h:85) */
 } or_params;
 
 /* Per-point data: the oracle's analogue of BSSNData.  Inputs first (filled
@@ -87,6 +104,15 @@ typedef struct {
   double Uricci[6], ricci_ij[6], ricci, ricciTF[6];
   double H;
   double rhs[OR_NF];
+  /* ---- shift build inputs (zero without USE_BSSN_SHIFT) ---- */
+  double xv[OR_NX];       /* beta1..3, expN, auxB1..3 (_a register) */
+  double dbeta[3][3];     /* d_j beta^i  [j][i] */
+  double ddbeta[3][6];    /* d_k d_l beta^i  [i][sym kl] */
+  double dexpN[3];
+  double adv[OR_NF];      /* per field: sum over d of upwind_derivative(d, field, beta^d) */
+  double kox[OR_NX];      /* KO_dissipation_Q of beta^i (the other extras have none) */
+  /* ---- shift build outputs ---- */
+  double rhsx[OR_NX];
 } or_point;
 
 typedef struct or_sim or_sim;
@@ -143,6 +169,9 @@ void or_point_constraints(const or_point *bd, double out[15]);
 void or_constraint_points(or_sim *s, double *out);
 /* grid stencil probes (for bitwise comparison with utils/math.h) */
 double or_stencil(or_sim *s, const double *f, int kind, int d1, int d2, long i, long j, long k);
+/* upwind_derivative (utils/math.h:1247-1252) along d with speed c at every point:
+ * out[(d-1)*P + idx], c per point */
+void or_upwind(or_sim *s, const double *f, const double *c, double *out);
 
 #ifdef __cplusplus
 }

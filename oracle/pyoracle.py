@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 PERF_LIB_PATH = os.path.join(HERE, "liboracle_perf.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
-NF, NSRC = 18, 11
+NF, NSRC, NX = 18, 11, 7
 FIELDS = ["DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIFFgamma22", "DIFFgamma23",
           "DIFFgamma33", "DIFFphi", "A11", "A12", "A13", "A22", "A23", "A33", "DIFFK",
           "Gamma1", "Gamma2", "Gamma3", "DIFFalpha"]
@@ -32,7 +32,11 @@ SOURCES = ["DIFFr", "DIFFS", "S1", "S2", "S3", "STF11", "STF12", "STF13", "STF22
 MATTER = {"none": 0, "static": 1, "dust": 2}
 LAPSE = {"Static": 0, "Harmonic": 1, "ConformalFLRW": 2, "Anharmonic": 3, "OnePlusLog": 4,
          "ConformalSync": 5, "AwAGaugeWave": 6, "AwAShiftedWave": 7, "DampedWave": 8,
-         "TestAijDriverLapse": 9}
+         "TestAijDriverLapse": 9, "TestKDriverLapse": 10}
+# shift gauges; "none" is the default build without USE_BSSN_SHIFT
+SHIFT = {"none": 0, "Static": 1, "GammaDriver": 2, "DampedWave": 3, "AwAShiftedWave": 4, "AijDriverShift": 5}
+# fields of the USE_BSSN_SHIFT build (auxB only with the gamma driver)
+X_FIELDS = ["beta1", "beta2", "beta3", "expN", "auxB1", "auxB2", "auxB3"]
 
 
 class Params(C.Structure):
@@ -41,7 +45,8 @@ class Params(C.Structure):
                 ("a_adj_amp", C.c_double), ("k_damping_amp", C.c_double),
                 ("normalize_metric", C.c_int), ("lapse", C.c_int), ("matter", C.c_int),
                 ("lambda_", C.c_double), ("gd_c", C.c_double), ("dw_mu_l", C.c_double),
-                ("dw_p", C.c_double)]
+                ("dw_p", C.c_double), ("shift", C.c_int), ("gauge_wave_dir", C.c_int),
+                ("gd_eta", C.c_double), ("dw_mu_s", C.c_double), ("k_driver_coeff", C.c_double)]
 
 
 def _point_fields():
@@ -55,7 +60,9 @@ def _point_fields():
             ("g", d * 6), ("gi", d * 6), ("Acont", d * 6), ("AijAij", d),
             ("G", (d * 6) * 3), ("GL", (d * 6) * 3), ("Gammad", d * 3), ("DDphi", d * 6),
             ("DDaTF", d * 6), ("DDaTR", d), ("Uricci", d * 6), ("ricci_ij", d * 6),
-            ("ricci", d), ("ricciTF", d * 6), ("H", d), ("rhs", d * NF)]
+            ("ricci", d), ("ricciTF", d * 6), ("H", d), ("rhs", d * NF),
+            ("xv", d * NX), ("dbeta", (d * 3) * 3), ("ddbeta", (d * 6) * 3), ("dexpN", d * 3),
+            ("adv", d * NF), ("kox", d * NX), ("rhsx", d * NX)]
 
 
 class Point(C.Structure):
@@ -114,17 +121,20 @@ def lib(perf: bool = False):
         L.or_reduction.restype = C.c_double
         L.or_constraint_strip.argtypes = [vp, C.c_int, C.c_long, C.c_long, dp, dp]
         L.or_rk4_demo.argtypes = [dp, C.c_long, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]
+        L.or_upwind.argtypes = [vp, dp, dp, dp]
         _libs[key] = L
     return _libs[key]
 
 
 def make_params(n, order=4, dx=None, dt_frac=0.1, ko=0.0, a_adj=0.0, k_damp=0.0,
                 normalize=1, lapse="Static", matter="static", lam=0.0, box=1.0, gd_c=1.0,
-                dw_mu_l=0.0, dw_p=0.0):
+                dw_mu_l=0.0, dw_p=0.0, shift="none", gauge_wave_dir=1, gd_eta=0.0, dw_mu_s=0.0,
+                k_driver_coeff=0.04):
     nx, ny, nz = (n, n, n) if np.isscalar(n) else n
     dx = box / nx if dx is None else dx
     return Params(nx, ny, nz, order, dx, dt_frac * dx, ko, a_adj, k_damp, normalize,
-                  LAPSE[lapse], MATTER[matter], lam, gd_c, dw_mu_l, dw_p)
+                  LAPSE[lapse], MATTER[matter], lam, gd_c, dw_mu_l, dw_p, SHIFT[shift], gauge_wave_dir,
+                  gd_eta, dw_mu_s, k_driver_coeff)
 
 
 class Sim:
@@ -234,6 +244,16 @@ def power_dump(field: np.ndarray) -> np.ndarray:
     return np.where(cnt > 0, f2 / np.maximum(cnt, 1), 0.0)
 
 
+def upwind(sim: Sim, f: np.ndarray, c: np.ndarray) -> np.ndarray:
+    """upwind_derivative (utils/math.h:1247-1252) of f along each axis with
+    the per-point speed c: [3, nx, ny, nz]."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    out = np.zeros((3,) + f.shape)
+    sim.L.or_upwind(sim.h, dptr(f), dptr(c), dptr(out))
+    return out
+
+
 def point_compute(params: Params, pt: Point) -> Point:
     lib().or_point_compute(C.byref(params), C.byref(pt))
     return pt
@@ -266,6 +286,7 @@ def ref_lib(order: int):
     L.ref_derivatives.argtypes = [dp, dp]
     L.ref_double_derivatives.argtypes = [dp, dp]
     L.ref_ko.argtypes = [dp, C.c_double, dp]
+    L.ref_upwind.argtypes = [dp, dp, dp]
     L.ref_reductions.argtypes = [dp, dp, C.c_double, dp]
     L.ref_static_src.argtypes = [dp, dp, C.c_double, C.c_double, dp]
     L.ref_rk4.argtypes = [dp, C.c_long, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]

@@ -56,6 +56,16 @@ void ref_double_derivatives(const double *f, double *out) {
     LOOP3(i, j, k) out[s * POINTS + NP_INDEX(i, j, k)] = double_derivative(i, j, k, P[s][0], P[s][1], a);
 }
 
+// upwind_derivative(i,j,k,d,f,c) with the speed c taken per point, d=1..3:
+// out[(d-1)*P + idx]
+void ref_upwind(const double *f, const double *c, double *out) {
+  arr_t a(NX, NY, NZ);
+  load(a, f);
+  idx_t i, j, k;
+  for (int d = 1; d <= 3; ++d)
+    LOOP3(i, j, k) out[(d - 1) * POINTS + NP_INDEX(i, j, k)] = upwind_derivative(i, j, k, d, a, c[NP_INDEX(i, j, k)]);
+}
+
 void ref_ko(const double *f, double coeff, double *out) {
   arr_t a(NX, NY, NZ);
   load(a, f);

@@ -3,7 +3,8 @@
 from their own sources (oracle/_ref, see oracle/Makefile target `ref`).
 
 Vectors (inputs are regenerated from the seeds recorded here):
-  * stencils: utils/math.h derivative / double_derivative / KO_dissipation_Q
+  * stencils: utils/math.h derivative / double_derivative / KO_dissipation_Q /
+    upwind_derivative
     on a seeded random field on the 12x10x14 grid the drivers are compiled
     for, orders 2/4/6/8 -> sha256 of the FP64 output bytes plus 16 sample
     values per array;
@@ -45,6 +46,12 @@ def field_input(shape, seed):
     return rng.standard_normal(shape)
 
 
+def upwind_speed(shape, order):
+    c = field_input(shape, SEED_FIELD + 200 + order)
+    c.reshape(-1)[::5] = 0.0
+    return np.ascontiguousarray(c)
+
+
 def main():
     out = {"grid": None, "seed_field": SEED_FIELD, "ko_coeff": KO_COEFF, "stencils": {}}
     for order in (2, 4, 6, 8):
@@ -60,6 +67,10 @@ def main():
         ent = {"derivative": {"sha256": digest(d1), "samples": samples(d1)},
                "double_derivative": {"sha256": digest(d2), "samples": samples(d2)},
                "ko": {"sha256": digest(ko), "samples": samples(ko)}}
+        # upwind_derivative with a signed per-point speed (zero at every 5th point)
+        c = upwind_speed((nx, ny, nz), order)
+        up = np.zeros((3, nx, ny, nz)); R.ref_upwind(po.dptr(f), po.dptr(c), po.dptr(up))
+        ent["upwind"] = {"sha256": digest(up), "samples": samples(up)}
         g = np.ascontiguousarray(0.01 * field_input((nx, ny, nz), SEED_FIELD + 100 + order))
         red = np.zeros(7)
         R.ref_reductions(po.dptr(f), po.dptr(g), 0.0123, po.dptr(red))

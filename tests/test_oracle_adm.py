@@ -105,6 +105,12 @@ class Data:
             G = G + self.dgam_amp * torch.stack([self.w(x, 15 + i) for i in range(3)])
         return G
 
+    def beta(self, x):  # shift vector beta^i
+        return torch.stack([0.6 * self.w(x, 15 + q) + 0.3 * self.w(x, 4 + q) for q in range(3)])
+
+    def expN(self, x):
+        return 0.3 * self.w(x, 19)
+
     # matter (physical, lower indices)
     def rho(self, x):
         return self.rho_F + 0.2 * self.w(x, 18)
@@ -146,8 +152,9 @@ def ricci(gfun, x):
     return R
 
 
-def adm_rhs(d, x):
-    """(d_t g_ij, d_t K_ij) of ADM with lapse alpha, zero shift."""
+def adm_rhs(d, x, shift=False):
+    """(d_t g_ij, d_t K_ij) of ADM with lapse alpha and zero shift, or with
+    shift beta^i the Lie derivatives L_beta g_ij and L_beta K_ij added."""
     g, K, al = d.g(x), d.Kij(x), d.alpha(x)
     gi = inv3(g)
     Gm = christoffel(d.g, x)
@@ -160,6 +167,15 @@ def adm_rhs(d, x):
     Sij = d.STF(x) + g * d.S(x) / 3.0
     dtg = -2 * al * K
     dtK = -DDal + al * (R - 2 * KK + trK * K) - 8 * PI * al * (Sij - 0.5 * g * (d.S(x) - d.rho(x)))
+    if shift:
+        b = d.beta(x)
+        db = jacfwd(d.beta)(x)  # [k][i] = d_i beta^k
+        for T, dT in ((g, jacfwd(d.g)(x)), (K, jacfwd(d.Kij)(x))):  # dT[a][b][k] = d_k T_ab
+            lie = torch.einsum("k,abk->ab", b, dT) + torch.einsum("kb,ka->ab", T, db) + torch.einsum("ak,kb->ab", T, db)
+            if T is g:
+                dtg = dtg + lie
+            else:
+                dtK = dtK + lie
     return dtg, dtK
 
 
@@ -180,7 +196,7 @@ def constraints(d, x):
     return H, M, R
 
 
-def oracle_point(po, params, d, x):
+def oracle_point(po, params, d, x, shift=False):
     pt = po.Point()
     gt, At = d.gt(x), d.At(x)
     for q, (a, b) in enumerate(SYM):
@@ -220,16 +236,41 @@ def oracle_point(po, params, d, x):
         pt.ddphi[s] = float(ddphi[k, l])
         pt.ddalpha[s] = float(ddal[k, l])
     pt.phi_F, pt.K_F, pt.rho_F, pt.S_F = d.phi_F, d.K_F, d.rho_F, d.S_F
+    if shift:
+        # shift inputs: beta^i, d_j beta^i, d_k d_l beta^i, d expN, and the
+        # advection term beta^k d_k f of every field (exact derivatives where
+        # the grid driver uses upwind stencils)
+        b = d.beta(x)
+        db = jacfwd(d.beta)(x)               # [i][j] = d_j beta^i
+        ddb = jacfwd(jacfwd(d.beta))(x)      # [i][k][l]
+        for i in range(3):
+            pt.xv[i] = float(b[i])
+            for j in range(3):
+                pt.dbeta[j][i] = float(db[i, j])
+            for s_, (k, l) in enumerate(SYM):
+                pt.ddbeta[i][s_] = float(ddb[i, k, l])
+        pt.xv[3] = float(d.expN(x))
+        dN = jacfwd(d.expN)(x)
+        for k in range(3):
+            pt.dexpN[k] = float(dN[k])
+        for q, (a, c) in enumerate(SYM):
+            pt.adv[q] = float(b @ dg[a, c])
+            pt.adv[7 + q] = float(b @ dA[a, c])
+        pt.adv[6] = float(b @ dphi)
+        pt.adv[13] = float(b @ dK)
+        for i in range(3):
+            pt.adv[14 + i] = float(b @ dG[i])
+        pt.adv[17] = float(b @ dal)
     po.point_compute(params, pt)
     return pt
 
 
-def expected(d, x, lapse="Static"):
+def expected(d, x, lapse="Static", shift=False):
     """BSSN right-hand sides from ADM + chain rule (+ constraint additions)."""
     phi = d.phi(x)
     e4 = torch.exp(4 * phi)
     al = d.alpha(x)
-    dtg, dtK = adm_rhs(d, x)
+    dtg, dtK = adm_rhs(d, x, shift)
     g, K = d.g(x), d.Kij(x)
     gi = inv3(g)
     trK = torch.einsum("ij,ij->", gi, K)
@@ -243,7 +284,7 @@ def expected(d, x, lapse="Static"):
     dt_trK = dt_trK_adm - al * H
     # d_t Gt^i = -d_j (d_t gt^{ij}) + 2 alpha e^{4 phi} M^i
     def dtgt_up(y):
-        dtg_y, _ = adm_rhs(d, y)
+        dtg_y, _ = adm_rhs(d, y, shift)
         g_y = d.g(y)
         gi_y = inv3(g_y)
         dtphi_y = torch.einsum("ij,ij->", gi_y, dtg_y) / 12.0
@@ -265,6 +306,9 @@ def expected(d, x, lapse="Static"):
     for i in range(3):
         out[14 + i] = float(dtGam[i])
     out[17] = 0.0 if lapse == "Static" else float(-al * trK)
+    if shift:
+        # the lapse is advected too (bssn.cc:822-831)
+        out[17] += float(d.beta(x) @ jacfwd(d.alpha)(x))
     return out, float(H), M, float(R)
 
 
@@ -366,3 +410,30 @@ def test_grid_rhs_converges_to_point_algebra(oracle_mod, order, expect):
         errs.append(e)
     ratio = errs[0] / errs[1]
     assert ratio > 0.6 * expect, (errs, ratio)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+@pytest.mark.parametrize("lapse", ["Static", "Harmonic"])
+def test_point_rhs_with_shift_matches_adm(oracle_mod, seed, lapse):
+    """The USE_BSSN_SHIFT build's right-hand side (bssn_macros.h:292-385,
+    bssn.cc:786-828): ADM with the Lie derivatives L_beta g_ij and L_beta K_ij,
+    carried through the same chain rule, gives every BSSN field's shift
+    terms (advection, d_j beta^i couplings, the second derivatives of beta in
+    d_t Gt^i and the divergence in d_t phi).  The expansion e-fold count
+    advances by -alpha K/3 plus its advection (bssn.cc:871-875)."""
+    d = Data(seed)
+    x = torch.tensor(np.random.default_rng(200 + seed).uniform(0, 1, 3))
+    params = oracle_mod.make_params(16, order=4, lapse=lapse, shift="Static")
+    pt = oracle_point(oracle_mod, params, d, x, shift=True)
+    want, H, _, _ = expected(d, x, lapse, shift=True)
+    got = np.array(pt.rhs[:])
+    scale = np.abs(want).max()
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-11 * scale)
+    # the shift terms are not negligible here
+    p0 = oracle_point(oracle_mod, oracle_mod.make_params(16, order=4, lapse=lapse), d, x)
+    assert np.abs(got - np.array(p0.rhs[:])).max() > 1e-3 * scale
+    trK = float(torch.einsum("ij,ij->", inv3(d.g(x)), d.Kij(x)))
+    dN = float(d.beta(x) @ jacfwd(d.expN)(x))
+    assert abs(pt.rhsx[3] - (dN - float(d.alpha(x)) * trK / 3)) < 1e-12 * max(1.0, abs(trK))
+    # Static shift: d_t beta^i = 0 (minus KO, zero here)
+    assert all(pt.rhsx[i] == 0.0 for i in range(3))

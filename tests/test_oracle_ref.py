@@ -180,3 +180,57 @@ def test_stencils_live_vs_reference(oracle_mod, order):
     R.ref_derivatives(oracle_mod.dptr(f), oracle_mod.dptr(d1))
     _, m1, _, _ = oracle_stencils(oracle_mod, order, f, 0.0)
     assert np.array_equal(d1, m1)
+
+
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+def test_upwind_bitwise_vs_reference(oracle_mod, order):
+    """upwind_derivative (utils/math.h:1247-1252: the lopsided derivative plus
+    its dissipation stencil, forward for a positive speed, backward
+    otherwise) with a signed per-point speed, bitwise; the lopsided stencils
+    are stubs returning 0 at orders 6 and 8 (math.h:539-549, 661-673)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(HERE, "golden", "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    shape = tuple(GOLD["grid"])
+    f = np.ascontiguousarray(field_input(shape, GOLD["seed_field"] + order))
+    c = mg.upwind_speed(shape, order)
+    s = oracle_mod.Sim(oracle_mod.make_params(shape, order=order))
+    up = oracle_mod.upwind(s, f, c)
+    assert digest(up) == GOLD["stencils"][str(order)]["upwind"]["sha256"]
+    assert (np.abs(up).max() > 0) == (order in (2, 4))
+    if oracle_mod.ref_available(order):
+        R = oracle_mod.ref_lib(order)
+        R.ref_set_steps(1.0 / shape[0], 0.1 / shape[0])
+        ref = np.zeros((3,) + shape)
+        R.ref_upwind(oracle_mod.dptr(f), oracle_mod.dptr(c), oracle_mod.dptr(ref))
+        assert np.array_equal(ref.view(np.int64), up.view(np.int64))
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_upwind_symbol(oracle_mod, order):
+    """On a Fourier mode e^{i k x}, the upwind derivative is c * sigma(k) e^{ikx}
+    with the lopsided stencil's symbol: order 4 forward
+    (e^{3ih}/12 - e^{2ih}/2 + 3e^{ih}/2 - 5/6 - e^{-ih}/4)/dx, order 2 the
+    one-sided second-order stencil minus dx/2 times the forward second
+    difference; the backward symbols are the complex conjugates times -1."""
+    n = 16
+    s = oracle_mod.Sim(oracle_mod.make_params((n, 4, 4), order=order))
+    dx = 1.0 / n
+    kk = 3
+    x = np.arange(n) * dx
+    h = 2 * np.pi * kk * dx
+    z = np.exp(1j * h)
+    if order == 4:
+        fw = (z**3 / 12 - z**2 / 2 + 1.5 * z - 5.0 / 6 - 1 / (4 * z)) / dx
+    else:
+        fw = (-0.5 * z**2 + 2 * z - 1.5) / dx - 0.5 * dx * (2 - 5 * z + 4 * z**2 - z**3) / dx**2
+    for part in (np.cos, np.sin):
+        f = np.ascontiguousarray(np.broadcast_to(part(2 * np.pi * kk * x)[:, None, None], (n, 4, 4)))
+        for sign in (1.0, -1.0):
+            c = np.full((n, 4, 4), 0.7 * sign)
+            up = oracle_mod.upwind(s, f, c)[0][:, 0, 0]
+            sym = fw if sign > 0 else -np.conj(fw)
+            e = np.exp(1j * 2 * np.pi * kk * x)
+            want = 0.7 * sign * (sym * e).real if part is np.cos else 0.7 * sign * (sym * e).imag
+            np.testing.assert_allclose(up, want, rtol=0, atol=1e-11 / dx)
