@@ -20,8 +20,17 @@ enum { MODE_STAGE = 0, MODE_CONS = 1, MODE_DUST = 2, MODE_DUST_CONS = 3 };
 enum Lapse {
   LAPSE_STATIC = 0, LAPSE_HARMONIC = 1, LAPSE_CONFORMAL_FLRW = 2, LAPSE_ANHARMONIC = 3, LAPSE_ONE_PLUS_LOG = 4,
   LAPSE_CONFORMAL_SYNC = 5, LAPSE_AWA_GAUGE_WAVE = 6, LAPSE_AWA_SHIFTED_WAVE = 7, LAPSE_DAMPED_WAVE = 8,
-  LAPSE_TEST_AIJ_DRIVER = 9, LAPSE_N = 10
+  LAPSE_TEST_AIJ_DRIVER = 9, LAPSE_TEST_K_DRIVER = 10, LAPSE_N = 11
 };
+// shift gauges (BSSNGaugeHandler.h:113-135); SHIFT_NONE: no shift registers
+enum Shift {
+  SHIFT_NONE = 0, SHIFT_STATIC = 1, SHIFT_GAMMA_DRIVER = 2, SHIFT_DAMPED_WAVE = 3, SHIFT_AWA_SHIFTED_WAVE = 4,
+  SHIFT_AIJ_DRIVER = 5, SHIFT_N = 6
+};
+// registers of the shift build: beta1..3, expN, auxB1..3 (bssn_macros.h:18-47)
+constexpr int NXF = 7;
+enum XField { XBETA1 = 0, XEXPN = 3, XAUXB1 = 4 };
+constexpr int n_xfields(int shift) { return shift == SHIFT_NONE ? 0 : shift == SHIFT_GAMMA_DRIVER ? NXF : XAUXB1; }
 // gauges that read K_avg of setExtraFieldData every stage
 __host__ __device__ constexpr bool lapse_needs_kavg(int l) {
   return l == LAPSE_CONFORMAL_FLRW || l == LAPSE_ANHARMONIC || l == LAPSE_ONE_PLUS_LOG;

@@ -491,6 +491,8 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   if (p.precision != CG_FP64 && p.precision != CG_FP32_SPLIT) return CG_ERR_ARG;
   if (p.matter != CG_MATTER_NONE && p.matter != CG_MATTER_STATIC && p.matter != CG_MATTER_DUST) return CG_ERR_ARG;
   if (p.lapse < 0 || p.lapse >= LAPSE_N) return CG_ERR_ARG;
+  if (p.shift < 0 || p.shift >= SHIFT_N) return CG_ERR_ARG;
+  if (p.shift != CG_SHIFT_NONE && p.matter == CG_MATTER_DUST) return CG_ERR_ARG;  // shift build: BSSN + static dust
   if (!(p.dx > 0) || !std::isfinite(p.dt)) return CG_ERR_ARG;
   cg_ctx* c = new cg_ctx();
   c->p = p;
@@ -537,6 +539,11 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   HIPCHK(c, hipMalloc((void**)&c->ext, 4 * sizeof(double)));
   HIPCHK(c, hipMemsetAsync(c->ext, 0, 4 * sizeof(double), c->stream));
   HIPCHK(c, hipMalloc((void**)&c->nan_cnt, sizeof(unsigned long long)));
+  c->nxf = n_xfields(p.shift);
+  if (c->nxf) {
+    HIPCHK(c, hipMalloc((void**)&c->xreg, (size_t)c->nxf * 4 * pad * sizeof(double)));
+    HIPCHK(c, hipMemsetAsync(c->xreg, 0, (size_t)c->nxf * 4 * pad * sizeof(double), c->stream));
+  }
   if (p.matter == CG_MATTER_DUST) {
     HIPCHK(c, hipMalloc(&c->dreg, (size_t)NDUST * 4 * pad * c->ts));
     HIPCHK(c, hipMemsetAsync(c->dreg, 0, (size_t)NDUST * 4 * pad * c->ts, c->stream));
@@ -566,6 +573,7 @@ int cg_destroy(cg_ctx* c) {
   if (c->dreg) (void)hipFree(c->dreg);
   if (c->flux) (void)hipFree(c->flux);
   if (c->rho_src) (void)hipFree(c->rho_src);
+  if (c->xreg) (void)hipFree(c->xreg);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CG_OK;
@@ -600,6 +608,15 @@ static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err, size
     return rr == 'p' ? c->sP : rr == 'f' ? (c->a_is_p ? c->sP : c->sF) : rr == 'a' ? (c->a_is_p ? c->sP : c->sA)
            : rr == 'c' ? c->sC : -1;
   };
+  static const char* kX[NXF] = {"beta1", "beta2", "beta3", "expN", "auxB1", "auxB2", "auxB3"};
+  for (int q = 0; q < c->nxf; ++q) {
+    if (base != kX[q]) continue;
+    const int slot = slot_of(r);
+    if (slot < 0) { *err = 1; return nullptr; }
+    if (for_write && c->a_is_p && (r == 'a' || r == 'f')) { *err = 2; return nullptr; }
+    *esz = 8;
+    return c->xreg + ((long)q * 4 + slot) * c->rstride;
+  }
   static const char* kDust[NDUST] = {"dust_D", "dust_S1", "dust_S2", "dust_S3"};
   for (int q = 0; q < NDUST && c->dreg; ++q) {
     if (base != kDust[q]) continue;
@@ -725,6 +742,13 @@ int cg_set_dma(cg_ctx* c, int on) {
 int cg_set_gauge_params(cg_ctx* c, double gd_c, double dw_mu_l, double dw_p) {
   if (!c) return CG_ERR_ARG;
   c->gd_c = gd_c; c->dw_mu_l = dw_mu_l; c->dw_p = dw_p;
+  return CG_OK;
+}
+
+int cg_set_shift_params(cg_ctx* c, int gauge_wave_dir, double gd_eta, double dw_mu_s, double k_driver_coeff) {
+  if (!c || gauge_wave_dir < 1 || gauge_wave_dir > 3) return CG_ERR_ARG;
+  c->gauge_wave_dir = gauge_wave_dir;
+  c->gd_eta = gd_eta; c->dw_mu_s = dw_mu_s; c->k_driver = k_driver_coeff;
   return CG_OK;
 }
 
@@ -909,6 +933,7 @@ int cg_step_init(cg_ctx* c) {
     c->dust_rk = 0;
     if (int rc = dust_reg_init(c, c->sP, c->sA, c->sF)) return rc;
   }
+  if (int rc = gauge_step_init(c)) return rc;
   return c->prec == CG_FP64 ? step_init_t<double>(c) : step_init_t<float>(c);
 }
 
@@ -929,7 +954,10 @@ int cg_rk_evolve(cg_ctx* c) {
     c->dust_lag_phiF = c->frw.g_phi;
     ++c->dust_rk;
   }
-  return launch(c, a);
+  if (int rc = launch(c, a)) return rc;
+  // the shift build's terms (and the lapses that read other fields'
+  // complete right-hand sides) on top of the stage kernel's _c
+  return gauge_pass(c) ? gauge_rhs(c) : CG_OK;
 }
 
 template <typename TS>
@@ -960,6 +988,7 @@ int cg_k_finalize(cg_ctx* c, int n) {
                          c->sP, c->sC, c->sF, c->npts, n, c->p.dt);
     HIPCHK(c, hipGetLastError());
   }
+  if (int rc = gauge_finalize(c, n)) return rc;
   int rc = c->prec == CG_FP64 ? finalize_t<double>(c, n, src) : finalize_t<float>(c, n, src);
   if (rc) return rc;
   std::swap(c->sA, c->sC);  // swap_a_c
@@ -969,6 +998,23 @@ int cg_k_finalize(cg_ctx* c, int n) {
 
 int cg_rk4_steps(cg_ctx* c, int nsteps) {
   if (!c || nsteps < 0) return CG_ERR_ARG;
+  if (gauge_pass(c)) {
+    // The shift build runs the reference's method sequence
+    // (sims/static.cc:74-170): stepInit, the NaN guard, then RKEvolve (stage
+    // kernel + gauge pass) and K{n}Finalize per stage.
+    for (int s = 0; s < nsteps; ++s) {
+      int rc;
+      if ((rc = cg_step_init(c))) return rc;
+      long long nn = 0;
+      if ((rc = cg_num_nans(c, "DIFFphi_a", &nn))) return rc;
+      if (nn) return fail(c, CG_ERR_NAN, "NAN detected in DIFFphi");
+      for (int n = 1; n <= 4; ++n) {
+        if ((rc = cg_rk_evolve(c))) return rc;
+        if ((rc = cg_k_finalize(c, n))) return rc;
+      }
+    }
+    return CG_OK;
+  }
   // fixed physical slots for the fused path: P=0, F=1, X=2, Y=3
   c->sP = 0; c->sF = 1;
   const int X = 2, Y = 3;

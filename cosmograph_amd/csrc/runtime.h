@@ -106,6 +106,13 @@ struct cg_ctx {
   // the autotuner; used where the variant has it and the grid fits
   bool dma_allowed = true, dma_pick = true;
   double gd_c = 1.0, dw_mu_l = 0.0, dw_p = 0.0;  // gauge parameters (BSSNGaugeHandler.h:139-149)
+  // shift build (cg_params::shift): FP64 registers [nxf][4][pad] of beta1..3,
+  // expN (and auxB1..3 with the gamma driver), in the slots of the BSSN
+  // fields; and the remaining gauge parameters
+  double* xreg = nullptr;
+  int nxf = 0;
+  int gauge_wave_dir = 1;
+  double gd_eta = 0.0, dw_mu_s = 0.0, k_driver = 0.04;
   bool dust_bg = false;                   // dust sources as perturbations of the FRW fluids (cg_set_dust_background)
   bool keep_accumulator = false;          // fused steps with the reference's _f accumulator (cg_set_fused_accumulator)
   bool incr_pick = true;                  // the autotuner's choice between the two RK4 forms
@@ -178,6 +185,12 @@ StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int
 int ext_resolve(cg_ctx* c);
 // device array of a field-map name for reading (err 1: unknown, 3: filled by cg_constraints)
 const void* field_ptr(cg_ctx* c, const char* name, size_t* esz, int* err);
+// the gauge pass (gauge.hip): shift terms, shift registers and the lapses
+// that read the complete right-hand side of other fields
+inline bool gauge_pass(const cg_ctx* c) { return c->nxf > 0 || c->p.lapse == LAPSE_TEST_K_DRIVER; }
+int gauge_rhs(cg_ctx* c);                 // after the stage kernel of cg_rk_evolve: adds to _c, fills the shift _c
+int gauge_step_init(cg_ctx* c);           // RK4Register::stepInit of the shift registers
+int gauge_finalize(cg_ctx* c, int n);     // RK4Register::K{n}Finalize of the shift registers (no swap)
 // KO at orders 2/8 with metric normalisation materialises stepInit's _a
 inline bool ko_norm_path(const cg_ctx* c) {
   return c->p.ko_coeff != 0.0 && c->p.normalize_metric && (c->p.stencil_order == 2 || c->p.stencil_order == 8);

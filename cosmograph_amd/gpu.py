@@ -23,7 +23,12 @@ FIELDS = ["DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIFFgamma22", "DIFFgamma
 PRECISION = {"fp64": 0, "fp32_split": 1}
 MATTER = {"none": 0, "vacuum": 0, "static": 1, "dust": 2}
 LAPSE = {"": 0, "Static": 0, "Harmonic": 1, "ConformalFLRW": 2, "Anharmonic": 3, "OnePlusLog": 4,
-         "ConformalSync": 5, "AwAGaugeWave": 6, "AwAShiftedWave": 7, "DampedWave": 8, "TestAijDriverLapse": 9}
+         "ConformalSync": 5, "AwAGaugeWave": 6, "AwAShiftedWave": 7, "DampedWave": 8, "TestAijDriverLapse": 9,
+         "TestKDriverLapse": 10}
+# shift gauges (BSSNGaugeHandler.h:113-135); "none" is the reference's default
+# build without USE_BSSN_SHIFT, any other name the shift build with that gauge
+SHIFT = {"none": 0, "Static": 1, "GammaDriver": 2, "DampedWave": 3, "AwAShiftedWave": 4, "AijDriverShift": 5}
+X_FIELDS = ["beta1", "beta2", "beta3", "expN", "auxB1", "auxB2", "auxB3"]
 ERR_NAN = -10
 
 
@@ -40,7 +45,7 @@ class _Params(C.Structure):
                 ("k_damping_amp", C.c_double), ("normalize_metric", C.c_int),
                 ("lapse", C.c_int), ("matter", C.c_int), ("lambda_", C.c_double),
                 ("device", C.c_int), ("tile_i", C.c_int), ("tile_j", C.c_int),
-                ("tile_k", C.c_int)]
+                ("tile_k", C.c_int), ("shift", C.c_int)]
 
 
 @dataclass
@@ -64,6 +69,11 @@ class GpuParams:
     gd_c: float = 1.0           # gauge parameters (BSSNGaugeHandler.h:139-149)
     dw_mu_l: float = 0.0
     dw_p: float = 0.0
+    shift: str = "none"         # shift gauge of the shift build ("none": default build)
+    gauge_wave_dir: int = 1     # BSSNGaugeHandler.h:139-149, bssn.cc:18
+    gd_eta: float = 0.0
+    dw_mu_s: float = 0.0
+    k_driver_coeff: float = 0.04
 
     def shape(self):
         return (self.n, self.n, self.n) if np.isscalar(self.n) else tuple(self.n)
@@ -75,7 +85,7 @@ class GpuParams:
         return _Params(nx, ny, nz, self.order, PRECISION[self.precision], dx, self.dt_frac * dx,
                        self.ko_coeff, self.a_adj_amp, self.k_damping_amp, self.normalize_metric,
                        LAPSE[self.lapse], MATTER[self.matter], self.lambda_, self.device,
-                       ti, tj, tk)
+                       ti, tj, tk, SHIFT[self.shift])
 
 
 _lib = None
@@ -106,6 +116,7 @@ def load_library():
         "cg_frw_get": ([vp, dp], C.c_int),
         "cg_set_lambda": ([vp, C.c_double], C.c_int),
         "cg_set_gauge_params": ([vp, C.c_double, C.c_double, C.c_double], C.c_int),
+        "cg_set_shift_params": ([vp, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
         "cg_step_init": ([vp], C.c_int),
         "cg_rk_evolve": ([vp], C.c_int),
         "cg_k_finalize": ([vp, C.c_int], C.c_int),
@@ -174,6 +185,8 @@ class BSSN:
             self.h = None
             raise CosmoGpuError(rc, msg)
         self._chk(self.L.cg_set_gauge_params(self.h, params.gd_c, params.dw_mu_l, params.dw_p))
+        self._chk(self.L.cg_set_shift_params(self.h, params.gauge_wave_dir, params.gd_eta, params.dw_mu_s,
+                                             params.k_driver_coeff))
 
     def __del__(self):
         if getattr(self, "h", None) and getattr(self, "_owner", None) is None:

@@ -117,7 +117,8 @@ cg_params params_from_config(const ConfigParser& cfg) {
       {"ConformalFLRW", CG_LAPSE_CONFORMAL_FLRW}, {"Anharmonic", CG_LAPSE_ANHARMONIC},
       {"OnePlusLog", CG_LAPSE_ONE_PLUS_LOG}, {"ConformalSync", CG_LAPSE_CONFORMAL_SYNC},
       {"AwAGaugeWave", CG_LAPSE_AWA_GAUGE_WAVE}, {"AwAShiftedWave", CG_LAPSE_AWA_SHIFTED_WAVE},
-      {"DampedWave", CG_LAPSE_DAMPED_WAVE}, {"TestAijDriverLapse", CG_LAPSE_TEST_AIJ_DRIVER}};
+      {"DampedWave", CG_LAPSE_DAMPED_WAVE}, {"TestAijDriverLapse", CG_LAPSE_TEST_AIJ_DRIVER},
+      {"TestKDriverLapse", CG_LAPSE_TEST_K_DRIVER}};
   if (kLapse.count(lapse)) {
     p.lapse = kLapse.at(lapse);
   } else {
@@ -134,10 +135,27 @@ cg_params params_from_config(const ConfigParser& cfg) {
     throw -1;
   }
   p.device = std::stoi(cfg("gpu_device", "0"));
+  // The shift (BSSNGaugeHandler.h:176-212).  The reference selects the
+  // USE_BSSN_SHIFT build at compile time; here the key `use_bssn_shift`
+  // (default 0) stands for it, and without it a non-Static shift is the
+  // reference's error.  The gamma driver is the USE_GAMMA_DRIVER build.
+  const std::string shift = cfg("shift", "Static");
+  static const std::map<std::string, int> kShift = {
+      {"Static", CG_SHIFT_STATIC}, {"GammaDriver", CG_SHIFT_GAMMA_DRIVER}, {"DampedWave", CG_SHIFT_DAMPED_WAVE},
+      {"AwAShiftedWave", CG_SHIFT_AWA_SHIFTED_WAVE}, {"AijDriverShift", CG_SHIFT_AIJ_DRIVER}};
+  const bool shift_build = std::stoi(cfg("use_bssn_shift", "0")) != 0;
+  if (!shift_build && shift != "Static") {
+    std::cerr << "Code must be compiled with shift enabled to use non-Static shift!";
+    throw -1;
+  }
+  if (!kShift.count(shift)) {
+    std::cout << "Error: Shift gauge not found: `" << shift << "`!\n";
+    throw -1;
+  }
+  p.shift = shift_build ? kShift.at(shift) : CG_SHIFT_NONE;
   // features of the reference that this path does not carry
-  if (std::stod(cfg("rescale_metric", "1.0")) != 1.0 || std::stod(cfg("gd_eta", "0.0")) != 0.0 ||
-      cfg("shift", "") != "") {
-    std::cerr << "Error - rescale_metric, gd_eta and shift are not available on the GPU path.\n";
+  if (std::stod(cfg("rescale_metric", "1.0")) != 1.0) {
+    std::cerr << "Error - rescale_metric is not available on the GPU path.\n";
     throw -1;
   }
   return p;
@@ -148,6 +166,8 @@ BSSN::BSSN(const ConfigParser& cfg) : BSSN(params_from_config(cfg)) {
   // BSSNGaugeHandler::_initDefaultParameters (BSSNGaugeHandler.h:139-149)
   check(cg_set_gauge_params(ctx, std::stod(cfg("gd_c", "1.0")), std::stod(cfg("dw_mu_l", "0.0")),
                             std::stod(cfg("dw_p", "0.0"))));
+  check(cg_set_shift_params(ctx, std::stoi(cfg("gauge_wave_dir", "1")), std::stod(cfg("gd_eta", "0.0")),
+                            std::stod(cfg("dw_mu_s", "0.0")), std::stod(cfg("k_driver_coeff", "0.04"))));
 }
 
 BSSN::BSSN(const cg_params& pin) : p(pin) {

@@ -30,11 +30,25 @@ typedef struct cg_ctx cg_ctx;
 enum { CG_OK = 0, CG_ERR_ARG = -1, CG_ERR_HIP = -2, CG_ERR_STATE = -3, CG_ERR_NAN = -10 };
 enum { CG_FP64 = 0, CG_FP32_SPLIT = 1 };
 enum { CG_MATTER_NONE = 0, CG_MATTER_STATIC = 1, CG_MATTER_DUST = 2 };
-/* the shift-free lapse gauges of BSSNGaugeHandler (BSSNGaugeHandler.h:96-111) */
+/* the lapse gauges of BSSNGaugeHandler (BSSNGaugeHandler.h:96-111) */
 enum {
   CG_LAPSE_STATIC = 0, CG_LAPSE_HARMONIC = 1, CG_LAPSE_CONFORMAL_FLRW = 2, CG_LAPSE_ANHARMONIC = 3,
   CG_LAPSE_ONE_PLUS_LOG = 4, CG_LAPSE_CONFORMAL_SYNC = 5, CG_LAPSE_AWA_GAUGE_WAVE = 6,
-  CG_LAPSE_AWA_SHIFTED_WAVE = 7, CG_LAPSE_DAMPED_WAVE = 8, CG_LAPSE_TEST_AIJ_DRIVER = 9
+  CG_LAPSE_AWA_SHIFTED_WAVE = 7, CG_LAPSE_DAMPED_WAVE = 8, CG_LAPSE_TEST_AIJ_DRIVER = 9,
+  CG_LAPSE_TEST_K_DRIVER = 10
+};
+/* The shift (BSSNGaugeHandler.h:113-135).  CG_SHIFT_NONE is the reference's
+ * default build (USE_BSSN_SHIFT false, cosmo_macros.h:43-44: beta = 0, no
+ * shift registers).  Any other value is the USE_BSSN_SHIFT build with that
+ * shift gauge: registers "beta1..3_<r>" and "expN_<r>" are added, and every
+ * field is advected with the upwind stencils of utils/math.h:1220-1252
+ * (identically zero at orders 6 and 8, where the reference's lopsided
+ * stencils are stubs).  CG_SHIFT_GAMMA_DRIVER is also the USE_GAMMA_DRIVER
+ * build (registers "auxB1..3_<r>", bssn_macros.h:38-47).  Not with the dust
+ * fluid, nor in a batch. */
+enum {
+  CG_SHIFT_NONE = 0, CG_SHIFT_STATIC = 1, CG_SHIFT_GAMMA_DRIVER = 2, CG_SHIFT_DAMPED_WAVE = 3,
+  CG_SHIFT_AWA_SHIFTED_WAVE = 4, CG_SHIFT_AIJ_DRIVER = 5
 };
 
 /* Construction parameters.  Mirrors the compile-time macros and config keys
@@ -57,6 +71,7 @@ typedef struct {
   double lambda;
   int device;             /* HIP device ordinal */
   int tile_i, tile_j, tile_k; /* 0 = autotuned/default */
+  int shift;              /* CG_SHIFT_* (config key `shift`, BSSNGaugeHandler.h:176) */
 } cg_params;
 
 /* BSSN::BSSN(ConfigParser*, Fourier*) (components/bssn/bssn.cc:13-44):
@@ -84,6 +99,11 @@ int cg_set_lambda(cg_ctx* ctx, double lambda);
  * (BSSNGaugeHandler.h:139-149): gd_c (OnePlusLog, default 1), dw_mu_l and
  * dw_p (DampedWave, default 0). */
 int cg_set_gauge_params(cg_ctx* ctx, double gd_c, double dw_mu_l, double dw_p);
+/* The remaining gauge parameters: gauge_wave_dir (AwAShiftedWave shift,
+ * default 1), gd_eta (Gamma driver, bssn.cc:18, default 0), dw_mu_s
+ * (DampedWave shift, default 0) and k_driver_coeff (TestKDriverLapse and
+ * AijDriverShift, default 0.04), BSSNGaugeHandler.h:139-149. */
+int cg_set_shift_params(cg_ctx* ctx, int gauge_wave_dir, double gd_eta, double dw_mu_s, double k_driver_coeff);
 
 /* Method-level drop-ins (unfused; each streams the grid once):
  * BSSN::stepInit (bssn.cc:219-226), BSSN::RKEvolve with the matter source

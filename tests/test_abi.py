@@ -53,6 +53,8 @@ def test_python_front_end_binds_every_symbol():
     {"n": (16, 16, 0), "dx": 1.0 / 16}, {"n": (2048, 1024, 1024)},
     # 32-bit byte offsets of the FP64 registers: 1024*1024*520 points > 2^29 (split precision too)
     {"n": (1024, 1024, 520), "precision": "fp32_split"}, {"n": (1024, 1024, 513)}, {"order": 3}, {"precision": "bf16"}, {"lapse": "GeneralizedNewton"}, {"dt_frac": float("nan")},
+    # the shift build carries BSSN + static dust only
+    {"shift": "Static", "matter": "dust"}, {"shift": "Shiftless"},
 ])
 def test_create_rejects_bad_parameters(bad):
     import cosmograph_amd as cg
