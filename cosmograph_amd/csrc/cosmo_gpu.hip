@@ -574,6 +574,7 @@ int cg_destroy(cg_ctx* c) {
   if (c->flux) (void)hipFree(c->flux);
   if (c->rho_src) (void)hipFree(c->rho_src);
   if (c->xreg) (void)hipFree(c->xreg);
+  bardeen_free(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CG_OK;
@@ -595,6 +596,14 @@ static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err, size
     // (initStaticStep's clearSrc + addBSSNSrc, sims/static.cc:74-82)
     if (!c->cons_valid) { *err = 3; return nullptr; }
     return c->cons + (long)(s == "DIFFr_a" ? CONS_RHO : CONS_SRCS) * c->rstride;
+  }
+  if (s.compare(0, 8, "Bardeen_") == 0) {  // Bardeen::Bardeen's field-map entries (bardeen.h:127-156)
+    *esz = 8;
+    int e4 = 0;
+    const double* p = bardeen_field(c, s.substr(8), &e4);
+    if (for_write) { *err = 1; return nullptr; }
+    *err = e4;
+    return (void*)p;
   }
   if (s == "ricci_a" || s == "AijAij_a") {
     if (for_write || !c->cons_valid) { *err = 3; return nullptr; }
@@ -678,8 +687,9 @@ int cg_field_download(cg_ctx* c, const char* name, double* host, size_t n) {
   int e;
   size_t esz;
   void* d = resolve(c, name, false, &e, &esz);
-  if (!d) return fail(c, e == 3 ? CG_ERR_STATE : CG_ERR_ARG,
-                      e == 3 ? std::string(name) + " is filled by cg_constraints" : std::string("unknown field ") + name);
+  if (!d) return fail(c, e >= 3 ? CG_ERR_STATE : CG_ERR_ARG,
+                      e == 3 ? std::string(name) + " is filled by cg_constraints"
+                      : e == 4 ? std::string(name) + " is filled by cg_bardeen" : std::string("unknown field ") + name);
   if (esz == 8) {
     HIPCHK(c, hipMemcpyAsync(host, d, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -957,8 +967,17 @@ int cg_rk_evolve(cg_ctx* c) {
   if (int rc = launch(c, a)) return rc;
   // the shift build's terms (and the lapses that read other fields'
   // complete right-hand sides) on top of the stage kernel's _c
-  return gauge_pass(c) ? gauge_rhs(c) : CG_OK;
+  return gauge_pass(c) ? gauge_rhs(c, c->sA, c->sC) : CG_OK;
 }
+
+namespace cg {
+int rhs_at(cg_ctx* c, int rin, int rout) {
+  StageArgs a = make_args(c, 0, rin, c->sP, c->sF, rout, 0);
+  if (c->dreg && dust_flux(c, rin, 0)) return CG_ERR_HIP;  // sources from this state itself (lag_mode 0)
+  if (int rc = launch(c, a)) return rc;
+  return gauge_pass(c) ? gauge_rhs(c, rin, rout) : CG_OK;
+}
+}  // namespace cg
 
 template <typename TS>
 static int finalize_t(cg_ctx* c, int n, const SrcArgs& src) {
@@ -1176,7 +1195,7 @@ int cg_kernel_info(const cg_ctx* cc, int* vgprs, int* scratch, int* lds, int* ti
 // fused step.  The momentum constraint differentiates A_ij, so the
 // normalised register is materialised (into the spare stage register and the
 // dead _f) rather than normalised on the fly.  Allocates nothing.
-static int cons_input(cg_ctx* c, int* rin_out) {
+int cg::cons_input(cg_ctx* c, int* rin_out) {
   // the pending K4Finalize averages read the dust source this pass rewrites
   if (int rc = ext_resolve(c)) return rc;
   if (c->p.nx <= 4 + c->R - 2 || c->p.ny <= 8 + c->R - 2 || c->p.nz <= 8 + c->R - 2)

@@ -337,14 +337,14 @@ int launch_gauge_order(int order, const GaugeArgs& g, hipStream_t s) {
 
 }  // namespace
 
-int gauge_rhs(cg_ctx* c) {
+int gauge_rhs(cg_ctx* c, int s_in, int s_out) {
   GaugeArgs g{};
   g.met = c->met;
   g.regs = c->regs;
   g.xreg = c->xreg;
   g.rstride = c->rstride;
-  g.s_in = c->sA;
-  g.s_out = c->sC;
+  g.s_in = s_in;
+  g.s_out = s_out;
   g.nx = c->p.nx; g.ny = c->p.ny; g.nz = c->p.nz;
   g.ntx = (c->p.nx + GTI - 1) / GTI; g.nty = (c->p.ny + GTJ - 1) / GTJ; g.ntz = (c->p.nz + GTK - 1) / GTK;
   g.inv_dx = 1.0 / c->p.dx;

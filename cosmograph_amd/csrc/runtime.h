@@ -89,6 +89,10 @@ struct SrcArgs {
 
 }  // namespace cg
 
+namespace cg {
+struct BrdWork;
+}
+
 // ------------------------------------------------------------ context --
 struct cg_ctx {
   using Frw = cg::Frw;
@@ -111,6 +115,7 @@ struct cg_ctx {
   // fields; and the remaining gauge parameters
   double* xreg = nullptr;
   int nxf = 0;
+  cg::BrdWork* brd = nullptr;             // Bardeen potentials workspace (cg_bardeen, lazy)
   int gauge_wave_dir = 1;
   double gd_eta = 0.0, dw_mu_s = 0.0, k_driver = 0.04;
   bool dust_bg = false;                   // dust sources as perturbations of the FRW fluids (cg_set_dust_background)
@@ -188,9 +193,18 @@ const void* field_ptr(cg_ctx* c, const char* name, size_t* esz, int* err);
 // the gauge pass (gauge.hip): shift terms, shift registers and the lapses
 // that read the complete right-hand side of other fields
 inline bool gauge_pass(const cg_ctx* c) { return c->nxf > 0 || c->p.lapse == LAPSE_TEST_K_DRIVER; }
-int gauge_rhs(cg_ctx* c);                 // after the stage kernel of cg_rk_evolve: adds to _c, fills the shift _c
+int gauge_rhs(cg_ctx* c, int s_in, int s_out);  // after the stage kernel's RHS into s_out: adds to it, fills the shift s_out
 int gauge_step_init(cg_ctx* c);           // RK4Register::stepInit of the shift registers
 int gauge_finalize(cg_ctx* c, int n);     // RK4Register::K{n}Finalize of the shift registers (no swap)
+// the register the output step's passes read (stepInit's _a, materialised
+// after a fused step); resolves the pending averages first
+int cons_input(cg_ctx* c, int* rin_out);
+// BSSN::RKEvolve at the state in slot rin into slot rout (matter sources of
+// that state, the gauge pass of the shift build)
+int rhs_at(cg_ctx* c, int rin, int rout);
+void bardeen_free(cg_ctx* c);
+// a "Bardeen_<name>" array of the last cg_bardeen (err 1: unknown name, 4: not computed)
+const double* bardeen_field(const cg_ctx* c, const std::string& name, int* err);
 // KO at orders 2/8 with metric normalisation materialises stepInit's _a
 inline bool ko_norm_path(const cg_ctx* c) {
   return c->p.ko_coeff != 0.0 && c->p.normalize_metric && (c->p.stencil_order == 2 || c->p.stencil_order == 8);

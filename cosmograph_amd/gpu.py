@@ -132,6 +132,7 @@ def load_library():
         "cg_autotune": ([vp, C.c_int, C.c_char_p, C.c_size_t], C.c_int),
         "cg_constraints": ([vp, dp, dp, dp, dp, dp], C.c_int),
         "cg_constraints_noalloc": ([vp, dp, dp, dp, dp, dp], C.c_int),
+        "cg_bardeen": ([vp, C.c_double, C.c_int, C.c_double, dp], C.c_int),
         "cg_constraint_strip": ([vp, C.c_int, C.c_long, C.c_long, dp, dp, C.c_size_t], C.c_int),
         "cg_field_strip": ([vp, C.c_char_p, C.c_int, C.c_long, C.c_long, dp, C.c_size_t], C.c_int),
         "cg_field_stats": ([vp, C.c_char_p, dp], C.c_int),
@@ -256,6 +257,14 @@ class BSSN:
         fn = self.L.cg_constraints_noalloc if noalloc else self.L.cg_constraints
         self._chk(fn(self.h, H, M, G, A, S))
         return {"H": list(H), "M": list(M), "G": list(G), "A": list(A), "S": list(S)}
+
+    def bardeen(self, t: float = 0.0, use_ml_scale_factor: bool = True, omega_L: float = 0.0) -> np.ndarray:
+        """Bardeen::setPotentials at simulation time t; returns the 17 SVT
+        violations (Bardeen::getSVTViolations).  The potentials are then
+        readable as "Bardeen_Phi", "Bardeen_Psi", ... through download()."""
+        out = np.zeros(17)
+        self._chk(self.L.cg_bardeen(self.h, float(t), int(bool(use_ml_scale_factor)), float(omega_L), _dptr(out)))
+        return out
 
     def set1DConstraintOutput(self, axis: int = 1, n1: int = 0, n2: int = 0):
         """BSSN::set1DConstraintOutput: (H, M_axis) at the points (x, n1, n2),

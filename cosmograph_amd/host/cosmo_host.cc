@@ -278,6 +278,17 @@ void CosmoSim::init() {
   bssnSim = new BSSN(cfg);
   lambda = new Lambda();
   dt = bssnSim->params().dt;
+  if (std::stoi(cfg("use_bardeen", "0"))) {
+    // Bardeen::useMLScaleFactor (bardeen.h:160-169), set up in simInit (sims/sim.cc:82-95)
+    const double OL = std::stod(cfg("Omega_L", "0.0"));
+    if (OL < 0 || OL > 1) {
+      std::cout << "Invalid Omega_Lambda specified." << std::endl;
+      throw -1;
+    }
+    if (std::stoi(cfg("use_ML_scale_factor", "1")))
+      iodata->log("Using Matter+Lambda FLRW scale factor for Bardeen calculations with Omega_L = " + cfg("Omega_L", "0.0") +
+                  ".");
+  }
   iodata->log("Initialized " + cfg("simulation_type", "static") + " simulation on the GPU: " +
               std::to_string(bssnSim->params().nx) + "x" + std::to_string(bssnSim->params().ny) + "x" +
               std::to_string(bssnSim->params().nz) + ", stencil order " +
@@ -391,6 +402,24 @@ void CosmoSim::outputStep() {
   // of this state come from the constraint pass
   if (strips || slices2 || slices3 || stats || power || cons || cons_snap) bssnSim->setConstraintCalcs(H, M, G, A, S);
   const std::string step_str = std::to_string(step);
+  // Bardeen::setPotentials at the end of prepBSSNOutput (sims/sim.cc:227-228;
+  // "Bardeen_<name>" arrays join the field map), then io_svt_violation
+  // (IO/io.cc:1116-1131) every SVT_constraint_interval steps
+  if (std::stoi(cfg("use_bardeen", "0"))) {
+    double viols[17];
+    const int rc = cg_bardeen(bssnSim->context(), t, std::stoi(cfg("use_ML_scale_factor", "1")),
+                              std::stod(cfg("Omega_L", "0.0")), viols);
+    if (rc == CG_ERR_STATE) {
+      iodata->log(cg_last_error(bssnSim->context()));  // the reference reports and skips (bardeen.cc:26-31)
+    } else if (rc) {
+      std::cerr << "GPU error: " << cg_last_error(bssnSim->context()) << "\n";
+      throw -1;
+    } else {
+      const long svt = std::stol(cfg("SVT_constraint_interval", "0"));
+      if (svt > 0 && step % svt == 0)
+        for (int q = 0; q < 17; ++q) dump_value(iodata, viols[q], "SVT_violations.dat.gz", q < 16 ? "\t" : "\n");
+    }
+  }
 
   // io_bssn_fields_snapshot (IO/io.cc:106-158): config keys IO_<d>D_<name>
   auto selected = [&](const char* dim) {

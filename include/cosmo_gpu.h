@@ -132,6 +132,23 @@ int cg_rk4_steps(cg_ctx* ctx, int nsteps);
  * stage's source). */
 int cg_constraints(cg_ctx* ctx, double H[8], double M[8], double G[7], double A[7], double S[7]);
 
+/* Bardeen::setPotentials + getSVTViolations (components/bssn/bardeen.cc:22-467)
+ * on the state cg_constraints reads (prepBSSNOutput, sims/sim.cc:214-231):
+ * the BSSN right-hand side there (left in the _c registers, as RKEvolve
+ * leaves it), the metric perturbation and its time derivatives, the SVT
+ * potentials through inverse Laplacians, the Bardeen potentials, and the 17
+ * linearised-constraint violations io_svt_violation writes (IO/io.cc:1116-1131)
+ * into viols.  Afterwards "Bardeen_Phi", "Bardeen_Psi", "Bardeen_A",
+ * "Bardeen_dt_A", "Bardeen_d2t_A", "Bardeen_B", "Bardeen_dt_B",
+ * "Bardeen_d2t_B", "Bardeen_E", "Bardeen_F", "Bardeen_G1..3",
+ * "Bardeen_C1..3", "Bardeen_dt_C1..3", "Bardeen_Vmag" and "Bardeen_D11..33"
+ * are readable with cg_field_download (bardeen.h:127-156).  elapsed_t is the
+ * simulation time; use_ml_scale_factor and omega_L are the config keys
+ * use_ML_scale_factor and Omega_L (sims/sim.cc:82-95).  CG_ERR_STATE with
+ * an FRW reference in use (K or phi non-zero), where the reference declines
+ * to compute them (bardeen.cc:26-31); nz must be even. */
+int cg_bardeen(cg_ctx* ctx, double elapsed_t, int use_ml_scale_factor, double omega_L, double viols[17]);
+
 /* cg_constraints without any allocation.  No counterpart in the reference:
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  The

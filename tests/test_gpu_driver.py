@@ -166,3 +166,61 @@ IO_powerspec_interval = 2
     assert len(spec) == len(want_spec)
     for got, w in zip(spec, want_spec):
         np.testing.assert_allclose(got, w, rtol=2e-5, atol=1e-12 * max(w.max(), 1e-300))
+
+
+def test_driver_bardeen_svt_output(oracle_mod, tmp_path):
+    """use_bardeen in a dust run (no FRW reference, as the reference requires):
+    Bardeen::setPotentials at every output step, SVT_violations.dat.gz every
+    SVT_constraint_interval steps (17 tab-separated values a row,
+    IO/io.cc:1116-1131), and the Bardeen_Phi field in the 3D dumps."""
+    import bardeen as bd
+    from cosmograph_amd.ics import save_ics
+    n, steps = 20, 1
+    _say(f"case driver bardeen n={n} steps={steps}")
+    st = smooth_random_state((n, n, n), seed=73)
+    st["DIFFK"] += -0.05
+    fields = {f + "_p": st[f] for f in FIELDS}
+    fields["dust_D_p"] = 0.05 + 0.1 * st["DIFFD"]
+    fields["dust_S2_p"] = 0.3 * st["DIFFK"]
+    frw = {"phi": 0.0, "K": 0.0, "fluids": []}
+    save_ics(str(tmp_path / "ic"), fields, frw, 0.0)
+    out = tmp_path / "out"
+    cfg = tmp_path / "config.txt"
+    cfg.write_text(f"""N = {n}
+stencil_order = 4
+H_LEN_FRAC = 1
+simulation_type = dust
+lapse = Harmonic
+steps = {steps}
+ic_dir = {tmp_path / 'ic'}
+output_dir = {out}
+use_bardeen = 1
+Omega_L = 0.2
+SVT_constraint_interval = 1
+IO_3D_grid_interval = 1
+IO_3D_Bardeen_Phi = 1
+""")
+    run_driver(cfg)
+    got = rows(out / "SVT_violations.dat.gz")
+    assert len(got) == steps + 1 and all(len(r_) == 17 for r_ in got)
+    o = oracle_mod.Sim(oracle_mod.make_params(n, order=4, matter="dust", lapse="Harmonic"))
+    for name, arr in fields.items():
+        o.field(name)[:] = arr
+    L = oracle_mod.lib()
+    dt = o.p.dt
+    for s in range(steps + 1):
+        L.or_bssn_step_init(o.h)
+        L.or_dust_step_init(o.h)
+        L.or_bssn_clear_src(o.h)
+        L.or_dust_add_src(o.h)
+        L.or_lambda_add_src(o.h)
+        o.RKEvolve()
+        a_reg = {f: o.field(f + "_a").copy() for f in FIELDS}
+        c_reg = {f: o.field(f + "_c").copy() for f in FIELDS}
+        wf, wv, _ = bd.set_potentials(a_reg, c_reg, s * dt, 4, 1.0 / n, 1.0, use_ml=True, omega_L=0.2)
+        for q in range(17):
+            # %.15g text: 15 significant digits
+            assert abs(got[s][q] - wv[q]) <= 1e-9 * abs(wv[q]) + 1e-300, (s, q, got[s][q], wv[q])
+        phi3d = np.fromfile(out / f"3D_Bardeen_Phi{s}.3d_grid.f64", dtype="<f8").reshape(n, n, n)
+        assert rel_l2(phi3d, wf["Phi"]) <= 1e-10
+        o.run_steps(1)
