@@ -255,6 +255,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="RK4 steps of the CPU-baseline sample")
     ap.add_argument("--detail-file", default=None, help="also write the line and the detail tables to this file")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-fp32-all", action="store_true", help="skip the all-FP32 storage leg")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="batched-ensemble leg (0 = skip)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -295,17 +296,23 @@ def main():
     rate32 = pts * args.steps / t32
 
     res = {}
+    detail_extra = {}
     if not args.skip_fp64:
         log("[bench] FP64 drop-in")
         g64 = make_ctx(cg, n, order, "fp64", modes, args.rms)
         t64, k64_ms, _, tune64 = time_steps(g64, args.warmup, args.steps, not args.no_autotune)
         rate64 = pts * args.steps / t64
-        errs = {}
-        for f in FIELDS:
-            a, b = g32.download(f + "_p"), g64.download(f + "_p")
-            nb = float(np.sqrt(np.sum(b * b)))
-            d = float(np.sqrt(np.sum((a - b) ** 2)))
-            errs[f] = d / nb if nb > 0 else d
+        f64 = {f: g64.download(f + "_p") for f in FIELDS}
+
+        def rel_errs(g):
+            out = {}
+            for f in FIELDS:
+                a, b = g.download(f + "_p"), f64[f]
+                nb = float(np.sqrt(np.sum(b * b)))
+                d = float(np.sqrt(np.sum((a - b) ** 2)))
+                out[f] = d / nb if nb > 0 else d
+            return out
+        errs = rel_errs(g32)
         # Hamiltonian / momentum constraint of the final states (L2 = H[7],
         # L-infinity = H[2]); the FP64 run stands for the oracle (held to it at
         # 1e-12 by the tests)
@@ -323,6 +330,28 @@ def main():
                "constraints": cons,
                "fp64_kernel": g64.kernel_info(), "fp64_autotune": tune64}
         del g64
+        if not args.skip_fp32_all:
+            # the configuration as written: every perturbation field in FP32,
+            # the conformal metric included (CG_FP32_ALL)
+            log("[bench] all-FP32 storage")
+            ga = make_ctx(cg, n, order, "fp32_all", modes, args.rms)
+            ta, ka_ms, _, tunea = time_steps(ga, args.warmup, args.steps, not args.no_autotune)
+            ea = rel_errs(ga)
+            ca = ga.setConstraintCalcs()
+            worst = max(ea, key=ea.get)
+            res["fp32_all"] = {
+                "value": pts * args.steps / ta, "ms_per_step": 1e3 * ta / args.steps, "stage_kernel_ms": ka_ms,
+                "frac": b_alg_words() * 4 / 4.0 * pts / (ka_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "over_fp64": (pts * args.steps / ta) / rate64,
+                "max_rel_l2_vs_fp64": float(f"{ea[worst]:.3e}"), "worst_field": worst,
+                "rel_l2_metric_vs_fp64": {f: float(f"{ea[f]:.2e}") for f in FIELDS[:7]},
+                "H_L2_over_fp64": float(ca["H"][7] / c64["H"][7]) if c64["H"][7] else None,
+                "kernel": ga.kernel_info()}
+            log(f"[bench] all-FP32: {res['fp32_all']['ms_per_step']:.3f} ms/step, worst field {worst} "
+                f"{ea[worst]:.2e}")
+            detail_extra["fp32_all_autotune"] = tunea
+            del ga
+        del f64
 
     words = b_alg_words()
     bytes_pt_upd = words * 4                       # FP32 storage
@@ -366,7 +395,7 @@ def main():
         "stage_kernel_launches_timed": nl32,
         "kernel": info32,
     }
-    detail = {"fp32_autotune": tune32}
+    detail = {"fp32_autotune": tune32, **detail_extra}
     if res:
         detail["fp64_autotune"] = res.pop("fp64_autotune")
         line.update(res)

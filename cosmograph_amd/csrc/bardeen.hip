@@ -620,6 +620,7 @@ using namespace cg;
 
 int cg_bardeen(cg_ctx* c, double elapsed_t, int use_ml_scale_factor, double omega_L, double viols[17]) {
   if (!c || !viols) return CG_ERR_ARG;
+  if (c->met32) return fail(c, CG_ERR_ARG, "the Bardeen potentials read the FP64 metric (not in the all-FP32 mode)");
   if (use_ml_scale_factor && (omega_L < 0 || omega_L > 1)) return fail(c, CG_ERR_ARG, "Invalid Omega_Lambda specified.");
   if (c->frw.g_K != 0.0 || c->frw.g_phi != 0.0)
     return fail(c, CG_ERR_STATE,

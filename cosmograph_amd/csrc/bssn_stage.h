@@ -17,6 +17,8 @@ constexpr int NDUST = 4;     // D, S1, S2, S3
 constexpr int NFLUX = 12;    // alpha D v^i (3), alpha S_j v^i (9)
 // stage-kernel modes: bit 0 constraint diagnostics, bit 1 dust fluid
 enum { MODE_STAGE = 0, MODE_CONS = 1, MODE_DUST = 2, MODE_DUST_CONS = 3 };
+// bit 2: all-FP32 storage (the conformal metric in FP32 as well)
+enum { MODE_M32 = 4, MODE_M32_CONS = 5 };
 enum Lapse {
   LAPSE_STATIC = 0, LAPSE_HARMONIC = 1, LAPSE_CONFORMAL_FLRW = 2, LAPSE_ANHARMONIC = 3, LAPSE_ONE_PLUS_LOG = 4,
   LAPSE_CONFORMAL_SYNC = 5, LAPSE_AWA_GAUGE_WAVE = 6, LAPSE_AWA_SHIFTED_WAVE = 7, LAPSE_DAMPED_WAVE = 8,

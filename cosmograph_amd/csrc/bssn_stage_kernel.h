@@ -82,6 +82,10 @@ __device__ __forceinline__ bool nan_as_float(T v) {
 // a cancellation of order 1/(k dx)^2 (~1600 for the longest mode at 256^3),
 // so FP32 storage would put its rounding noise straight into A_ij.
 __device__ __host__ constexpr bool metric_field(int f) { return f <= G33 || f == PHI; }
+// ... except in the all-FP32 mode (MODE_M32), where the metric is stored
+// and differentiated in the storage type like every other field
+template <bool M32>
+__device__ __host__ constexpr bool mfield(int f) { return !M32 && metric_field(f); }
 
 // codes of the dust fluxes in the staging sequence (beyond the field codes)
 constexpr int FLUX0 = 32;
@@ -190,6 +194,9 @@ __device__ __forceinline__ void morton_xy(int m, int lx, int ly, int& x, int& y)
 // differentiates A_ij for the momentum constraint and writes per-point
 // values instead of the RK4 epilogue.  MODE bit 1: the dust fluid of
 // components/dust_fluid/dust.cc is evolved with the metric and sources it.
+// MODE bit 2 (all-FP32 storage): the conformal metric is one more storage-
+// type field; the host points a.regs at field 7 of one 18-field allocation,
+// so RS(f) addresses the metric fields f < 7 too.
 //
 // stage_body is one tile t (< a.ntx*a.nty*a.ntz) of one grid; fv holds the
 // FRW values of the stage.  Every thread of the block must call it (it holds
@@ -226,9 +233,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   constexpr int NPR = HV / 2, NQD = HV4 / 4;                    // FP64 pairs, FP32 quads per tile
   constexpr int CPR = (NPR + 63) / 64, CQD = (NQD + 63) / 64;   // chunks per field
   constexpr int P64 = CPR * 1024, P32 = SPLIT ? CQD * 1024 : P64;  // staging buffer bytes
-  constexpr bool PF = DMA && MODE == 0;                         // epilogue prefetch
-  constexpr int N64 = SPLIT ? 7 : (LIVE_ALPHA ? NF : NF - 1);   // FP64-stored fields updated (ALPHA last)
-  constexpr int N32 = SPLIT ? (LIVE_ALPHA ? NF - 7 : NF - 8) : 0;  // FP32-stored fields updated
+  constexpr bool M32 = (MODE & MODE_M32) != 0;                  // metric in the storage type
+  constexpr bool PF = DMA && (MODE & ~MODE_M32) == 0;           // epilogue prefetch
+  constexpr int F32 = M32 ? 0 : 7;                              // first field of the FP32 prefetch chunks
+  constexpr int N64 = M32 ? 0 : SPLIT ? 7 : (LIVE_ALPHA ? NF : NF - 1);  // FP64-stored fields updated (ALPHA last)
+  constexpr int N32 = M32 ? (LIVE_ALPHA ? NF : NF - 1) : SPLIT ? (LIVE_ALPHA ? NF - 7 : NF - 8) : 0;  // FP32 ones
   constexpr int PFR = (N64 + (N32 + 1) / 2) * 1024;             // prefetch bytes per wave
   constexpr int LSTAGE = 2 * P64 + P32, LPF = PF ? NW * PFR : 0;
   __shared__ __align__(16) char ldma[DMA ? (LSTAGE > LPF ? LSTAGE : LPF) : 16];
@@ -323,7 +332,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   auto INM = [&](int f) -> const double* { return RM(f, (f <= G33 && a.s_in_norm >= 0) ? a.s_in_norm : a.s_in); };
   auto INS = [&](int f) -> const TS* {
     return f >= FLUX0 ? (const TS*)a.flux + (long)(f - FLUX0) * a.rstride
-                      : RS(f, (f >= A11 && f <= A33 && a.s_in_norm >= 0) ? a.s_in_norm : a.s_in);
+                      : RS(f, (((f >= A11 && f <= A33) || (M32 && f <= G33)) && a.s_in_norm >= 0) ? a.s_in_norm
+                                                                                                 : a.s_in);
   };
   auto RD = [&](int q, int slot) -> TS* { return (TS*)a.dreg + ((long)q * 4 + slot) * a.rstride; };
   constexpr bool CONS = (MODE & MODE_CONS) != 0;
@@ -339,8 +349,18 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       const int pp = tid + q * NT;
       if (pp < HP) {
         double g[6][2];
+        if constexpr (M32) {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) pair(INM(G11 + m), q, g[m][0], g[m][1]);
+          for (int m = 0; m < 6; ++m) {
+            TS g0, g1;
+            pair(INS(G11 + m), q, g0, g1);
+            g[m][0] = (double)g0;
+            g[m][1] = (double)g1;
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < 6; ++m) pair(INM(G11 + m), q, g[m][0], g[m][1]);
+        }
 #pragma unroll
         for (int h = 0; h < 2; ++h)
           ldn[2 * pp + h] = norm_factor<double>(g[0][h], g[1][h], g[2][h], g[3][h], g[4][h], g[5][h]);
@@ -351,7 +371,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   double pd[NPP][2];
   TS pf[NPP][2];
   auto fetch = [&](int f) {
-    if (metric_field(f)) {
+    if (mfield<M32>(f)) {
       const double* __restrict__ src = INM(f);
 #pragma unroll
       for (int q = 0; q < NPP; ++q)
@@ -364,7 +384,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     }
   };
   auto store = [&](int buf, int f) {
-    if (metric_field(f)) {
+    if (mfield<M32>(f)) {
       double* __restrict__ dst = ldsraw[buf];
 #pragma unroll
       for (int q = 0; q < NPP; ++q) {
@@ -385,8 +405,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       for (int q = 0; q < NPP; ++q) {
         const int pp = tid + q * NT;
         if (pp < HP) {
-          dst[2 * pp] = (TC)pf[q][0];
-          dst[2 * pp + 1] = (TC)pf[q][1];
+          if (M32 && norm && f <= G33) {
+            const double t0 = ldn[2 * pp], t1 = ldn[2 * pp + 1];
+            dst[2 * pp] = (TC)(((double)pf[q][0] + (diag6(f) ? t0 : 0.0)) / (1.0 - t0));
+            dst[2 * pp + 1] = (TC)(((double)pf[q][1] + (diag6(f) ? t1 : 0.0)) / (1.0 - t1));
+          } else {
+            dst[2 * pp] = (TC)pf[q][0];
+            dst[2 * pp + 1] = (TC)pf[q][1];
+          }
         }
       }
     }
@@ -396,8 +422,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   TC v[NF];
   double gc[7];  // metric at the point in FP64 (DIFFgamma_ij, DIFFphi)
 #pragma unroll
-  for (int f = 0; f < 6; ++f) gc[f] = ldo(INM(f), idx);  // idx = 0 (in bounds) outside the grid
-  const double phi_c = ldo(INM(PHI), idx);
+  for (int f = 0; f < 6; ++f) gc[f] = M32 ? (double)ldo(INS(f), idx) : ldo(INM(f), idx);  // idx = 0 outside the grid
+  const double phi_c = M32 ? (double)ldo(INS(PHI), idx) : ldo(INM(PHI), idx);
   gc[6] = phi_c;
 #pragma unroll
   for (int f = 0; f < NF; ++f)
@@ -480,7 +506,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   // point, folded into the per-point tensors (a macro: the same code as a
   // lambda allocates registers differently and spills)
 #define CG_DERIVE(f, BUF) \
-    if (metric_field(f)) {                                                                                               \
+    if (mfield<M32>(f)) {                                                                                               \
       const double* __restrict__ L = reinterpret_cast<const double*>(BUF);                                               \
       if (KO_ORDER && use_ko) ko[f] = (TC)(a.ko_pref * ko_sum<KO_ORDER, double>(L, c, SI, SJ, SK));                      \
       const double d0 = fd1<R>(L, c, SI) * i1, d1 = fd1<R>(L, c, SJ) * i1, d2 = fd1<R>(L, c, SK) * i1;                   \
@@ -499,7 +525,20 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     } else {                                                                                                             \
       const TC* __restrict__ L = reinterpret_cast<const TC*>(BUF);                                                       \
       if (KO_ORDER && use_ko && f < NF) ko[f] = (TC)a.ko_pref * ko_sum<KO_ORDER, TC>(L, c4, SI4, SJ4, SK);               \
-      if (f == ALPHA) {                                                                                                  \
+      if (M32 && (f <= G33 || f == PHI)) {                                                                               \
+        const TC idx2 = (TC)a.inv_dx2;                                                                                   \
+        const TC d0 = fd1<R>(L, c4, SI4) * idx1, d1 = fd1<R>(L, c4, SJ4) * idx1, d2 = fd1<R>(L, c4, SK) * idx1;          \
+        const TC e00 = fd2<R>(L, c4, SI4) * idx2, e11 = fd2<R>(L, c4, SJ4) * idx2, e22 = fd2<R>(L, c4, SK) * idx2;       \
+        const TC e01 = fdm<R>(L, c4, SI4, SJ4) * idx2, e02 = fdm<R>(L, c4, SI4, SK) * idx2,                              \
+                 e12 = fdm<R>(L, c4, SJ4, SK) * idx2;                                                                    \
+        if (f == PHI) {                                                                                                  \
+          dphi[0] = d0; dphi[1] = d1; dphi[2] = d2;                                                                      \
+          ddphi[0] = e00; ddphi[1] = e01; ddphi[2] = e02; ddphi[3] = e11; ddphi[4] = e12; ddphi[5] = e22;                \
+        } else {                                                                                                         \
+          dg[0][f] = d0; dg[1][f] = d1; dg[2][f] = d2;                                                                   \
+          T1[f] = gi[0] * e00 + gi[3] * e11 + gi[5] * e22 + TC(2) * (gi[1] * e01 + gi[2] * e02 + gi[4] * e12);          \
+        }                                                                                                                \
+      } else if (f == ALPHA) {                                                                                           \
         const TC idx2 = (TC)a.inv_dx2;                                                                                   \
         dal[0] = fd1<R>(L, c4, SI4) * idx1; dal[1] = fd1<R>(L, c4, SJ4) * idx1; dal[2] = fd1<R>(L, c4, SK) * idx1;       \
         ddal[0] = fd2<R>(L, c4, SI4) * idx2; ddal[3] = fd2<R>(L, c4, SJ4) * idx2; ddal[5] = fd2<R>(L, c4, SK) * idx2;    \
@@ -568,8 +607,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     const unsigned lds0 = (unsigned)(size_t)(lds_char*)lb0, lds1 = lds0 + P64, lds2 = lds0 + P64 + P32;
     auto dma = [&](const int f, const int b) {
       const unsigned dst = b == 0 ? lds0 : b == 1 ? lds1 : lds2;
-      if (!SPLIT || metric_field(f)) {
-        const void* src = metric_field(f) ? (const void*)INM(f) : (const void*)INS(f);
+      if (!SPLIT || mfield<M32>(f)) {
+        const void* src = mfield<M32>(f) ? (const void*)INM(f) : (const void*)INS(f);
 #pragma unroll
         for (int m = 0; m < MPR; ++m) dma16(src, poff[m] * 8u, dst + pchunk(m) * 1024);
       } else {
@@ -590,7 +629,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         // position s + 1 are outstanding; for every wave after the barrier
         const int f1 = seq(s + 1 < NSEQ ? s + 1 : 0);
         if (s + 1 < nseq) {
-          if (!SPLIT || metric_field(f1)) dma_wait_barrier<MPR>(); else dma_wait_barrier<MQD>();
+          if (!SPLIT || mfield<M32>(f1)) dma_wait_barrier<MPR>(); else dma_wait_barrier<MQD>();
         } else {
           dma_wait_barrier<0>();
         }
@@ -635,7 +674,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   #pragma unroll
         for (int i = 0; i < (N32 + 1) / 2; ++i) {
           const int n = 2 * i + (ln >> 5);
-          const char* b = (const char*)RS(7 + (n < N32 ? n : 0), f32 ? a.s_f : a.s_p);
+          const char* b = (const char*)RS(F32 + (n < N32 ? n : 0), f32 ? a.s_f : a.s_p);
           if (n < N32 && (!f32 || need_f)) dma16v(b + (size_t)o4 * 4u, reg + (N64 + i) * 1024);
         }
       }
@@ -1027,8 +1066,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       const char* const pfr = ldma + (tid >> 6) * PFR;
       const int pq = tid & 63;
       auto lpf = [&](int f, int kind) -> double {
-        if (!SPLIT || f < 7) return *reinterpret_cast<const double*>(pfr + f * 1024 + kind * 512 + pq * 8);
-        const int n = f - 7;
+        if (!SPLIT || (!M32 && f < 7)) return *reinterpret_cast<const double*>(pfr + f * 1024 + kind * 512 + pq * 8);
+        const int n = f - F32;
         return (double)*reinterpret_cast<const TS*>(pfr + (N64 + (n >> 1)) * 1024 + (n & 1) * 512 + kind * 256 + pq * 4);
       };
       if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -1046,14 +1085,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         auto pre = [&](int g) {
           if (g >= NF || st == 0 || (g == ALPHA && !LIVE_ALPHA)) return;
           if constexpr (!PF) {
-            const bool m = metric_field(g);
+            const bool m = mfield<M32>(g);
             pv[g] = m ? ldo((const double*)RM(g, a.s_p), idx) : (double)ldo((const TS*)RS(g, a.s_p), idx);
             if (INC ? st == 4 : st >= 2)
               qv[g] = m ? ldo((const double*)RM(g, a.s_f), idx) : (double)ldo((const TS*)RS(g, a.s_f), idx);
           }
           if constexpr (INC) {
             if (st == 4) {
-              if (metric_field(g)) {
+              if (mfield<M32>(g)) {
                 a2v[g] = ldo((const double*)RM(g, a.s_a2), idx);
                 a3m[g] = ldo(INM(g), idx);
               } else {
@@ -1069,7 +1108,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
           pre(f + AHEAD);
           if (f == ALPHA && !LIVE_ALPHA) continue;  // lapse registers stay identically zero
           const double cc = (double)rhs[f];
-          if (metric_field(f)) {
+          if (mfield<M32>(f)) {
             double* __restrict__ o = RM(f, a.s_out);
             if (st == 0) {
               sto(o, idx, cc);
@@ -1116,6 +1155,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
             const TS nv = st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc);
             if (f == KK) K_new = (double)nv;
             if (f >= A11 && f <= A33) An[f - A11] = (double)nv;
+            if (M32 && f == PHI) {
+              phi_new = (double)nv;
+              if (st == 4 && nan_as_float(nv)) atomicAdd(a.nan_count, 1u);
+            }
+            if (M32 && f <= G33) gn[f] = (double)nv;
             sto(o, idx, nv);
             if (st != 4 && !INC) sto(fr, idx, (TS)(fo + inc));
           }
@@ -1140,7 +1184,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         const double trA = i11 * An[0] + i22 * An[3] + i33 * An[5] + 2.0 * (i12 * An[1] + i13 * An[2] + i23 * An[4]);
     #pragma unroll
         for (int s = 0; s < 6; ++s) {
-          sto(RM(G11 + s, a.s_norm_out), idx, g[s]);
+          if constexpr (M32) sto(RS(G11 + s, a.s_norm_out), idx, (TS)g[s]);
+          else sto(RM(G11 + s, a.s_norm_out), idx, g[s]);
           sto(RS(A11 + s, a.s_norm_out), idx,
               (TS)((An[s] - (1.0 / 3.0) * ((diag6(s) ? 1.0 : 0.0) + g[s]) * trA) / (1.0 - tn)));
         }

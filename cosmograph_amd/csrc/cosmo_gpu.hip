@@ -34,23 +34,24 @@ struct RegTable {
   void* r[NF][4];
 };
 
-// the conformal metric (DIFFgamma_ij, DIFFphi) is stored in FP64 in every mode
+// the conformal metric (DIFFgamma_ij, DIFFphi) has its own storage type MT:
+// FP64 in the drop-in and the split mode, FP32 in the all-FP32 mode
 __device__ __host__ inline bool is_metric(int f) { return f <= G33 || f == PHI; }
 
-template <typename TS>
+template <typename TS, typename MT>
 __device__ __forceinline__ double ld(const RegTable& t, int f, int s, long x) {
-  return is_metric(f) ? ((const double*)t.r[f][s])[x] : (double)((const TS*)t.r[f][s])[x];
+  return is_metric(f) ? (double)((const MT*)t.r[f][s])[x] : (double)((const TS*)t.r[f][s])[x];
 }
-template <typename TS>
+template <typename TS, typename MT>
 __device__ __forceinline__ void st(const RegTable& t, int f, int s, long x, double v) {
-  if (is_metric(f)) ((double*)t.r[f][s])[x] = v;
+  if (is_metric(f)) ((MT*)t.r[f][s])[x] = (MT)v;
   else ((TS*)t.r[f][s])[x] = (TS)v;
 }
 
 // Elementwise kernels run 256 threads per block; threads past the end keep
 // going with point 0 and store nothing, so whole blocks reach the
 // setExtraFieldData reduction (one partial row per block, see bssn_stage.h).
-template <typename TS>
+template <typename TS, typename MT>
 __global__ void k_step_init(RegTable t, int rp, int ra, int rf, long n, int normalize, double* ext_part,
                             double phiF) {
   const long xg = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -59,8 +60,8 @@ __global__ void k_step_init(RegTable t, int rp, int ra, int rf, long n, int norm
   // RK4Register::stepInit (RK4Register.h:118-127)
   double v[NF];
   for (int f = 0; f < NF; ++f) {
-    v[f] = ld<TS>(t, f, rp, x);
-    if (valid) st<TS>(t, f, rf, x, 0.0);
+    v[f] = ld<TS, MT>(t, f, rp, x);
+    if (valid) st<TS, MT>(t, f, rf, x, 0.0);
   }
   // BSSN::stepInit calls setExtraFieldData before the normalisation, which
   // leaves DIFFphi and DIFFK alone (bssn.cc:219-226); DIFFr is not touched
@@ -86,7 +87,7 @@ __global__ void k_step_init(RegTable t, int rp, int ra, int rf, long n, int norm
     v[A13] = (v[A13] - 1.0 / 3.0 * g13 * trA) / (1.0 - tn);
     v[A23] = (v[A23] - 1.0 / 3.0 * g23 * trA) / (1.0 - tn);
   }
-  for (int f = 0; f < NF; ++f) st<TS>(t, f, ra, x, v[f]);
+  for (int f = 0; f < NF; ++f) st<TS, MT>(t, f, ra, x, v[f]);
 }
 
 
@@ -102,51 +103,51 @@ __device__ __forceinline__ double diff_r(const SrcArgs& s, double dphi, long x) 
 // RK4Register::K{n}Finalize (RK4Register.h:129-176) without the pointer swap,
 // then setExtraFieldData (bssn.cc:267-305) of the new _a: ra is the _a of the
 // finished stage, whose DIFFphi gives that stage's DIFFr
-template <typename TS>
+template <typename TS, typename MT>
 __global__ void k_finalize(RegTable t, int rp, int rc, int rf, int ra, long n, int stage, double dt,
                            double* ext_part, SrcArgs src, double phiF_next) {
   const long xg = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = xg < n;
   const long x = valid ? xg : 0;
   double phi_new = 0.0, K_new = 0.0;
-  const double phi_old = ld<TS>(t, PHI, ra, x);
+  const double phi_old = ld<TS, MT>(t, PHI, ra, x);
   for (int f = 0; f < NF; ++f) {
-    const double c = ld<TS>(t, f, rc, x), p = ld<TS>(t, f, rp, x), fo = ld<TS>(t, f, rf, x);
+    const double c = ld<TS, MT>(t, f, rc, x), p = ld<TS, MT>(t, f, rp, x), fo = ld<TS, MT>(t, f, rf, x);
     double fn, cn;
     if (stage == 1) { fn = fo + dt * c / 6.0; cn = p + dt * c / 2.0; }
     else if (stage == 2) { fn = fo + dt * c / 3.0; cn = p + dt * c / 2.0; }
     else if (stage == 3) { fn = fo + dt * c / 3.0; cn = p + dt * c; }
     else { fn = fo + (dt * c / 6.0 + p); cn = fn; }
-    if (f == PHI) phi_new = cn;
+    if (f == PHI) phi_new = (double)(MT)cn;
     if (f == KK) K_new = is_metric(f) ? cn : (double)(TS)cn;
     if (!valid) continue;
-    st<TS>(t, f, rf, x, fn);
-    st<TS>(t, f, rc, x, cn);
-    if (stage == 4) st<TS>(t, f, rp, x, fn);
+    st<TS, MT>(t, f, rf, x, fn);
+    st<TS, MT>(t, f, rc, x, cn);
+    if (stage == 4) st<TS, MT>(t, f, rp, x, fn);
   }
   if (ext_part) block_ext_put(ext_part, blockIdx.x, valid, phi_new, K_new, diff_r<TS>(src, phi_old, x), phiF_next);
 }
 
 // setExtraFieldData after a fused step's K4Finalize: the new _p, and the
 // stage-4 input register for that stage's DIFFr
-template <typename TS>
-__global__ void k_ext_post(const double* __restrict__ phi, const TS* __restrict__ K,
-                           const double* __restrict__ phi_in, long n, double* ext_part, SrcArgs src, double phiF) {
+template <typename TS, typename MT>
+__global__ void k_ext_post(const MT* __restrict__ phi, const TS* __restrict__ K,
+                           const MT* __restrict__ phi_in, long n, double* ext_part, SrcArgs src, double phiF) {
   const long xg = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = xg < n;
   const long x = valid ? xg : 0;
-  block_ext_put(ext_part, blockIdx.x, valid, phi[x], (double)K[x], diff_r<TS>(src, phi_in[x], x), phiF);
+  block_ext_put(ext_part, blockIdx.x, valid, (double)phi[x], (double)K[x], diff_r<TS>(src, (double)phi_in[x], x), phiF);
 }
 
 // setExtraFieldData over a register alone (DIFFr part left out): used when
 // the averages are needed before any stage has produced them
-template <typename TS>
-__global__ void k_ext_state(const double* __restrict__ phi, const TS* __restrict__ K, long n, double* ext_part,
+template <typename TS, typename MT>
+__global__ void k_ext_state(const MT* __restrict__ phi, const TS* __restrict__ K, long n, double* ext_part,
                             double phiF) {
   const long xg = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = xg < n;
   const long x = valid ? xg : 0;
-  block_ext_put(ext_part, blockIdx.x, valid, phi[x], (double)K[x], 0.0, phiF);
+  block_ext_put(ext_part, blockIdx.x, valid, (double)phi[x], (double)K[x], 0.0, phiF);
 }
 
 // Deterministic combine of partial rows: block b reduces the contiguous
@@ -406,6 +407,8 @@ static __device__ __align__(256) double scratch[kNaScratch];
 namespace cg {
 
 void* reg_ptr(cg_ctx* c, int f, int slot) {
+  // all-FP32 mode: one storage-type allocation of all 18 fields
+  if (c->met32) return (char*)c->regs + ((long)f * c->fstride + (long)slot * c->rstride) * (long)c->ts;
   if (is_metric(f)) return c->met + ((long)f * 4 + slot) * c->rstride;  // f = 0..6
   return (char*)c->regs + ((long)(f - 7) * c->fstride + (long)slot * c->rstride) * (long)c->ts;  // f = 7..17
 }
@@ -488,7 +491,12 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   // the largest element offset times 8 must stay below 2^32.
   if ((long long)p.nx * p.ny * p.nz > (1ll << 29)) return CG_ERR_ARG;
   if (p.stencil_order != 2 && p.stencil_order != 4 && p.stencil_order != 6 && p.stencil_order != 8) return CG_ERR_ARG;
-  if (p.precision != CG_FP64 && p.precision != CG_FP32_SPLIT) return CG_ERR_ARG;
+  if (p.precision != CG_FP64 && p.precision != CG_FP32_SPLIT && p.precision != CG_FP32_ALL) return CG_ERR_ARG;
+  // all-FP32 storage: BSSN with static dust or vacuum, the shift-free gauges
+  // the stage kernel forms by itself, on single grids
+  if (p.precision == CG_FP32_ALL &&
+      (p.matter == CG_MATTER_DUST || p.shift != CG_SHIFT_NONE || p.lapse == LAPSE_TEST_K_DRIVER))
+    return CG_ERR_ARG;
   if (p.matter != CG_MATTER_NONE && p.matter != CG_MATTER_STATIC && p.matter != CG_MATTER_DUST) return CG_ERR_ARG;
   if (p.lapse < 0 || p.lapse >= LAPSE_N) return CG_ERR_ARG;
   if (p.shift < 0 || p.shift >= SHIFT_N) return CG_ERR_ARG;
@@ -498,6 +506,7 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   c->p = p;
   c->prec = p.precision;
   c->ts = p.precision == CG_FP64 ? 8 : 4;
+  c->met32 = p.precision == CG_FP32_ALL;
   c->R = p.stencil_order / 2;
   c->H = halo_for(p.stencil_order, p.ko_coeff);
   c->lambda = p.lambda;
@@ -518,10 +527,13 @@ int cg_create(const cg_params* params, cg_ctx** out) {
     }
   }
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIPCHK(c, hipMalloc(&c->regs, (size_t)(NF - 7) * 4 * pad * c->ts));
-  HIPCHK(c, hipMemsetAsync(c->regs, 0, (size_t)(NF - 7) * 4 * pad * c->ts, c->stream));
-  HIPCHK(c, hipMalloc((void**)&c->met, (size_t)7 * 4 * pad * sizeof(double)));
-  HIPCHK(c, hipMemsetAsync(c->met, 0, (size_t)7 * 4 * pad * sizeof(double), c->stream));
+  const int nreg = c->met32 ? NF : NF - 7;  // storage-type fields
+  HIPCHK(c, hipMalloc(&c->regs, (size_t)nreg * 4 * pad * c->ts));
+  HIPCHK(c, hipMemsetAsync(c->regs, 0, (size_t)nreg * 4 * pad * c->ts, c->stream));
+  if (!c->met32) {
+    HIPCHK(c, hipMalloc((void**)&c->met, (size_t)7 * 4 * pad * sizeof(double)));
+    HIPCHK(c, hipMemsetAsync(c->met, 0, (size_t)7 * 4 * pad * sizeof(double), c->stream));
+  }
   HIPCHK(c, hipMalloc(&c->DIFFD, (size_t)pad * c->ts));
   HIPCHK(c, hipMemsetAsync(c->DIFFD, 0, (size_t)pad * c->ts, c->stream));
   HIPCHK(c, hipMalloc(&c->nan_dev, sizeof(unsigned)));
@@ -643,7 +655,7 @@ static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err, size
     else if (r == 'c') slot = c->sC;
     else { *err = 1; return nullptr; }
     if (for_write && c->a_is_p && (r == 'a' || r == 'f')) { *err = 2; return nullptr; }
-    if (is_metric(f)) *esz = 8;
+    if (is_metric(f) && !c->met32) *esz = 8;
     return reg_ptr(c, f, slot);
   }
   *err = 1;
@@ -766,7 +778,9 @@ namespace cg {
 StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int normalize) {
   StageArgs a{};
   a.met = c->met;
-  a.regs = c->regs;
+  // all-FP32 mode: regs points at field 7 so that the kernel's
+  // regs + (f - 7)*4*rstride addresses the metric fields too
+  a.regs = c->met32 ? (void*)((char*)c->regs + 7 * c->fstride * (long)c->ts) : c->regs;
   a.rstride = c->rstride;
   a.s_in = rin; a.s_p = rp; a.s_f = rf; a.s_out = rout;
   a.s_in_norm = -1; a.s_norm_out = -1;
@@ -858,12 +872,15 @@ static int ext_combine(cg_ctx* c, long rows, bool keep_rho) {
 static int ext_from_state(cg_ctx* c) {
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
-  const double* phi = (const double*)reg_ptr(c, PHI, c->sP);
+  const void* phi = reg_ptr(c, PHI, c->sP);
   if (c->prec == CG_FP64)
-    hipLaunchKernelGGL(k_ext_state<double>, dim3(nb), dim3(bs), 0, c->stream, phi,
+    hipLaunchKernelGGL((k_ext_state<double, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)phi,
                        (const double*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi);
+  else if (c->met32)
+    hipLaunchKernelGGL((k_ext_state<float, float>), dim3(nb), dim3(bs), 0, c->stream, (const float*)phi,
+                       (const float*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi);
   else
-    hipLaunchKernelGGL(k_ext_state<float>, dim3(nb), dim3(bs), 0, c->stream, phi,
+    hipLaunchKernelGGL((k_ext_state<float, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)phi,
                        (const float*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi);
   HIPCHK(c, hipGetLastError());
   int rc = ext_combine(c, nb, true);
@@ -878,26 +895,30 @@ int ext_resolve(cg_ctx* c) {
   c->ext_lazy = false;
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
-  const double* phi = (const double*)reg_ptr(c, PHI, c->sP);
-  const double* phi_in = (const double*)reg_ptr(c, PHI, c->ext_lazy_in);
+  const void* phi = reg_ptr(c, PHI, c->sP);
+  const void* phi_in = reg_ptr(c, PHI, c->ext_lazy_in);
   if (c->prec == CG_FP64)
-    hipLaunchKernelGGL(k_ext_post<double>, dim3(nb), dim3(bs), 0, c->stream, phi,
-                       (const double*)reg_ptr(c, KK, c->sP), phi_in, c->npts, c->ext_part, c->ext_lazy_src,
-                       c->ext_lazy_phiF);
+    hipLaunchKernelGGL((k_ext_post<double, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)phi,
+                       (const double*)reg_ptr(c, KK, c->sP), (const double*)phi_in, c->npts, c->ext_part,
+                       c->ext_lazy_src, c->ext_lazy_phiF);
+  else if (c->met32)
+    hipLaunchKernelGGL((k_ext_post<float, float>), dim3(nb), dim3(bs), 0, c->stream, (const float*)phi,
+                       (const float*)reg_ptr(c, KK, c->sP), (const float*)phi_in, c->npts, c->ext_part,
+                       c->ext_lazy_src, c->ext_lazy_phiF);
   else
-    hipLaunchKernelGGL(k_ext_post<float>, dim3(nb), dim3(bs), 0, c->stream, phi,
-                       (const float*)reg_ptr(c, KK, c->sP), phi_in, c->npts, c->ext_part, c->ext_lazy_src,
-                       c->ext_lazy_phiF);
+    hipLaunchKernelGGL((k_ext_post<float, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)phi,
+                       (const float*)reg_ptr(c, KK, c->sP), (const double*)phi_in, c->npts, c->ext_part,
+                       c->ext_lazy_src, c->ext_lazy_phiF);
   HIPCHK(c, hipGetLastError());
   return ext_combine(c, nb, false);
 }
 }  // namespace cg
 
-template <typename TS>
+template <typename TS, typename MT>
 static int step_init_t(cg_ctx* c) {
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
-  hipLaunchKernelGGL(k_step_init<TS>, dim3(nb), dim3(bs), 0, c->stream, reg_table(c), c->sP, c->sA, c->sF, c->npts,
+  hipLaunchKernelGGL((k_step_init<TS, MT>), dim3(nb), dim3(bs), 0, c->stream, reg_table(c), c->sP, c->sA, c->sF, c->npts,
                      c->p.normalize_metric, c->ext_part, c->frw.g_phi);
   HIPCHK(c, hipGetLastError());
   int rc = ext_combine(c, nb, true);
@@ -944,7 +965,9 @@ int cg_step_init(cg_ctx* c) {
     if (int rc = dust_reg_init(c, c->sP, c->sA, c->sF)) return rc;
   }
   if (int rc = gauge_step_init(c)) return rc;
-  return c->prec == CG_FP64 ? step_init_t<double>(c) : step_init_t<float>(c);
+  return c->prec == CG_FP64 ? step_init_t<double, double>(c)
+         : c->met32          ? step_init_t<float, float>(c)
+                             : step_init_t<float, double>(c);
 }
 
 int cg_rk_evolve(cg_ctx* c) {
@@ -979,11 +1002,11 @@ int rhs_at(cg_ctx* c, int rin, int rout) {
 }
 }  // namespace cg
 
-template <typename TS>
+template <typename TS, typename MT>
 static int finalize_t(cg_ctx* c, int n, const SrcArgs& src) {
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
-  hipLaunchKernelGGL(k_finalize<TS>, dim3(nb), dim3(bs), 0, c->stream, reg_table(c), c->sP, c->sC, c->sF, c->sA,
+  hipLaunchKernelGGL((k_finalize<TS, MT>), dim3(nb), dim3(bs), 0, c->stream, reg_table(c), c->sP, c->sC, c->sF, c->sA,
                      c->npts, n, c->p.dt, c->ext_part, src, c->frw.g_phi);
   HIPCHK(c, hipGetLastError());
   return ext_combine(c, nb, false);
@@ -1008,7 +1031,9 @@ int cg_k_finalize(cg_ctx* c, int n) {
     HIPCHK(c, hipGetLastError());
   }
   if (int rc = gauge_finalize(c, n)) return rc;
-  int rc = c->prec == CG_FP64 ? finalize_t<double>(c, n, src) : finalize_t<float>(c, n, src);
+  int rc = c->prec == CG_FP64 ? finalize_t<double, double>(c, n, src)
+           : c->met32          ? finalize_t<float, float>(c, n, src)
+                               : finalize_t<float, double>(c, n, src);
   if (rc) return rc;
   std::swap(c->sA, c->sC);  // swap_a_c
   c->ext_fresh = (n == 4);
@@ -1206,11 +1231,14 @@ int cg::cons_input(cg_ctx* c, int* rin_out) {
     const int bs = 256;
     const long nbk = (c->npts + bs - 1) / bs;
     if (c->prec == CG_FP64)
-      hipLaunchKernelGGL(k_step_init<double>, dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin, c->sF,
-                         c->npts, c->p.normalize_metric, (double*)nullptr, 0.0);
+      hipLaunchKernelGGL((k_step_init<double, double>), dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin,
+                         c->sF, c->npts, c->p.normalize_metric, (double*)nullptr, 0.0);
+    else if (c->met32)
+      hipLaunchKernelGGL((k_step_init<float, float>), dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin,
+                         c->sF, c->npts, c->p.normalize_metric, (double*)nullptr, 0.0);
     else
-      hipLaunchKernelGGL(k_step_init<float>, dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin, c->sF,
-                         c->npts, c->p.normalize_metric, (double*)nullptr, 0.0);
+      hipLaunchKernelGGL((k_step_init<float, double>), dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin,
+                         c->sF, c->npts, c->p.normalize_metric, (double*)nullptr, 0.0);
     HIPCHK(c, hipGetLastError());
     if (c->dreg && dust_reg_init(c, c->sP, rin, c->sF)) return CG_ERR_HIP;
   }
@@ -1462,6 +1490,7 @@ __global__ void __launch_bounds__(kCwThreads) k_conformal_sums(const T* __restri
 
 int cg_field_stats(cg_ctx* c, const char* name, double out[3]) {
   if (!c || !out) return CG_ERR_ARG;
+  if (c->met32) return fail(c, CG_ERR_ARG, "conformal statistics read an FP64 DIFFphi (not in the all-FP32 mode)");
   int e;
   size_t esz, pesz;
   const void* d = resolve(c, name, false, &e, &esz);

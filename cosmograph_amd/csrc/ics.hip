@@ -161,12 +161,12 @@ __global__ void k_update(double* __restrict__ u, const double* __restrict__ x, l
 // DIFFphi = log u and the conserved density e^{6 phi} rho - sub (static dust:
 // sub = rho_bar, DIFFD; dust fluid at rest: sub = 0, D); kk (dust fluid
 // only, else null) = the uniform extrinsic curvature
-template <typename TS>
+template <typename TS, typename MT>
 __global__ void k_out(const double* __restrict__ u, const double* __restrict__ drho, double c, double rho_bar,
-                      double sub, long n, double* __restrict__ phi, TS* __restrict__ D, TS* __restrict__ kk, double K) {
+                      double sub, long n, MT* __restrict__ phi, TS* __restrict__ D, TS* __restrict__ kk, double K) {
   for (long i = blockIdx.x * (long)kT + threadIdx.x; i < n; i += (long)kT * gridDim.x) {
     const double p = log(u[i]);
-    phi[i] = p;
+    phi[i] = (MT)p;
     D[i] = (TS)(exp(6.0 * p) * ((rho_bar + drho[i]) - c) - sub);
     if (kk) kk[i] = (TS)K;
   }
@@ -280,22 +280,26 @@ int cg_ics_static_dust(cg_ctx* c, const double* modes, size_t n_modes, double rm
   // the FRW fluids (the dust and, for Lambda, a w = -1 fluid) and DIFFK = 0
   const bool bg = fluid && c->dust_bg;
   for (int f = 0; f < NF; ++f)
-    if (f != PHI) HIPCHK(c, hipMemsetAsync(reg_ptr(c, f, c->sP), 0, n * (f <= PHI ? sizeof(double) : c->ts), s));
+    if (f != PHI)
+      HIPCHK(c, hipMemsetAsync(reg_ptr(c, f, c->sP), 0, n * (f <= PHI && !c->met32 ? sizeof(double) : c->ts), s));
   if (fluid)
     for (int q = 0; q < NDUST; ++q)
       HIPCHK(c, hipMemsetAsync((char*)c->dreg + ((long)q * 4 + c->sP) * c->rstride * (long)c->ts, 0, n * c->ts, s));
   else
     HIPCHK(c, hipMemsetAsync(c->DIFFD, 0, n * c->ts, s));
-  double* phi = (double*)reg_ptr(c, PHI, c->sP);
+  void* phi = reg_ptr(c, PHI, c->sP);
   void* D = fluid ? (void*)((char*)c->dreg + (long)c->sP * c->rstride * (long)c->ts) : c->DIFFD;
   void* kk = fluid && !bg ? reg_ptr(c, KK, c->sP) : nullptr;
   const double sub = fluid ? 0.0 : rho_bar;
   if (c->ts == 8)
-    hipLaunchKernelGGL(k_out<double>, dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, sub, n, phi, (double*)D,
-                       (double*)kk, K0);
+    hipLaunchKernelGGL((k_out<double, double>), dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, sub, n,
+                       (double*)phi, (double*)D, (double*)kk, K0);
+  else if (c->met32)
+    hipLaunchKernelGGL((k_out<float, float>), dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, sub, n,
+                       (float*)phi, (float*)D, (float*)kk, K0);
   else
-    hipLaunchKernelGGL(k_out<float>, dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, sub, n, phi, (float*)D,
-                       (float*)kk, K0);
+    hipLaunchKernelGGL((k_out<float, double>), dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, sub, n,
+                       (double*)phi, (float*)D, (float*)kk, K0);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(s));
   c->frw = Frw{};

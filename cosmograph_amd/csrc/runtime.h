@@ -122,6 +122,7 @@ struct cg_ctx {
   bool keep_accumulator = false;          // fused steps with the reference's _f accumulator (cg_set_fused_accumulator)
   bool incr_pick = true;                  // the autotuner's choice between the two RK4 forms
   bool alpha_nonzero = false;             // a non-zero DIFFalpha was uploaded
+  bool met32 = false;                     // all-FP32 storage: the metric in regs too (CG_FP32_ALL)
   double* cons = nullptr;                 // [CONS_N][pad] per-point constraint values (lazy)
   double* cons_part = nullptr;            // reduction partials
   bool cons_valid = false;                // ricci_a / AijAij_a filled

@@ -55,6 +55,9 @@ def test_python_front_end_binds_every_symbol():
     {"n": (1024, 1024, 520), "precision": "fp32_split"}, {"n": (1024, 1024, 513)}, {"order": 3}, {"precision": "bf16"}, {"lapse": "GeneralizedNewton"}, {"dt_frac": float("nan")},
     # the shift build carries BSSN + static dust only
     {"shift": "Static", "matter": "dust"}, {"shift": "Shiftless"},
+    # all-FP32 storage: BSSN + static dust on single grids, the stage kernel's own gauges
+    {"precision": "fp32_all", "matter": "dust"}, {"precision": "fp32_all", "shift": "Static"},
+    {"precision": "fp32_all", "lapse": "TestKDriverLapse"},
 ])
 def test_create_rejects_bad_parameters(bad):
     import cosmograph_amd as cg
