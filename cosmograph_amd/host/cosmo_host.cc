@@ -130,8 +130,9 @@ cg_params params_from_config(const ConfigParser& cfg) {
   const std::string prec = cfg("gpu_precision", "fp64");
   if (prec == "fp64") p.precision = CG_FP64;
   else if (prec == "fp32_split") p.precision = CG_FP32_SPLIT;
+  else if (prec == "fp32_all") p.precision = CG_FP32_ALL;
   else {
-    std::cerr << "Error - gpu_precision must be fp64 or fp32_split.\n";
+    std::cerr << "Error - gpu_precision must be fp64, fp32_split or fp32_all.\n";
     throw -1;
   }
   p.device = std::stoi(cfg("gpu_device", "0"));

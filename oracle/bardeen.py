@@ -342,7 +342,8 @@ def set_potentials(a_reg, c_reg, t, order, dx, box, use_ml=True, omega_L=0.0):
               "dt_C1": dt_C[0], "dt_C2": dt_C[1], "dt_C3": dt_C[2], "Vmag": Vmag,
               "D11": Dt[(0, 0)], "D12": Dt[(0, 1)], "D13": Dt[(0, 2)], "D22": Dt[(1, 1)], "D23": Dt[(1, 2)],
               "D33": Dt[(2, 2)]}
-    inter = {"d2t_phi": d2t_phi, "h0": h0, "a": a, "dadt": dadt, "H": H, "d2adt2": d2adt2}
+    inter = {"d2t_phi": d2t_phi, "h0": h0, "a": a, "dadt": dadt, "H": H, "d2adt2": d2adt2, "lin_viol": lin_viol,
+             "lin_viol_der": lin_viol_der}
     for I, J in SYM:
         inter[f"d2t_g{I + 1}{J + 1}"] = d2t_g[(I, J)]
         inter[f"h{I + 1}{J + 1}"] = h[(I, J)]

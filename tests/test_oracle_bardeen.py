@@ -15,7 +15,10 @@ components/bssn/bardeen.cc) without the reference, which needs FFTW:
     time derivatives of A and B are recovered from those of h;
   * d^2/dt^2 of the conformal metric and of phi (bardeen.cc:102-143) equal
     the time derivative of the C oracle's right-hand side along an RK4
-    trajectory (central difference), with a live lapse.
+    trajectory (central difference), with a live lapse;
+  * the linearised-constraint violation (bardeen.cc:412-415) is the
+    no-anisotropic-stress condition 2 (Phi - Psi) of the Bardeen potentials
+    formed separately (bardeen.cc:329-340), at every point.
 """
 import math
 
@@ -214,3 +217,28 @@ def test_second_time_derivatives_follow_the_trajectory(oracle_mod):
         gp = {k: v for k, v in cp.items()}
         np.testing.assert_allclose(inter[f"h{I + 1}{J + 1}"] + (inter["a"] ** 2 if I == J else 0.0),
                                    np.exp(4 * a0["DIFFphi"]) * g, rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("use_ml", [True, False])
+def test_linear_violation_is_phi_minus_psi(oracle_mod, use_ml):
+    """lin_viol = E + A - a^2 B'' - 3 a a' B' + 2 a F' + 4 a' F equals
+    2 (Phi - Psi) with Phi = (E + 2(a' F + a F') - 2 a a' B' - a^2 B'')/2 and
+    Psi = -(A + 2 H (a F - a^2 B'/2))/2: a dropped term or a wrong factor in
+    either formula breaks the identity.  Data with a shift, so that F != 0."""
+    po = oracle_mod
+    n = 12
+    X, Y, Z = grid(n)
+    a_reg, c_reg = static_background((n, n, n))
+    for q, f in enumerate(FIELDS):
+        a_reg[f] = 1e-3 * np.sin(2 * np.pi * (X + q * Y) + q) + (0.02 if f == "DIFFK" else 0.0)
+        c_reg[f] = 2e-3 * np.cos(2 * np.pi * (Z - q * X) + 0.3 * q)
+    for m in range(3):
+        a_reg[f"beta{m + 1}"] = 1e-2 * np.sin(2 * np.pi * (Y + m * Z) + m)
+        c_reg[f"beta{m + 1}"] = 3e-3 * np.cos(2 * np.pi * (X - m * Y))
+    f, v, inter = bd.set_potentials(a_reg, c_reg, 0.3, 4, 1.0 / n, 1.0, use_ml=use_ml, omega_L=0.4)
+    lv = inter["lin_viol"]
+    assert np.abs(f["F"]).max() > 1e-6 and np.abs(lv).max() > 1e-5
+    np.testing.assert_allclose(lv, 2.0 * (f["Phi"] - f["Psi"]), rtol=0, atol=1e-13 * np.abs(lv).max())
+    # the signed maxima among the 17 violations (bardeen.cc:422-444)
+    assert v[7] == float(np.max(lv))
+    assert v[3] == float(np.max(inter["lin_viol_der"]))
