@@ -223,7 +223,9 @@ def c5_leg(cg, args, log):
         "ms_per_step": r32["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 (FP64 metric)", "data": "synthetic: seeded Gaussian random field ICs, constraint-solved",
         "config": {"workload": f"dust fluid + Lambda {n}^3, order {order}, k^{args.spectral_index}, "
-                               f"rms {args.rms}, Lambda energy density 0.7 rho_bar"},
+                               f"rms {args.rms}, Lambda energy density 0.7 rho_bar; KO coefficient 0.1, "
+                               f"which is zero at order 6 as the reference's KO stencil; fluid D and S_i "
+                               f"only (no energy variable E, as the reference)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "b_alg_bytes_per_point_update": b_alg,

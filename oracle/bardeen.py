@@ -27,6 +27,12 @@ Reference behaviour kept on purpose (they change the outputs):
     coordinate average of DIFFK;
   * the violation "max" is the largest signed value (math.h:1489-1500);
   * standard deviations divide by N - 1 (math.h:1424-1435).
+
+Parity unpinned: which statistic (average, maximum, standard deviation) and
+which normalisation each of the 17 violations takes (bardeen.cc:382-472).
+The fields they reduce are pinned by pure scalar, vector and tensor modes,
+the matter + Lambda scale factor by its closed form, the second time
+derivatives by the trajectory, and lin_viol by lin_viol = 2 (Phi - Psi).
 """
 from __future__ import annotations
 

@@ -7,6 +7,15 @@
  * the association order of the reference line they cite, so that pieces
  * which the reference lets us build from source (stencils, RK4 registers,
  * FRW, the static-dust source) agree bitwise.
+ *
+ * Parity unpinned (no check beyond the reference's own formula, since
+ * BSSNGaugeHandler.cc and bssn.cc do not build here): the Anharmonic,
+ * ConformalSync, TestKDriverLapse and TestAijDriverLapse lapses, the mu_L
+ * and p terms of the DampedWave lapse, the mu_S term of the DampedWave
+ * shift, the AijDriverShift, and the scale (normalisation) arrays of the
+ * H, M, G, A, S constraint statistics.  The tests fix their wiring (which
+ * K, phi and K_avg each stage sees; homogeneous closed forms) but restate
+ * the formula itself.
  */
 #include "oracle.h"
 
