@@ -89,7 +89,8 @@ def check(g, o, names, regs, tol, what):
 
 CASES = [("Static", 4, "OnePlusLog"), ("GammaDriver", 4, "Harmonic"), ("DampedWave", 4, "DampedWave"),
          ("AwAShiftedWave", 4, "AwAShiftedWave"), ("AijDriverShift", 4, "Static"), ("DampedWave", 2, "OnePlusLog"),
-         ("GammaDriver", 6, "Static"), ("AwAShiftedWave", 8, "AwAGaugeWave")]
+         ("GammaDriver", 6, "Static"), ("AwAShiftedWave", 8, "AwAGaugeWave"), ("DampedWave", 6, "OnePlusLog"),
+         ("AijDriverShift", 2, "TestKDriverLapse"), ("Static", 6, "TestKDriverLapse")]
 
 
 @pytest.mark.parametrize("shift,order,lapse", CASES)
