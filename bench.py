@@ -258,6 +258,7 @@ def main():
     ap.add_argument("--detail-file", default=None, help="also write the line and the detail tables to this file")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-fp32-all", action="store_true", help="skip the all-FP32 storage leg")
+    ap.add_argument("--skip-fp32-fix", action="store_true", help="skip the fixed-point metric leg")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="batched-ensemble leg (0 = skip)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -332,16 +333,18 @@ def main():
                "constraints": cons,
                "fp64_kernel": g64.kernel_info(), "fp64_autotune": tune64}
         del g64
-        if not args.skip_fp32_all:
-            # the configuration as written: every perturbation field in FP32,
-            # the conformal metric included (CG_FP32_ALL)
-            log("[bench] all-FP32 storage")
-            ga = make_ctx(cg, n, order, "fp32_all", modes, args.rms)
+        # 4-byte storage of every perturbation field: the conformal metric in
+        # FP32 (CG_FP32_ALL) or in 32-bit fixed point (CG_FP32_FIX)
+        for prec, skip in (("fp32_all", args.skip_fp32_all), ("fp32_fix", args.skip_fp32_fix)):
+            if skip:
+                continue
+            log(f"[bench] {prec} storage")
+            ga = make_ctx(cg, n, order, prec, modes, args.rms)
             ta, ka_ms, _, tunea = time_steps(ga, args.warmup, args.steps, not args.no_autotune)
             ea = rel_errs(ga)
             ca = ga.setConstraintCalcs()
             worst = max(ea, key=ea.get)
-            res["fp32_all"] = {
+            res[prec] = {
                 "value": pts * args.steps / ta, "ms_per_step": 1e3 * ta / args.steps, "stage_kernel_ms": ka_ms,
                 "frac": b_alg_words() * 4 / 4.0 * pts / (ka_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "over_fp64": (pts * args.steps / ta) / rate64,
@@ -349,9 +352,8 @@ def main():
                 "rel_l2_metric_vs_fp64": {f: float(f"{ea[f]:.2e}") for f in FIELDS[:7]},
                 "H_L2_over_fp64": float(ca["H"][7] / c64["H"][7]) if c64["H"][7] else None,
                 "kernel": ga.kernel_info()}
-            log(f"[bench] all-FP32: {res['fp32_all']['ms_per_step']:.3f} ms/step, worst field {worst} "
-                f"{ea[worst]:.2e}")
-            detail_extra["fp32_all_autotune"] = tunea
+            log(f"[bench] {prec}: {res[prec]['ms_per_step']:.3f} ms/step, worst field {worst} {ea[worst]:.2e}")
+            detail_extra[f"{prec}_autotune"] = tunea
             del ga
         del f64
 

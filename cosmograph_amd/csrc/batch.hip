@@ -154,7 +154,7 @@ int cg_batch_create(const cg_params* params, int n, cg_batch** out) {
       return CG_ERR_ARG;  // one kernel variant serves every grid
     if (lapse_needs_kavg(p.lapse)) return CG_ERR_ARG;  // per-grid K_avg every stage: single grids only
     if (p.matter == CG_MATTER_DUST) return CG_ERR_ARG;           // the dust fluid runs on single grids
-    if (p.precision == CG_FP32_ALL) return CG_ERR_ARG;          // all-FP32 storage: single grids
+    if (p.precision == CG_FP32_ALL || p.precision == CG_FP32_FIX) return CG_ERR_ARG;  // single grids only
     if (p.shift != CG_SHIFT_NONE || p.lapse == CG_LAPSE_TEST_K_DRIVER) return CG_ERR_ARG;  // gauge pass: single grids
     if (p.ko_coeff != 0.0 && p.normalize_metric && (p.stencil_order == 2 || p.stencil_order == 8))
       return CG_ERR_ARG;  // needs the materialised stepInit of the single-grid path

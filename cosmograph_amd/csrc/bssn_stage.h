@@ -17,8 +17,23 @@ constexpr int NDUST = 4;     // D, S1, S2, S3
 constexpr int NFLUX = 12;    // alpha D v^i (3), alpha S_j v^i (9)
 // stage-kernel modes: bit 0 constraint diagnostics, bit 1 dust fluid
 enum { MODE_STAGE = 0, MODE_CONS = 1, MODE_DUST = 2, MODE_DUST_CONS = 3 };
-// bit 2: all-FP32 storage (the conformal metric in FP32 as well)
-enum { MODE_M32 = 4, MODE_M32_CONS = 5 };
+// bit 2: all-FP32 storage (the conformal metric in FP32 as well); bit 3
+// (with bit 2): the conformal metric in 32-bit fixed point instead, value =
+// integer word x 2^e with one power-of-two scale per field
+enum { MODE_M32 = 4, MODE_M32_CONS = 5, MODE_MQ = 8, MODE_MQ_STAGE = 12, MODE_MQ_CONS = 13 };
+// fixed-point metric word (MODE_MQ): value / 2^e rounded to the nearest
+// integer, saturated at MQ_WMAX = 2^30 - 1 in magnitude, so that the sum or
+// difference of two words never overflows int32 (between exponent checks
+// the host keeps |word| below 2^28; saturation means a field grew fourfold
+// within one check interval, and the stage kernels count it)
+constexpr double MQ_WMAX = 1073741823.0;
+__device__ __forceinline__ int mq_word(double v, double inv) {
+  const double t = fmin(fmax(v * inv, -MQ_WMAX), MQ_WMAX);
+  return (int)rint(t);
+}
+constexpr int MQ_BITS = 28;        // words stay below 2^MQ_BITS after each exponent check
+constexpr int MQ_FLOOR = 6;        // exponents stay within 6 of the largest metric field's
+
 enum Lapse {
   LAPSE_STATIC = 0, LAPSE_HARMONIC = 1, LAPSE_CONFORMAL_FLRW = 2, LAPSE_ANHARMONIC = 3, LAPSE_ONE_PLUS_LOG = 4,
   LAPSE_CONFORMAL_SYNC = 5, LAPSE_AWA_GAUGE_WAVE = 6, LAPSE_AWA_SHIFTED_WAVE = 7, LAPSE_DAMPED_WAVE = 8,
@@ -110,6 +125,10 @@ struct StageArgs {
   const double* cons_means;
   int cons_pass;
   int tile_base, tile_count;
+  // fixed-point metric (MODE_MQ): scale 2^e and its inverse of field f < 7,
+  // and the count of saturated words
+  double msc[7], minv[7];
+  unsigned int* mq_sat;
 };
 
 // row of the reduced constraint mode, per constraint q (H, M, G, A, S):

@@ -69,6 +69,7 @@ __device__ __forceinline__ void ld2(const float* __restrict__ base, unsigned eof
   v1 = v.y;
 }
 
+
 template <typename T>
 __device__ __forceinline__ bool nan_as_float(T v) {
   // numNaNs tests the float-rounded value (utils/math.h:1524-1541)
@@ -234,7 +235,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   constexpr int CPR = (NPR + 63) / 64, CQD = (NQD + 63) / 64;   // chunks per field
   constexpr int P64 = CPR * 1024, P32 = SPLIT ? CQD * 1024 : P64;  // staging buffer bytes
   constexpr bool M32 = (MODE & MODE_M32) != 0;                  // metric in the storage type
-  constexpr bool PF = DMA && (MODE & ~MODE_M32) == 0;           // epilogue prefetch
+  constexpr bool MQ = M32 && (MODE & MODE_MQ) != 0;             // ... as 32-bit fixed point
+  // metric tiles hold the fixed-point words (FP32 compute); with FP64
+  // compute (constraint mode) they hold the scaled FP64 values instead
+  constexpr bool MQW = MQ && sizeof(TC) == 4;
+  static_assert(!MQ || sizeof(TS) == 4, "the fixed-point metric shares the 4-byte storage");
+  constexpr bool PF = DMA && (MODE & ~(MODE_M32 | MODE_MQ)) == 0;  // epilogue prefetch
   constexpr int F32 = M32 ? 0 : 7;                              // first field of the FP32 prefetch chunks
   constexpr int N64 = M32 ? 0 : SPLIT ? 7 : (LIVE_ALPHA ? NF : NF - 1);  // FP64-stored fields updated (ALPHA last)
   constexpr int N32 = M32 ? (LIVE_ALPHA ? NF : NF - 1) : SPLIT ? (LIVE_ALPHA ? NF - 7 : NF - 8) : 0;  // FP32 ones
@@ -354,8 +360,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
           for (int m = 0; m < 6; ++m) {
             TS g0, g1;
             pair(INS(G11 + m), q, g0, g1);
-            g[m][0] = (double)g0;
-            g[m][1] = (double)g1;
+            g[m][0] = MQ ? (double)__float_as_int(g0) * a.msc[G11 + m] : (double)g0;
+            g[m][1] = MQ ? (double)__float_as_int(g1) * a.msc[G11 + m] : (double)g1;
           }
         } else {
 #pragma unroll
@@ -405,7 +411,26 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       for (int q = 0; q < NPP; ++q) {
         const int pp = tid + q * NT;
         if (pp < HP) {
-          if (M32 && norm && f <= G33) {
+          if (MQ && !MQW && f < 7) {
+            // FP64 compute: the scaled values (normalised at stage 1)
+            const double sc = a.msc[f < 7 ? f : 0];
+            double w0 = (double)__float_as_int(pf[q][0]) * sc, w1 = (double)__float_as_int(pf[q][1]) * sc;
+            if (norm && f <= G33) {
+              const double t0 = ldn[2 * pp], t1 = ldn[2 * pp + 1];
+              w0 = (w0 + (diag6(f) ? t0 : 0.0)) / (1.0 - t0);
+              w1 = (w1 + (diag6(f) ? t1 : 0.0)) / (1.0 - t1);
+            }
+            dst[2 * pp] = (TC)w0;
+            dst[2 * pp + 1] = (TC)w1;
+          } else if (MQW && norm && f <= G33) {
+            // the normalised halo value, back in the field's fixed point
+            const double t0 = ldn[2 * pp], t1 = ldn[2 * pp + 1], sc = a.msc[f < 7 ? f : 0], iv = a.minv[f < 7 ? f : 0];
+            const double w0 = ((double)__float_as_int(pf[q][0]) * sc + (diag6(f) ? t0 : 0.0)) / (1.0 - t0);
+            const double w1 = ((double)__float_as_int(pf[q][1]) * sc + (diag6(f) ? t1 : 0.0)) / (1.0 - t1);
+            if (fmax(fabs(w0), fabs(w1)) * iv > MQ_WMAX) atomicAdd(a.mq_sat, 1u);
+            dst[2 * pp] = __int_as_float(mq_word(w0, iv));
+            dst[2 * pp + 1] = __int_as_float(mq_word(w1, iv));
+          } else if (M32 && norm && f <= G33) {
             const double t0 = ldn[2 * pp], t1 = ldn[2 * pp + 1];
             dst[2 * pp] = (TC)(((double)pf[q][0] + (diag6(f) ? t0 : 0.0)) / (1.0 - t0));
             dst[2 * pp + 1] = (TC)(((double)pf[q][1] + (diag6(f) ? t1 : 0.0)) / (1.0 - t1));
@@ -421,9 +446,16 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   // ---------------------------------------------------------- point values
   TC v[NF];
   double gc[7];  // metric at the point in FP64 (DIFFgamma_ij, DIFFphi)
+  // fixed-point metric: the words at the point (stage 4's a3 of the metric)
+  int gw[7];
 #pragma unroll
-  for (int f = 0; f < 6; ++f) gc[f] = M32 ? (double)ldo(INS(f), idx) : ldo(INM(f), idx);  // idx = 0 outside the grid
-  const double phi_c = M32 ? (double)ldo(INS(PHI), idx) : ldo(INM(PHI), idx);
+  for (int f = 0; f < 7; ++f) gw[f] = MQ ? ldo((const int*)INS(f), idx) : 0;
+#pragma unroll
+  for (int f = 0; f < 6; ++f)  // idx = 0 outside the grid
+    gc[f] = MQ ? (double)gw[f] * a.msc[f] : M32 ? (double)ldo(INS(f), idx) : ldo(INM(f), idx);
+  const double phi_c = MQ    ? (double)gw[PHI] * a.msc[PHI]
+                       : M32 ? (double)ldo(INS(PHI), idx)
+                             : ldo(INM(PHI), idx);
   gc[6] = phi_c;
 #pragma unroll
   for (int f = 0; f < NF; ++f)
@@ -524,8 +556,28 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       }                                                                                                                  \
     } else {                                                                                                             \
       const TC* __restrict__ L = reinterpret_cast<const TC*>(BUF);                                                       \
-      if (KO_ORDER && use_ko && f < NF) ko[f] = (TC)a.ko_pref * ko_sum<KO_ORDER, TC>(L, c4, SI4, SJ4, SK);               \
-      if (M32 && (f <= G33 || f == PHI)) {                                                                               \
+      const int* __restrict__ LQ = reinterpret_cast<const int*>(BUF);                                                    \
+      if (KO_ORDER && use_ko && f < NF) {                                                                                \
+        if (MQW && (f <= G33 || f == PHI))                                                                               \
+          ko[f] = (TC)(a.ko_pref * a.msc[f < 7 ? f : 0] * ko_sumq<KO_ORDER>(LQ, c4, SI4, SJ4, SK));                      \
+        else                                                                                                             \
+          ko[f] = (TC)a.ko_pref * ko_sum<KO_ORDER, TC>(L, c4, SI4, SJ4, SK);                                             \
+      }                                                                                                                  \
+      if (MQW && (f <= G33 || f == PHI)) {                                                                               \
+        /* exact integer differences of the words, FP32 sums, the field's scale */                                      \
+        const TC s1 = (TC)(a.msc[f < 7 ? f : 0] * i1), s2 = (TC)(a.msc[f < 7 ? f : 0] * i2);                           \
+        const TC d0 = fd1qf<R>(LQ, c4, SI4) * s1, d1 = fd1qf<R>(LQ, c4, SJ4) * s1, d2 = fd1qf<R>(LQ, c4, SK) * s1;       \
+        const TC e00 = fd2qf<R>(LQ, c4, SI4) * s2, e11 = fd2qf<R>(LQ, c4, SJ4) * s2, e22 = fd2qf<R>(LQ, c4, SK) * s2;    \
+        const TC e01 = fdmqf<R>(LQ, c4, SI4, SJ4) * s2, e02 = fdmqf<R>(LQ, c4, SI4, SK) * s2,                           \
+                 e12 = fdmqf<R>(LQ, c4, SJ4, SK) * s2;                                                                   \
+        if (f == PHI) {                                                                                                  \
+          dphi[0] = d0; dphi[1] = d1; dphi[2] = d2;                                                                      \
+          ddphi[0] = e00; ddphi[1] = e01; ddphi[2] = e02; ddphi[3] = e11; ddphi[4] = e12; ddphi[5] = e22;                \
+        } else {                                                                                                         \
+          dg[0][f] = d0; dg[1][f] = d1; dg[2][f] = d2;                                                                   \
+          T1[f] = gi[0] * e00 + gi[3] * e11 + gi[5] * e22 + TC(2) * (gi[1] * e01 + gi[2] * e02 + gi[4] * e12);          \
+        }                                                                                                                \
+      } else if (M32 && (f <= G33 || f == PHI)) {                                                                        \
         const TC idx2 = (TC)a.inv_dx2;                                                                                   \
         const TC d0 = fd1<R>(L, c4, SI4) * idx1, d1 = fd1<R>(L, c4, SJ4) * idx1, d2 = fd1<R>(L, c4, SK) * idx1;          \
         const TC e00 = fd2<R>(L, c4, SI4) * idx2, e11 = fd2<R>(L, c4, SJ4) * idx2, e22 = fd2<R>(L, c4, SK) * idx2;       \
@@ -1068,7 +1120,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       auto lpf = [&](int f, int kind) -> double {
         if (!SPLIT || (!M32 && f < 7)) return *reinterpret_cast<const double*>(pfr + f * 1024 + kind * 512 + pq * 8);
         const int n = f - F32;
-        return (double)*reinterpret_cast<const TS*>(pfr + (N64 + (n >> 1)) * 1024 + (n & 1) * 512 + kind * 256 + pq * 4);
+        const char* w = pfr + (N64 + (n >> 1)) * 1024 + (n & 1) * 512 + kind * 256 + pq * 4;
+        if (MQ && f < 7) return (double)*reinterpret_cast<const int*>(w) * a.msc[f < 7 ? f : 0];
+        return (double)*reinterpret_cast<const TS*>(w);
       };
       if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       // one loop per RK4 form, each with only its own loads and stores
@@ -1084,11 +1138,15 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         double a2v[NF], a3m[7], pv[NF], qv[NF];
         auto pre = [&](int g) {
           if (g >= NF || st == 0 || (g == ALPHA && !LIVE_ALPHA)) return;
+          // a storage-type word as FP64 (fixed-point metric words scaled)
+          auto lw = [&](int slot) -> double {
+            if (MQ && g < 7) return (double)ldo((const int*)RS(g, slot), idx) * a.msc[g < 7 ? g : 0];
+            return (double)ldo((const TS*)RS(g, slot), idx);
+          };
           if constexpr (!PF) {
             const bool m = mfield<M32>(g);
-            pv[g] = m ? ldo((const double*)RM(g, a.s_p), idx) : (double)ldo((const TS*)RS(g, a.s_p), idx);
-            if (INC ? st == 4 : st >= 2)
-              qv[g] = m ? ldo((const double*)RM(g, a.s_f), idx) : (double)ldo((const TS*)RS(g, a.s_f), idx);
+            pv[g] = m ? ldo((const double*)RM(g, a.s_p), idx) : lw(a.s_p);
+            if (INC ? st == 4 : st >= 2) qv[g] = m ? ldo((const double*)RM(g, a.s_f), idx) : lw(a.s_f);
           }
           if constexpr (INC) {
             if (st == 4) {
@@ -1096,7 +1154,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
                 a2v[g] = ldo((const double*)RM(g, a.s_a2), idx);
                 a3m[g] = ldo(INM(g), idx);
               } else {
-                a2v[g] = (double)ldo((const TS*)RS(g, a.s_a2), idx);
+                a2v[g] = lw(a.s_a2);
               }
             }
           }
@@ -1133,6 +1191,37 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
             } else if (!INC) {
               sto(fr, idx, fo + inc);
             }
+          } else if (MQ && f < 7) {
+            // fixed-point metric: the RK4 value in FP64, stored as the
+            // field's word; phi_new and gn are the stored values
+            int* __restrict__ o = (int*)RS(f, a.s_out);
+            const double sc = a.msc[f < 7 ? f : 0], iv = a.minv[f < 7 ? f : 0];
+            if (st == 0) {
+              sto(o, idx, mq_word(cc, iv));
+              continue;
+            }
+            const double p = PF ? lpf(f, 0) : pv[f];
+            double fo = 0.0;
+            if constexpr (!INC) {
+              if (st >= 2) fo = PF ? lpf(f, 1) : qv[f];
+            } else if (st == 4) {
+              const double a1 = PF ? lpf(f, 1) : qv[f];
+              const double a3 = (double)gw[f < 7 ? f : 0] * sc;
+              fo = ((a1 - p) + 2.0 * (a2v[f] - p) + (a3 - p)) * (1.0 / 3.0);
+            }
+            const double inc = a.cf * cc;
+            const double nd = st == 4 ? fo + (inc + p) : p + a.ca * cc;
+            if (fabs(nd * iv) > MQ_WMAX) atomicAdd(a.mq_sat, 1u);
+            const int w = mq_word(nd, iv);
+            const double nv = (double)w * sc;
+            if (f == PHI) {
+              phi_new = nv;
+              if (st == 4 && nan_as_float(nd)) atomicAdd(a.nan_count, 1u);
+            } else {
+              gn[f < 6 ? f : 0] = nv;
+            }
+            sto(o, idx, w);
+            if (st != 4 && !INC) sto((int*)RS(f, a.s_f), idx, mq_word(fo + inc, iv));
           } else {
             TS* __restrict__ o = RS(f, a.s_out);
             if (st == 0) {
@@ -1184,7 +1273,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         const double trA = i11 * An[0] + i22 * An[3] + i33 * An[5] + 2.0 * (i12 * An[1] + i13 * An[2] + i23 * An[4]);
     #pragma unroll
         for (int s = 0; s < 6; ++s) {
-          if constexpr (M32) sto(RS(G11 + s, a.s_norm_out), idx, (TS)g[s]);
+          if constexpr (MQ) {
+            if (fabs(g[s] * a.minv[s]) > MQ_WMAX) atomicAdd(a.mq_sat, 1u);
+            sto((int*)RS(G11 + s, a.s_norm_out), idx, mq_word(g[s], a.minv[s]));
+          }
+          else if constexpr (M32) sto(RS(G11 + s, a.s_norm_out), idx, (TS)g[s]);
           else sto(RM(G11 + s, a.s_norm_out), idx, g[s]);
           sto(RS(A11 + s, a.s_norm_out), idx,
               (TS)((An[s] - (1.0 / 3.0) * ((diag6(s) ? 1.0 : 0.0) + g[s]) * trA) / (1.0 - tn)));

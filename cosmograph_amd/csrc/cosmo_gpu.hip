@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cosmo_gpu.h"
@@ -32,20 +33,34 @@ const char* kFieldNames[NF] = {"DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIF
 // Register table: slot pointers of every field (DIFFphi always FP64).
 struct RegTable {
   void* r[NF][4];
+  double ms[7], mi[7];  // fixed-point metric: scale 2^e and 2^-e per field
+  unsigned* sat;        // ... and its count of saturated words
 };
 
 // the conformal metric (DIFFgamma_ij, DIFFphi) has its own storage type MT:
-// FP64 in the drop-in and the split mode, FP32 in the all-FP32 mode
+// FP64 in the drop-in and the split mode, FP32 in the all-FP32 mode, int
+// (32-bit fixed point, scaled per field) in the fixed-point mode
 __device__ __host__ inline bool is_metric(int f) { return f <= G33 || f == PHI; }
 
 template <typename TS, typename MT>
 __device__ __forceinline__ double ld(const RegTable& t, int f, int s, long x) {
-  return is_metric(f) ? (double)((const MT*)t.r[f][s])[x] : (double)((const TS*)t.r[f][s])[x];
+  if (is_metric(f)) {
+    if constexpr (std::is_same<MT, int>::value) return (double)((const int*)t.r[f][s])[x] * t.ms[f < 7 ? f : 0];
+    return (double)((const MT*)t.r[f][s])[x];
+  }
+  return (double)((const TS*)t.r[f][s])[x];
 }
 template <typename TS, typename MT>
 __device__ __forceinline__ void st(const RegTable& t, int f, int s, long x, double v) {
-  if (is_metric(f)) ((MT*)t.r[f][s])[x] = (MT)v;
-  else ((TS*)t.r[f][s])[x] = (TS)v;
+  if (is_metric(f)) {
+    if constexpr (std::is_same<MT, int>::value) {
+      if (fabs(v * t.mi[f < 7 ? f : 0]) > MQ_WMAX) atomicAdd(t.sat, 1u);
+      ((int*)t.r[f][s])[x] = mq_word(v, t.mi[f < 7 ? f : 0]);
+    }
+    else ((MT*)t.r[f][s])[x] = (MT)v;
+  } else {
+    ((TS*)t.r[f][s])[x] = (TS)v;
+  }
 }
 
 // Elementwise kernels run 256 threads per block; threads past the end keep
@@ -130,24 +145,27 @@ __global__ void k_finalize(RegTable t, int rp, int rc, int rf, int ra, long n, i
 
 // setExtraFieldData after a fused step's K4Finalize: the new _p, and the
 // stage-4 input register for that stage's DIFFr
+// (phi words times ms: 1 except for the fixed-point metric)
 template <typename TS, typename MT>
 __global__ void k_ext_post(const MT* __restrict__ phi, const TS* __restrict__ K,
-                           const MT* __restrict__ phi_in, long n, double* ext_part, SrcArgs src, double phiF) {
+                           const MT* __restrict__ phi_in, long n, double* ext_part, SrcArgs src, double phiF,
+                           double ms) {
   const long xg = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = xg < n;
   const long x = valid ? xg : 0;
-  block_ext_put(ext_part, blockIdx.x, valid, (double)phi[x], (double)K[x], diff_r<TS>(src, (double)phi_in[x], x), phiF);
+  block_ext_put(ext_part, blockIdx.x, valid, (double)phi[x] * ms, (double)K[x],
+                diff_r<TS>(src, (double)phi_in[x] * ms, x), phiF);
 }
 
 // setExtraFieldData over a register alone (DIFFr part left out): used when
 // the averages are needed before any stage has produced them
 template <typename TS, typename MT>
 __global__ void k_ext_state(const MT* __restrict__ phi, const TS* __restrict__ K, long n, double* ext_part,
-                            double phiF) {
+                            double phiF, double ms) {
   const long xg = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = xg < n;
   const long x = valid ? xg : 0;
-  block_ext_put(ext_part, blockIdx.x, valid, (double)phi[x], (double)K[x], 0.0, phiF);
+  block_ext_put(ext_part, blockIdx.x, valid, (double)phi[x] * ms, (double)K[x], 0.0, phiF);
 }
 
 // Deterministic combine of partial rows: block b reduces the contiguous
@@ -448,6 +466,8 @@ static RegTable reg_table(cg_ctx* c) {
   RegTable t;
   for (int f = 0; f < NF; ++f)
     for (int s = 0; s < 4; ++s) t.r[f][s] = reg_ptr(c, f, s);
+  mq_scales(c, t.ms, t.mi);
+  t.sat = c->mq_sat;
   return t;
 }
 
@@ -491,10 +511,12 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   // the largest element offset times 8 must stay below 2^32.
   if ((long long)p.nx * p.ny * p.nz > (1ll << 29)) return CG_ERR_ARG;
   if (p.stencil_order != 2 && p.stencil_order != 4 && p.stencil_order != 6 && p.stencil_order != 8) return CG_ERR_ARG;
-  if (p.precision != CG_FP64 && p.precision != CG_FP32_SPLIT && p.precision != CG_FP32_ALL) return CG_ERR_ARG;
-  // all-FP32 storage: BSSN with static dust or vacuum, the shift-free gauges
-  // the stage kernel forms by itself, on single grids
-  if (p.precision == CG_FP32_ALL &&
+  if (p.precision != CG_FP64 && p.precision != CG_FP32_SPLIT && p.precision != CG_FP32_ALL &&
+      p.precision != CG_FP32_FIX)
+    return CG_ERR_ARG;
+  // all-FP32 and fixed-point storage: BSSN with static dust or vacuum, the
+  // shift-free gauges the stage kernel forms by itself, on single grids
+  if ((p.precision == CG_FP32_ALL || p.precision == CG_FP32_FIX) &&
       (p.matter == CG_MATTER_DUST || p.shift != CG_SHIFT_NONE || p.lapse == LAPSE_TEST_K_DRIVER))
     return CG_ERR_ARG;
   if (p.matter != CG_MATTER_NONE && p.matter != CG_MATTER_STATIC && p.matter != CG_MATTER_DUST) return CG_ERR_ARG;
@@ -506,7 +528,8 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   c->p = p;
   c->prec = p.precision;
   c->ts = p.precision == CG_FP64 ? 8 : 4;
-  c->met32 = p.precision == CG_FP32_ALL;
+  c->met32 = p.precision == CG_FP32_ALL || p.precision == CG_FP32_FIX;
+  c->mq = p.precision == CG_FP32_FIX;
   c->R = p.stencil_order / 2;
   c->H = halo_for(p.stencil_order, p.ko_coeff);
   c->lambda = p.lambda;
@@ -538,6 +561,10 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   HIPCHK(c, hipMemsetAsync(c->DIFFD, 0, (size_t)pad * c->ts, c->stream));
   HIPCHK(c, hipMalloc(&c->nan_dev, sizeof(unsigned)));
   HIPCHK(c, hipMemsetAsync(c->nan_dev, 0, sizeof(unsigned), c->stream));
+  if (c->mq) {
+    HIPCHK(c, hipMalloc((void**)&c->mq_sat, sizeof(unsigned)));
+    HIPCHK(c, hipMemsetAsync(c->mq_sat, 0, sizeof(unsigned), c->stream));
+  }
   // partial rows: one per tile of any compiled stage variant, or one per
   // 256-point block of the elementwise kernels
   long rows = (c->npts + 255) / 256;
@@ -586,10 +613,23 @@ int cg_destroy(cg_ctx* c) {
   if (c->flux) (void)hipFree(c->flux);
   if (c->rho_src) (void)hipFree(c->rho_src);
   if (c->xreg) (void)hipFree(c->xreg);
+  if (c->mq_sat) (void)hipFree(c->mq_sat);
+  mq_free(c);
   bardeen_free(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CG_OK;
+}
+
+// a device pointer that is a register slot of a fixed-point metric field:
+// its field and slot
+static bool mq_register(const cg_ctx* c, const void* d, int* f, int* slot) {
+  if (!c->mq || !d || d < c->regs) return false;
+  const long off = ((const char*)d - (const char*)c->regs) / (long)c->ts;
+  if (off >= 7 * c->fstride) return false;
+  *f = (int)(off / c->fstride);
+  *slot = (int)((off % c->fstride) / c->rstride);
+  return true;
 }
 
 // resolve a reference field-map key to a device pointer and element size
@@ -664,7 +704,13 @@ static void* resolve(cg_ctx* c, const char* name, bool for_write, int* err, size
 
 namespace cg {
 const void* field_ptr(cg_ctx* c, const char* name, size_t* esz, int* err) {
-  return resolve(c, name, false, err, esz);
+  const void* d = resolve(c, name, false, err, esz);
+  int f, s;
+  if (mq_register(c, d, &f, &s)) {  // words, not values: read through cg_field_download
+    *err = 5;
+    return nullptr;
+  }
+  return d;
 }
 }  // namespace cg
 
@@ -681,6 +727,13 @@ int cg_field_upload(cg_ctx* c, const char* name, const double* host, size_t n) {
   if (!strncmp(name, "DIFFalpha_", 10))
     for (size_t x = 0; x < n; ++x)
       if (host[x] != 0.0) { c->alpha_nonzero = true; break; }
+  int mf, ms;
+  if (mq_register(c, d, &mf, &ms)) {
+    // fixed-point metric: the FP64 values through the device, scaled there
+    if (!c->mq_tmp) HIPCHK(c, hipMalloc((void**)&c->mq_tmp, (size_t)c->rstride * sizeof(double)));
+    HIPCHK(c, hipMemcpyAsync(c->mq_tmp, host, n * 8, hipMemcpyHostToDevice, c->stream));
+    return mq_store(c, mf, ms, c->mq_tmp);
+  }
   if (esz == 8) {
     HIPCHK(c, hipMemcpyAsync(d, host, n * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -702,6 +755,14 @@ int cg_field_download(cg_ctx* c, const char* name, double* host, size_t n) {
   if (!d) return fail(c, e >= 3 ? CG_ERR_STATE : CG_ERR_ARG,
                       e == 3 ? std::string(name) + " is filled by cg_constraints"
                       : e == 4 ? std::string(name) + " is filled by cg_bardeen" : std::string("unknown field ") + name);
+  int mf, ms;
+  if (mq_register(c, d, &mf, &ms)) {
+    if (!c->mq_tmp) HIPCHK(c, hipMalloc((void**)&c->mq_tmp, (size_t)c->rstride * sizeof(double)));
+    if (int rc = mq_load(c, mf, ms, c->mq_tmp)) return rc;
+    HIPCHK(c, hipMemcpyAsync(host, c->mq_tmp, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CG_OK;
+  }
   if (esz == 8) {
     HIPCHK(c, hipMemcpyAsync(host, d, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -782,6 +843,8 @@ StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int
   // regs + (f - 7)*4*rstride addresses the metric fields too
   a.regs = c->met32 ? (void*)((char*)c->regs + 7 * c->fstride * (long)c->ts) : c->regs;
   a.rstride = c->rstride;
+  mq_scales(c, a.msc, a.minv);
+  a.mq_sat = c->mq_sat;
   a.s_in = rin; a.s_p = rp; a.s_f = rf; a.s_out = rout;
   a.s_in_norm = -1; a.s_norm_out = -1;
   a.incr = 0; a.s_a2 = -1;
@@ -873,15 +936,20 @@ static int ext_from_state(cg_ctx* c) {
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
   const void* phi = reg_ptr(c, PHI, c->sP);
+  double ms[7], mi[7];
+  mq_scales(c, ms, mi);
   if (c->prec == CG_FP64)
     hipLaunchKernelGGL((k_ext_state<double, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)phi,
-                       (const double*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi);
+                       (const double*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi, 1.0);
+  else if (c->mq)
+    hipLaunchKernelGGL((k_ext_state<float, int>), dim3(nb), dim3(bs), 0, c->stream, (const int*)phi,
+                       (const float*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi, ms[PHI]);
   else if (c->met32)
     hipLaunchKernelGGL((k_ext_state<float, float>), dim3(nb), dim3(bs), 0, c->stream, (const float*)phi,
-                       (const float*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi);
+                       (const float*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi, 1.0);
   else
     hipLaunchKernelGGL((k_ext_state<float, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)phi,
-                       (const float*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi);
+                       (const float*)reg_ptr(c, KK, c->sP), c->npts, c->ext_part, c->frw.g_phi, 1.0);
   HIPCHK(c, hipGetLastError());
   int rc = ext_combine(c, nb, true);
   if (!rc) c->ext_fresh = true;
@@ -897,18 +965,24 @@ int ext_resolve(cg_ctx* c) {
   const long nb = (c->npts + bs - 1) / bs;
   const void* phi = reg_ptr(c, PHI, c->sP);
   const void* phi_in = reg_ptr(c, PHI, c->ext_lazy_in);
+  double ms[7], mi[7];
+  mq_scales(c, ms, mi);
   if (c->prec == CG_FP64)
     hipLaunchKernelGGL((k_ext_post<double, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)phi,
                        (const double*)reg_ptr(c, KK, c->sP), (const double*)phi_in, c->npts, c->ext_part,
-                       c->ext_lazy_src, c->ext_lazy_phiF);
+                       c->ext_lazy_src, c->ext_lazy_phiF, 1.0);
+  else if (c->mq)
+    hipLaunchKernelGGL((k_ext_post<float, int>), dim3(nb), dim3(bs), 0, c->stream, (const int*)phi,
+                       (const float*)reg_ptr(c, KK, c->sP), (const int*)phi_in, c->npts, c->ext_part,
+                       c->ext_lazy_src, c->ext_lazy_phiF, ms[PHI]);
   else if (c->met32)
     hipLaunchKernelGGL((k_ext_post<float, float>), dim3(nb), dim3(bs), 0, c->stream, (const float*)phi,
                        (const float*)reg_ptr(c, KK, c->sP), (const float*)phi_in, c->npts, c->ext_part,
-                       c->ext_lazy_src, c->ext_lazy_phiF);
+                       c->ext_lazy_src, c->ext_lazy_phiF, 1.0);
   else
     hipLaunchKernelGGL((k_ext_post<float, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)phi,
                        (const float*)reg_ptr(c, KK, c->sP), (const double*)phi_in, c->npts, c->ext_part,
-                       c->ext_lazy_src, c->ext_lazy_phiF);
+                       c->ext_lazy_src, c->ext_lazy_phiF, 1.0);
   HIPCHK(c, hipGetLastError());
   return ext_combine(c, nb, false);
 }
@@ -966,6 +1040,7 @@ int cg_step_init(cg_ctx* c) {
   }
   if (int rc = gauge_step_init(c)) return rc;
   return c->prec == CG_FP64 ? step_init_t<double, double>(c)
+         : c->mq             ? step_init_t<float, int>(c)
          : c->met32          ? step_init_t<float, float>(c)
                              : step_init_t<float, double>(c);
 }
@@ -973,6 +1048,8 @@ int cg_step_init(cg_ctx* c) {
 int cg_rk_evolve(cg_ctx* c) {
   if (!c) return CG_ERR_ARG;
   if (c->a_is_p) return fail(c, CG_ERR_STATE, "RKEvolve before stepInit");
+  if (c->mq)
+    return fail(c, CG_ERR_STATE, "the fixed-point metric holds states, not right-hand sides: step with cg_rk4_steps");
   StageArgs a = make_args(c, 0, c->sA, c->sP, c->sF, c->sC, 0);
   if (c->dreg) {
     // Dust::RKEvolve (dust.cc:160-174) with the sources addBSSNSrc formed
@@ -1015,6 +1092,8 @@ static int finalize_t(cg_ctx* c, int n, const SrcArgs& src) {
 int cg_k_finalize(cg_ctx* c, int n) {
   if (!c || n < 1 || n > 4) return CG_ERR_ARG;
   if (c->a_is_p) return fail(c, CG_ERR_STATE, "K-finalize before stepInit");
+  if (c->mq)
+    return fail(c, CG_ERR_STATE, "the fixed-point metric holds states, not right-hand sides: step with cg_rk4_steps");
   c->ext_lazy = false;
   // the finished stage's sources were formed with the FRW values before this step
   const SrcArgs src{c->DIFFD, c->p.matter, c->frw.g_phi, c->frw.g_rho, c->lambda, c->rho_src};
@@ -1033,12 +1112,14 @@ int cg_k_finalize(cg_ctx* c, int n) {
   if (int rc = gauge_finalize(c, n)) return rc;
   int rc = c->prec == CG_FP64 ? finalize_t<double, double>(c, n, src)
            : c->met32          ? finalize_t<float, float>(c, n, src)
-                               : finalize_t<float, double>(c, n, src);
+                               : finalize_t<float, double>(c, n, src);  // (not the fixed-point mode: rejected above)
   if (rc) return rc;
   std::swap(c->sA, c->sC);  // swap_a_c
   c->ext_fresh = (n == 4);
   return CG_OK;
 }
+
+constexpr int kMqCheck = 4;
 
 int cg_rk4_steps(cg_ctx* c, int nsteps) {
   if (!c || nsteps < 0) return CG_ERR_ARG;
@@ -1106,6 +1187,9 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
   for (int s = 0; s < nsteps; ++s) {
     int rc;
     c->sP = 0; c->sF = 1;
+    // fixed-point metric: exponents checked every kMqCheck steps (a field's
+    // words stay below 2^MQ_BITS; four times that saturates)
+    if (c->mq && s % kMqCheck == 0 && (rc = mq_check(c))) return rc;
     const bool pre = !ko_norm && !c->dreg && c->p.normalize_metric && s + 1 < nsteps && c->var &&
                      c->dma_allowed && c->dma_pick && dma_fits(c, *c->var);
     if (incr) {
@@ -1180,6 +1264,13 @@ int cg_num_nans(cg_ctx* c, const char* name, long long* count) {
   size_t esz;
   void* d = resolve(c, name, false, &e, &esz);
   if (!d) return fail(c, CG_ERR_ARG, std::string("unknown field ") + (name ? name : ""));
+  int mf, ms;
+  if (mq_register(c, d, &mf, &ms)) {
+    // fixed-point words hold no NaN: the stage kernels count a NaN DIFFphi
+    // before rounding it (cg_rk4_steps reports it)
+    *count = 0;
+    return CG_OK;
+  }
   HIPCHK(c, hipMemsetAsync(c->nan_cnt, 0, sizeof(unsigned long long), c->stream));
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
@@ -1232,6 +1323,9 @@ int cg::cons_input(cg_ctx* c, int* rin_out) {
     const long nbk = (c->npts + bs - 1) / bs;
     if (c->prec == CG_FP64)
       hipLaunchKernelGGL((k_step_init<double, double>), dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin,
+                         c->sF, c->npts, c->p.normalize_metric, (double*)nullptr, 0.0);
+    else if (c->mq)
+      hipLaunchKernelGGL((k_step_init<float, int>), dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin,
                          c->sF, c->npts, c->p.normalize_metric, (double*)nullptr, 0.0);
     else if (c->met32)
       hipLaunchKernelGGL((k_step_init<float, float>), dim3(nbk), dim3(bs), 0, c->stream, reg_table(c), c->sP, rin,
@@ -1413,9 +1507,11 @@ int cg_field_strip(cg_ctx* c, const char* name, int axis, long n1, long n2, doub
   if ((long)n != ext[axis - 1]) return fail(c, CG_ERR_ARG, "a strip holds the grid's extent along its axis");
   int e;
   size_t esz;
-  const void* d = resolve(c, name, false, &e, &esz);
-  if (!d) return fail(c, e == 3 ? CG_ERR_STATE : CG_ERR_ARG,
-                      e == 3 ? std::string(name) + " is filled by cg_constraints" : std::string("unknown field ") + name);
+  const void* d = field_ptr(c, name, &esz, &e);
+  if (!d) return fail(c, e == 3 || e == 5 ? CG_ERR_STATE : CG_ERR_ARG,
+                      e == 3   ? std::string(name) + " is filled by cg_constraints"
+                      : e == 5 ? std::string(name) + " is stored in fixed point: read it with cg_field_download"
+                               : std::string("unknown field ") + name);
   // field(i, n1, n2), field(n1, j, n2), field(n1, n2, k) with the offsets
   // wrapped as CosmoArray::idx wraps them (utils/Array.h:149-159)
   long base, run;
@@ -1592,6 +1688,7 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
             a.incr = form;
             a.s_a2 = 3;
             a.nan_count = nan_scratch;
+            if (c->mq) a.mq_sat = nan_scratch;  // timing launches count nothing real
             int rc = launch(c, a);
             if (rc) return rc;
             HIPCHK(c, hipEventRecord(e0, c->stream));

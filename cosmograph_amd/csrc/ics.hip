@@ -291,7 +291,14 @@ int cg_ics_static_dust(cg_ctx* c, const double* modes, size_t n_modes, double rm
   void* D = fluid ? (void*)((char*)c->dreg + (long)c->sP * c->rstride * (long)c->ts) : c->DIFFD;
   void* kk = fluid && !bg ? reg_ptr(c, KK, c->sP) : nullptr;
   const double sub = fluid ? 0.0 : rho_bar;
-  if (c->ts == 8)
+  if (c->mq) {
+    // fixed-point metric: phi in FP64 first, then words at a scale that fits it
+    if (!c->mq_tmp) HIPCHK(c, hipMalloc((void**)&c->mq_tmp, (size_t)c->rstride * sizeof(double)));
+    hipLaunchKernelGGL((k_out<float, double>), dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, sub, n,
+                       c->mq_tmp, (float*)D, (float*)kk, K0);
+    HIPCHK(c, hipGetLastError());
+    if (int rc = mq_store(c, PHI, c->sP, c->mq_tmp)) return rc;
+  } else if (c->ts == 8)
     hipLaunchKernelGGL((k_out<double, double>), dim3(kB), dim3(kT), 0, s, w.u, w.drho, cs, rho_bar, sub, n,
                        (double*)phi, (double*)D, (double*)kk, K0);
   else if (c->met32)
@@ -371,8 +378,10 @@ int cg_power_spectrum(cg_ctx* c, const char* name, double* out, int nbins) {
   size_t esz;
   int e;
   const void* d = field_ptr(c, name, &esz, &e);
-  if (!d) return fail(c, e == 3 ? CG_ERR_STATE : CG_ERR_ARG,
-                      e == 3 ? std::string(name) + " is filled by cg_constraints" : std::string("unknown field ") + name);
+  if (!d) return fail(c, e == 3 || e == 5 ? CG_ERR_STATE : CG_ERR_ARG,
+                      e == 3   ? std::string(name) + " is filled by cg_constraints"
+                      : e == 5 ? std::string(name) + " is stored in fixed point: read it with cg_field_download"
+                               : std::string("unknown field ") + name);
   const long n = c->npts, nspec = (long)nx * ny * (nz / 2 + 1);
   constexpr int kBlocks = 512;
   hipStream_t s = c->stream;

@@ -122,7 +122,13 @@ struct cg_ctx {
   bool keep_accumulator = false;          // fused steps with the reference's _f accumulator (cg_set_fused_accumulator)
   bool incr_pick = true;                  // the autotuner's choice between the two RK4 forms
   bool alpha_nonzero = false;             // a non-zero DIFFalpha was uploaded
-  bool met32 = false;                     // all-FP32 storage: the metric in regs too (CG_FP32_ALL)
+  bool met32 = false;                     // all-FP32 storage: the metric in regs too (CG_FP32_ALL, CG_FP32_FIX)
+  // fixed-point metric (CG_FP32_FIX): field f < 7 holds words w, value w * 2^mq_exp[f]
+  bool mq = false;
+  int mq_exp[7] = {-40, -40, -40, -40, -40, -40, -40};
+  double* mq_tmp = nullptr;               // FP64 staging of one field (lazy)
+  unsigned* mq_part = nullptr;            // per-block maxima (lazy)
+  unsigned* mq_sat = nullptr;             // count of saturated words written by the stage kernels
   double* cons = nullptr;                 // [CONS_N][pad] per-point constraint values (lazy)
   double* cons_part = nullptr;            // reduction partials
   bool cons_valid = false;                // ricci_a / AijAij_a filled
@@ -204,6 +210,14 @@ int cons_input(cg_ctx* c, int* rin_out);
 // that state, the gauge pass of the shift build)
 int rhs_at(cg_ctx* c, int rin, int rout);
 void bardeen_free(cg_ctx* c);
+// fixed-point metric (CG_FP32_FIX): scales 2^e, 2^-e of the seven fields;
+// FP64 device values x into words of field f, slot (raising e to fit);
+// words back to FP64; the exponent check of cg_rk4_steps
+void mq_scales(const cg_ctx* c, double* sc, double* inv);
+int mq_store(cg_ctx* c, int f, int slot, const double* x);
+int mq_load(cg_ctx* c, int f, int slot, double* x);
+int mq_check(cg_ctx* c);
+void mq_free(cg_ctx* c);
 // a "Bardeen_<name>" array of the last cg_bardeen (err 1: unknown name, 4: not computed)
 const double* bardeen_field(const cg_ctx* c, const std::string& name, int* err);
 // KO at orders 2/8 with metric normalisation materialises stepInit's _a

@@ -66,6 +66,72 @@ __device__ __forceinline__ T fdm(const T* s, int c, int sa, int sb) {
   return acc;
 }
 
+// The same stencils on a tile of 32-bit fixed-point words (the conformal
+// metric of the fixed-point storage mode).  Words stay below 2^30 in
+// magnitude, so the sum or difference of two words is exact in int32; each
+// such pair is converted to FP64 once (exactly) and the weighted sums are
+// FP64.  The caller multiplies by the field's scale.
+template <int R>
+__device__ __forceinline__ double fd1q(const int* s, int c, int st) {
+  double acc = c1<R>(1) * (double)(s[c + st] - s[c - st]);
+#pragma unroll
+  for (int m = 2; m <= R; ++m) acc += c1<R>(m) * (double)(s[c + m * st] - s[c - m * st]);
+  return acc;
+}
+
+template <int R>
+__device__ __forceinline__ double fd2q(const int* s, int c, int st) {
+  double acc = c2<R>(0) * (double)s[c];
+#pragma unroll
+  for (int m = 1; m <= R; ++m) acc += c2<R>(m) * (double)(s[c + m * st] + s[c - m * st]);
+  return acc;
+}
+
+template <int R>
+__device__ __forceinline__ double fdmq(const int* s, int c, int sa, int sb) {
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 1; m <= R; ++m) {
+    const int p = m * (sa + sb), q = m * (sa - sb);
+    acc += cm<R>(m) * ((double)(s[c + p] - s[c + q]) - (double)(s[c - q] - s[c - p]));
+  }
+  return acc;
+}
+
+// FP32 versions for the FP32 algebra: first differences q(+m) - q(-m),
+// second differences about the centre q(+m) + q(-m) - 2 q(0) (the stencil
+// weights sum to zero) and the mixed cross differences are formed exactly in
+// int32 and only then converted, so the FP32 arithmetic keeps the relative
+// precision of the derivative itself; FP32 storage of the values would
+// instead lose it to the cancellation of neighbouring values.
+template <int R>
+__device__ __forceinline__ float fd1qf(const int* s, int c, int st) {
+  float acc = float(c1<R>(1)) * (float)(s[c + st] - s[c - st]);
+#pragma unroll
+  for (int m = 2; m <= R; ++m) acc += float(c1<R>(m)) * (float)(s[c + m * st] - s[c - m * st]);
+  return acc;
+}
+
+template <int R>
+__device__ __forceinline__ float fd2qf(const int* s, int c, int st) {
+  const int q0 = s[c];
+  float acc = float(c2<R>(1)) * (float)((s[c + st] - q0) + (s[c - st] - q0));
+#pragma unroll
+  for (int m = 2; m <= R; ++m) acc += float(c2<R>(m)) * (float)((s[c + m * st] - q0) + (s[c - m * st] - q0));
+  return acc;
+}
+
+template <int R>
+__device__ __forceinline__ float fdmqf(const int* s, int c, int sa, int sb) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int m = 1; m <= R; ++m) {
+    const int p = m * (sa + sb), q = m * (sa - sb);
+    acc += float(cm<R>(m)) * (float)((s[c + p] - s[c + q]) - (s[c - q] - s[c - p]));
+  }
+  return acc;
+}
+
 // Kreiss-Oliger stencil sum (without prefactor) over the three axes.
 // order 2: [1,-4,6,-4,1];  order 8: [1,-10,45,-120,210,-252,...]
 template <int ORDER, typename T>
@@ -88,6 +154,25 @@ __device__ __forceinline__ T ko_sum(const T* s, int c, int si, int sj, int sk) {
     return acc;
   } else {
     return T(0);
+  }
+}
+
+// ko_sum on a fixed-point tile, in FP64
+template <int ORDER>
+__device__ __forceinline__ double ko_sumq(const int* s, int c, int si, int sj, int sk) {
+  if constexpr (ORDER == 2 || ORDER == 8) {
+    constexpr int NM = ORDER == 2 ? 2 : 5;
+    constexpr double w2[3] = {6.0, -4.0, 1.0};
+    constexpr double w8[6] = {-252.0, 210.0, -120.0, 45.0, -10.0, 1.0};
+    auto w = [&](int m) { return ORDER == 2 ? w2[m] : w8[m]; };
+    double acc = 3.0 * w(0) * (double)s[c];
+#pragma unroll
+    for (int m = 1; m <= NM; ++m)
+      acc += w(m) * (((double)(s[c + m * si] + s[c - m * si]) + (double)(s[c + m * sj] + s[c - m * sj])) +
+                     (double)(s[c + m * sk] + s[c - m * sk]));
+    return acc;
+  } else {
+    return 0.0;
   }
 }
 

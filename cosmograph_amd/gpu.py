@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 FIELDS = ["DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIFFgamma22", "DIFFgamma23",
           "DIFFgamma33", "DIFFphi", "A11", "A12", "A13", "A22", "A23", "A33", "DIFFK",
           "Gamma1", "Gamma2", "Gamma3", "DIFFalpha"]
-PRECISION = {"fp64": 0, "fp32_split": 1, "fp32_all": 2}
+PRECISION = {"fp64": 0, "fp32_split": 1, "fp32_all": 2, "fp32_fix": 3}
 MATTER = {"none": 0, "vacuum": 0, "static": 1, "dust": 2}
 LAPSE = {"": 0, "Static": 0, "Harmonic": 1, "ConformalFLRW": 2, "Anharmonic": 3, "OnePlusLog": 4,
          "ConformalSync": 5, "AwAGaugeWave": 6, "AwAShiftedWave": 7, "DampedWave": 8, "TestAijDriverLapse": 9,

@@ -131,8 +131,9 @@ cg_params params_from_config(const ConfigParser& cfg) {
   if (prec == "fp64") p.precision = CG_FP64;
   else if (prec == "fp32_split") p.precision = CG_FP32_SPLIT;
   else if (prec == "fp32_all") p.precision = CG_FP32_ALL;
+  else if (prec == "fp32_fix") p.precision = CG_FP32_FIX;
   else {
-    std::cerr << "Error - gpu_precision must be fp64, fp32_split or fp32_all.\n";
+    std::cerr << "Error - gpu_precision must be fp64, fp32_split, fp32_all or fp32_fix.\n";
     throw -1;
   }
   p.device = std::stoi(cfg("gpu_device", "0"));

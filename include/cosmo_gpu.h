@@ -30,9 +30,14 @@ typedef struct cg_ctx cg_ctx;
 enum { CG_OK = 0, CG_ERR_ARG = -1, CG_ERR_HIP = -2, CG_ERR_STATE = -3, CG_ERR_NAN = -10 };
 /* CG_FP32_SPLIT: A_ij, K, Gamma^i, alpha, D in FP32, the conformal metric
  * (DIFFgamma_ij, DIFFphi) in FP64.  CG_FP32_ALL: every BSSN field in FP32,
- * the metric included (BSSN + static dust or vacuum, single grids).  Both
+ * the metric included (BSSN + static dust or vacuum, single grids).
+ * CG_FP32_FIX: as CG_FP32_ALL, but the conformal metric in 32-bit fixed
+ * point (value = word x 2^e, one power-of-two scale per field, raised by
+ * cg_rk4_steps as the field grows) and differentiated in FP64; steps run
+ * through cg_rk4_steps only (cg_rk_evolve / cg_k_finalize return
+ * CG_ERR_STATE: a right-hand side does not fit the field's scale).  All
  * keep the RK4 accumulation, the FRW background and the reductions in FP64. */
-enum { CG_FP64 = 0, CG_FP32_SPLIT = 1, CG_FP32_ALL = 2 };
+enum { CG_FP64 = 0, CG_FP32_SPLIT = 1, CG_FP32_ALL = 2, CG_FP32_FIX = 3 };
 enum { CG_MATTER_NONE = 0, CG_MATTER_STATIC = 1, CG_MATTER_DUST = 2 };
 /* the lapse gauges of BSSNGaugeHandler (BSSNGaugeHandler.h:96-111) */
 enum {

@@ -29,6 +29,11 @@ from test_gpu_c2 import check_constraints
 pytestmark = pytest.mark.gpu
 
 H_TOL = 1e-4
+# The fixed-point metric rounds the initial DIFFphi to words of 2^-28 of its
+# largest value, which adds about 1e-9 (quadrature) to the constraint-solved
+# initial residual H L2 ~ 9e-6 (measured: +1.4e-4 of it); at step 0 the bar is
+# 5e-4 of the oracle's value, at every later output step H_TOL.
+H_TOL0_FIX = 5e-4
 
 
 @pytest.fixture(scope="module")
@@ -73,24 +78,35 @@ def test_c3_fp64_parity(gpu, digest):
             assert errs[f][k] <= 1e-12, (f, k, errs[f][k])
 
 
-def test_c3_fp32_split_gate(gpu, digest):
+@pytest.mark.parametrize("prec", ["fp32_split", "fp32_fix"])
+def test_c3_fp32_split_gate(gpu, digest, prec):
+    """The same gate for both 4-byte-perturbation modes: the split mode (FP64
+    conformal metric) and the fixed-point metric mode (the metric in 32-bit
+    words, 4 bytes per value like every other field)."""
     from cosmograph_amd.ics import static_dust_ics
     d, sample = digest
     cfg = d["config"]
-    _say(f"case c3 fp32_split n={cfg['n']} order={cfg['order']} steps={cfg['steps']}")
-    g = digests.gpu_context(gpu, d, "fp32_split", static_dust_ics)
+    _say(f"case c3 {prec} n={cfg['n']} order={cfg['order']} steps={cfg['steps']}")
+    g = digests.gpu_context(gpu, d, prec, static_dust_ics)
     excess = {"L2": 0.0, "Linf": 0.0}
+    bad = []
     for rec in d["constraints"]:
         if rec["step"]:
             g.step(cfg["every"])
         got = g.setConstraintCalcs()["H"]
         want = [digests.fhex(x) for x in rec["H"]]
+        tol = H_TOL0_FIX if (prec == "fp32_fix" and rec["step"] == 0) else H_TOL
         for key, q in (("L2", 7), ("Linf", 2)):
             r = got[q] / want[q] - 1.0
-            excess[key] = max(excess[key], r)
-            assert got[q] <= want[q] * (1.0 + H_TOL), (rec["step"], key, got[q], want[q])
+            if rec["step"]:
+                excess[key] = max(excess[key], r)
+            else:
+                _say(f"c3 {prec} step 0 H {key} excess {r:.2e}")
+            if got[q] > want[q] * (1.0 + tol):
+                bad.append((rec["step"], key, r))
+    assert not bad, bad
     errs = digests.sample_errors(g, d, sample)
-    _say(f"c3 fp32_split H excess over the oracle: L2 {excess['L2']:.2e}, Linf {excess['Linf']:.2e}")
-    _say("c3 fp32_split field errors (sample rel L2): " + " ".join(f"{f} {errs[f]['sample']:.1e}" for f in FIELDS))
+    _say(f"c3 {prec} H excess over the oracle after step 0: L2 {excess['L2']:.2e}, Linf {excess['Linf']:.2e}")
+    _say(f"c3 {prec} field errors (sample rel L2): " + " ".join(f"{f} {errs[f]['sample']:.1e}" for f in FIELDS))
     for f in FIELDS:
         assert errs[f]["sample"] <= 1e-5, (f, errs[f]["sample"])
