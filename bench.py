@@ -9,10 +9,14 @@ rms 1e-2 (seed 9), constraint-solved metric.  A "step" is one whole RK4 step
 
     python bench.py [--steps K] [--warmup W] [--n 256] [--order 6]
 
-Prints ONE JSON line.  `value` is the FP32-split rate; the FP64 drop-in is
-timed on the same initial data and reported beside it (`fp64_value`,
-`fp32_over_fp64`), with the relative L2 difference of every field between
-the two after the same number of steps.
+Prints ONE JSON line.  `value` is the rate of the headline storage mode
+(--precision, default fp32_fix: every perturbation field stored in 4 bytes,
+the conformal metric as 32-bit fixed point, which holds every field within
+1e-5 of FP64); the FP64 drop-in is timed on the same initial data and
+reported beside it (`fp64_value`, `fp32_over_fp64`), with the relative L2
+difference of every field between the two after the same number of steps,
+and so are the other 4-byte modes (`fp32_split`: FP64 conformal metric;
+`fp32_all`: FP32 metric).
 """
 from __future__ import annotations
 
@@ -31,6 +35,16 @@ FIELDS = ["DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIFFgamma22", "DIFFgamma
           "DIFFgamma33", "DIFFphi", "A11", "A12", "A13", "A22", "A23", "A33", "DIFFK",
           "Gamma1", "Gamma2", "Gamma3", "DIFFalpha"]
 HBM_PEAK_GBS = 8000.0
+# what each storage mode computes in (the headline's `dtype`)
+DTYPES = {
+    "fp32_fix": "fp32 perturbation storage, 4 bytes per value: A_ij, K, Gamma^i, alpha, D in FP32; conformal metric "
+                "(DIFFgamma_ij, DIFFphi) in 32-bit fixed point (per-field power-of-two scale); FP32-FMA stencils "
+                "(metric: of exact integer differences); FP64 RK4 accumulation/FRW/reductions",
+    "fp32_split": "fp32 split: f32 storage+stencils of A_ij, K, Gamma^i, alpha, D; FP64 metric "
+                  "(DIFFgamma_ij, DIFFphi), FP64 RK4 accumulation/FRW/reductions",
+    "fp32_all": "fp32: every perturbation field (metric included) in FP32 storage and FP32-FMA stencils; "
+                "FP64 RK4 accumulation/FRW/reductions",
+}
 F_FIELDS, G_FIELDS = 18, 1  # evolved fields, pointwise matter fields read per stage
 
 
@@ -259,6 +273,10 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-fp32-all", action="store_true", help="skip the all-FP32 storage leg")
     ap.add_argument("--skip-fp32-fix", action="store_true", help="skip the fixed-point metric leg")
+    ap.add_argument("--skip-fp32-split", action="store_true", help="skip the FP64-metric split leg")
+    ap.add_argument("--precision", choices=sorted(DTYPES), default="fp32_fix",
+                    help="storage mode of the headline value (default: the fixed-point metric, the 4-byte mode "
+                         "that holds every field within 1e-5 of FP64)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="batched-ensemble leg (0 = skip)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -290,13 +308,17 @@ def main():
     from cosmograph_amd.ics import gaussian_modes
     modes = gaussian_modes(n, args.spectral_index, args.seed)
 
-    log("[bench] FP32 split")
-    g32 = make_ctx(cg, n, order, "fp32_split", modes, args.rms)
+    headline = args.precision
+    log(f"[bench] headline: {headline}")
+    g32 = make_ctx(cg, n, order, headline, modes, args.rms)
     t32, k32_ms, nl32, tune32 = time_steps(g32, args.warmup, args.steps, not args.no_autotune)
     info32 = g32.kernel_info()
-    log(f"[bench] FP32 split: {1e3 * t32 / args.steps:.3f} ms/step, kernel {info32}")
+    log(f"[bench] {headline}: {1e3 * t32 / args.steps:.3f} ms/step, kernel {info32}")
     pts = n ** 3
     rate32 = pts * args.steps / t32
+    words = b_alg_words()
+    bytes_pt_upd = words * 4                       # 4-byte storage of every value
+    bytes_per_launch = bytes_pt_upd / 4.0 * pts    # average stage
 
     res = {}
     detail_extra = {}
@@ -306,6 +328,9 @@ def main():
         t64, k64_ms, _, tune64 = time_steps(g64, args.warmup, args.steps, not args.no_autotune)
         rate64 = pts * args.steps / t64
         f64 = {f: g64.download(f + "_p") for f in FIELDS}
+        c64 = g64.setConstraintCalcs()
+        detail_extra["fp64_autotune"] = tune64
+        del g64
 
         def rel_errs(g):
             out = {}
@@ -319,24 +344,24 @@ def main():
         # Hamiltonian / momentum constraint of the final states (L2 = H[7],
         # L-infinity = H[2]); the FP64 run stands for the oracle (held to it at
         # 1e-12 by the tests)
-        c32, c64 = g32.setConstraintCalcs(), g64.setConstraintCalcs()
+        c32 = g32.setConstraintCalcs()
         cons = {f"{c}_{k}_{p}": float(cc[c][q]) for c in ("H", "M") for k, q in (("L2", 7), ("Linf", 2))
                 for p, cc in (("fp32", c32), ("fp64", c64))}
         cons["H_L2_fp32_over_fp64"] = float(c32["H"][7] / c64["H"][7]) if c64["H"][7] else None
         cons["H_Linf_fp32_over_fp64"] = float(c32["H"][2] / c64["H"][2]) if c64["H"][2] else None
-        log(f"[bench] H L2 / Linf: FP32 {c32['H'][7]:.6e} / {c32['H'][2]:.6e}, "
+        log(f"[bench] H L2 / Linf: {headline} {c32['H'][7]:.6e} / {c32['H'][2]:.6e}, "
             f"FP64 {c64['H'][7]:.6e} / {c64['H'][2]:.6e}")
         res = {"fp64_value": rate64, "fp64_ms_per_step": 1e3 * t64 / args.steps,
                "fp64_stage_kernel_ms": k64_ms, "fp32_over_fp64": rate32 / rate64,
                "max_rel_l2_fp32_vs_fp64": max(errs.values()),
                "rel_l2_fp32_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
-               "constraints": cons,
-               "fp64_kernel": g64.kernel_info(), "fp64_autotune": tune64}
-        del g64
-        # 4-byte storage of every perturbation field: the conformal metric in
-        # FP32 (CG_FP32_ALL) or in 32-bit fixed point (CG_FP32_FIX)
-        for prec, skip in (("fp32_all", args.skip_fp32_all), ("fp32_fix", args.skip_fp32_fix)):
-            if skip:
+               "constraints": cons}
+        # the other 4-byte-perturbation modes on the same data: the FP64
+        # conformal metric (CG_FP32_SPLIT), the metric in FP32 (CG_FP32_ALL)
+        # and the metric in 32-bit fixed point (CG_FP32_FIX)
+        for prec, skip in (("fp32_split", args.skip_fp32_split), ("fp32_all", args.skip_fp32_all),
+                           ("fp32_fix", args.skip_fp32_fix)):
+            if skip or prec == headline:
                 continue
             log(f"[bench] {prec} storage")
             ga = make_ctx(cg, n, order, prec, modes, args.rms)
@@ -346,10 +371,9 @@ def main():
             worst = max(ea, key=ea.get)
             res[prec] = {
                 "value": pts * args.steps / ta, "ms_per_step": 1e3 * ta / args.steps, "stage_kernel_ms": ka_ms,
-                "frac": b_alg_words() * 4 / 4.0 * pts / (ka_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "frac": bytes_per_launch / (ka_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "over_fp64": (pts * args.steps / ta) / rate64,
                 "max_rel_l2_vs_fp64": float(f"{ea[worst]:.3e}"), "worst_field": worst,
-                "rel_l2_metric_vs_fp64": {f: float(f"{ea[f]:.2e}") for f in FIELDS[:7]},
                 "H_L2_over_fp64": float(ca["H"][7] / c64["H"][7]) if c64["H"][7] else None,
                 "kernel": ga.kernel_info()}
             log(f"[bench] {prec}: {res[prec]['ms_per_step']:.3f} ms/step, worst field {worst} {ea[worst]:.2e}")
@@ -357,11 +381,8 @@ def main():
             del ga
         del f64
 
-    words = b_alg_words()
-    bytes_pt_upd = words * 4                       # FP32 storage
-    bytes_per_launch = bytes_pt_upd / 4.0 * pts    # average stage
     achieved = bytes_per_launch / (k32_ms * 1e-3) / 1e9
-    traffic = load_traffic(f"bssn_static_dust_{n}_o{order}_fp32")
+    traffic = load_traffic(f"bssn_static_dust_{n}_o{order}_{headline}")
 
     del g32
     ens = None
@@ -385,12 +406,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 split: f32 storage+stencils of A_ij, K, Gamma^i, alpha, D; FP64 metric "
-                 "(DIFFgamma_ij, DIFFphi), FP64 RK4 accumulation/FRW/reductions",
+        "dtype": DTYPES[headline],
         "data": "synthetic: seeded Gaussian random field ICs, constraint-solved",
         "config": {"workload": f"BSSN + static dust, {n}^3 periodic, order {order}, "
-                               f"P(k)~k^{args.spectral_index:g}, rms {args.rms:g}, dt=0.1dx, FP32 split",
-                   "grid": n, "stencil_order": order, "precision": "fp32_split"},
+                               f"P(k)~k^{args.spectral_index:g}, rms {args.rms:g}, dt=0.1dx, {headline}",
+                   "grid": n, "stencil_order": order, "precision": headline},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
@@ -399,9 +419,8 @@ def main():
         "stage_kernel_launches_timed": nl32,
         "kernel": info32,
     }
-    detail = {"fp32_autotune": tune32, **detail_extra}
+    detail = {f"{headline}_autotune": tune32, **detail_extra}
     if res:
-        detail["fp64_autotune"] = res.pop("fp64_autotune")
         line.update(res)
     if ens is not None:
         detail["ensemble"] = dict(ens)
