@@ -446,14 +446,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   // ---------------------------------------------------------- point values
   TC v[NF];
   double gc[7];  // metric at the point in FP64 (DIFFgamma_ij, DIFFphi)
-  // fixed-point metric: the words at the point (stage 4's a3 of the metric)
-  int gw[7];
-#pragma unroll
-  for (int f = 0; f < 7; ++f) gw[f] = MQ ? ldo((const int*)INS(f), idx) : 0;
 #pragma unroll
   for (int f = 0; f < 6; ++f)  // idx = 0 outside the grid
-    gc[f] = MQ ? (double)gw[f] * a.msc[f] : M32 ? (double)ldo(INS(f), idx) : ldo(INM(f), idx);
-  const double phi_c = MQ    ? (double)gw[PHI] * a.msc[PHI]
+    gc[f] = MQ    ? (double)ldo((const int*)INS(f), idx) * a.msc[f]
+            : M32 ? (double)ldo(INS(f), idx)
+                  : ldo(INM(f), idx);
+  const double phi_c = MQ    ? (double)ldo((const int*)INS(PHI), idx) * a.msc[PHI]
                        : M32 ? (double)ldo(INS(PHI), idx)
                              : ldo(INM(PHI), idx);
   gc[6] = phi_c;
@@ -1155,6 +1153,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
                 a3m[g] = ldo(INM(g), idx);
               } else {
                 a2v[g] = lw(a.s_a2);
+                // the fixed-point metric's stage-4 input, re-read here rather
+                // than kept in registers through the stencil phase
+                if (MQ && g < 7) a3m[g < 7 ? g : 0] = lw(a.s_in);
               }
             }
           }
@@ -1206,8 +1207,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
               if (st >= 2) fo = PF ? lpf(f, 1) : qv[f];
             } else if (st == 4) {
               const double a1 = PF ? lpf(f, 1) : qv[f];
-              const double a3 = (double)gw[f < 7 ? f : 0] * sc;
-              fo = ((a1 - p) + 2.0 * (a2v[f] - p) + (a3 - p)) * (1.0 / 3.0);
+              fo = ((a1 - p) + 2.0 * (a2v[f] - p) + (a3m[f < 7 ? f : 0] - p)) * (1.0 / 3.0);
             }
             const double inc = a.cf * cc;
             const double nd = st == 4 ? fo + (inc + p) : p + a.ca * cc;
