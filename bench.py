@@ -194,17 +194,21 @@ def c5_leg(cg, args, log):
     3e-2, ICs on the device (cg_ics_static_dust: the fluid at rest), evolved
     as perturbations of the FRW fluids (cg_set_dust_background: the mean dust
     and Lambda as a w = -1 fluid in the FP64 background, the perturbations in
-    FP32).  FP32 split and FP64 runs of the same ICs; reports the
-    rate, the stage kernel's HBM fraction of B_alg = (17*22)*4 B per
-    point-update, and the Hamiltonian-constraint growth (H L2 after / before)
-    of both.  Prints one JSON line."""
+    4-byte storage).  The headline storage mode (--precision: the conformal
+    metric in 32-bit fixed point by default), the FP64-metric split mode and
+    FP64 runs of the same ICs; reports the rates, the stage kernel's HBM
+    fraction of B_alg = (17*22)*4 B per point-update, the Hamiltonian-
+    constraint growth (H L2 after / before) and the relative L2 of the 4-byte
+    runs against FP64.  Prints one JSON line."""
     from cosmograph_amd.ics import RHO_BAR, gaussian_modes
     n, order, steps = args.n, args.order, args.steps
     lam = 0.7 * RHO_BAR
     log(f"[c5] {n}^3 dust + Lambda, order {order}, {steps} steps; modes")
     modes = gaussian_modes(n, args.spectral_index, args.seed)
+    head = args.precision
+    precs = [head] + [p for p in ("fp32_split",) if p != head] + ["fp64"]
     runs, keep = {}, {}
-    for prec in ("fp32_split", "fp64"):
+    for prec in precs:
         g = cg.BSSN(cg.GpuParams(n=n, order=order, precision=prec, matter="dust", lambda_=lam, ko_coeff=0.1))
         g.set_dust_background(True)   # perturbations of the FRW fluids (mean dust, Lambda as w = -1)
         t0 = time.perf_counter()
@@ -215,36 +219,40 @@ def c5_leg(cg, args, log):
         h1 = g.setConstraintCalcs()["H"][7]
         runs[prec] = {"value": n ** 3 * steps / t, "ms_per_step": 1e3 * t / steps, "stage_kernel_ms": k_ms,
                       "H_L2_start": h0, "H_L2_end": h1, "H_growth": h1 / h0 if h0 else None,
-                      "kernel": g.kernel_info(), "nan_points": g.numNaNs("DIFFphi_p")}
+                      "kernel": g.kernel_info(), "nan_points": g.numNaNs("dust_D_p")}
         log(f"[c5] {prec}: {runs[prec]['ms_per_step']:.2f} ms/step, H growth {runs[prec]['H_growth']:.4g}")
-        if prec == "fp32_split":
-            keep = {f: g.download(f + "_p") for f in C5_FIELDS}
+        if prec != "fp64":
+            keep[prec] = {f: g.download(f + "_p") for f in C5_FIELDS}
         else:
-            errs = {}
-            for f in C5_FIELDS:
-                b = g.download(f + "_p")
-                nb = float(np.sqrt(np.sum(b * b)))
-                d = float(np.sqrt(np.sum((keep[f] - b) ** 2)))
-                errs[f] = float(f"{(d / nb if nb > 0 else d):.3e}")
-            runs["rel_l2_fp32_vs_fp64"] = errs
+            for p4, fields in keep.items():
+                errs = {}
+                for f in C5_FIELDS:
+                    b = g.download(f + "_p")
+                    nb = float(np.sqrt(np.sum(b * b)))
+                    d = float(np.sqrt(np.sum((fields[f] - b) ** 2)))
+                    errs[f] = float(f"{(d / nb if nb > 0 else d):.3e}")
+                runs[p4]["rel_l2_vs_fp64"] = errs
         del g
-    b_alg = (17 * 22) * 4   # FP32 words of the dust-fluid step (F = 22, G = 0)
-    achieved = b_alg / 4.0 * n ** 3 / (runs["fp32_split"]["stage_kernel_ms"] * 1e-3) / 1e9
-    r32, r64 = runs["fp32_split"], runs["fp64"]
-    emit({
+    b_alg = (17 * 22) * 4   # 4-byte words of the dust-fluid step (F = 22, G = 0)
+    achieved = b_alg / 4.0 * n ** 3 / (runs[head]["stage_kernel_ms"] * 1e-3) / 1e9
+    r32, r64 = runs[head], runs["fp64"]
+    line = {
         "metric": f"BSSN + dust fluid + Lambda RK4 point-updates/s at {n}^3, FP32 split",
         "value": r32["value"], "unit": "point-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
         "ms_per_step": r32["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 (FP64 metric)", "data": "synthetic: seeded Gaussian random field ICs, constraint-solved",
+        "dtype": DTYPES[head], "data": "synthetic: seeded Gaussian random field ICs, constraint-solved",
         "config": {"workload": f"dust fluid + Lambda {n}^3, order {order}, k^{args.spectral_index}, "
                                f"rms {args.rms}, Lambda energy density 0.7 rho_bar; KO coefficient 0.1, "
                                f"which is zero at order 6 as the reference's KO stencil; fluid D and S_i "
-                               f"only (no energy variable E, as the reference)"},
+                               f"only (no energy variable E, as the reference)", "precision": head},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "b_alg_bytes_per_point_update": b_alg,
-        "fp32": r32, "fp64": r64, "fp32_over_fp64": r32["value"] / r64["value"],
-        "rel_l2_fp32_vs_fp64": runs["rel_l2_fp32_vs_fp64"]}, {}, args.detail_file, log)
+        "fp32": r32, "fp64": r64, "fp32_over_fp64": r32["value"] / r64["value"]}
+    for p4 in keep:
+        if p4 != head:
+            line[p4] = runs[p4]
+    emit(line, {}, args.detail_file, log)
 
 
 def load_traffic(workload):

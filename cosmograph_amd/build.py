@@ -21,7 +21,7 @@ ARCH = os.environ.get("COSMO_GPU_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
-SOURCES = ["stage_inst_f32.hip", "stage_inst_f32all.hip", "stage_inst_f32fix.hip", "stage_inst_f64.hip", "stage_inst_misc.hip", "stage_inst_dust.hip", "cons_inst.hip",
+SOURCES = ["stage_inst_f32.hip", "stage_inst_f32all.hip", "stage_inst_f32fix.hip", "stage_inst_f64.hip", "stage_inst_misc.hip", "stage_inst_dust.hip", "stage_inst_dustfix.hip", "cons_inst.hip",
            "cosmo_gpu.hip", "batch.hip", "ics.hip", "gauge.hip", "bardeen.hip", "fixmetric.hip"]
 
 

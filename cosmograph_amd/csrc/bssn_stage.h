@@ -22,16 +22,19 @@ enum { MODE_STAGE = 0, MODE_CONS = 1, MODE_DUST = 2, MODE_DUST_CONS = 3 };
 // integer word x 2^e with one power-of-two scale per field
 enum { MODE_M32 = 4, MODE_M32_CONS = 5, MODE_MQ = 8, MODE_MQ_STAGE = 12, MODE_MQ_CONS = 13 };
 // fixed-point metric word (MODE_MQ): value / 2^e rounded to the nearest
-// integer, saturated at MQ_WMAX = 2^30 - 1 in magnitude, so that the sum or
-// difference of two words never overflows int32 (between exponent checks
-// the host keeps |word| below 2^28; saturation means a field grew fourfold
-// within one check interval, and the stage kernels count it)
+// integer, saturated at MQ_WMAX = 2^30 - 1 in magnitude.  After each
+// exponent check the host keeps |word| below 2^MQ_BITS = 2^29, so a field
+// can double before the next check; saturation means it grew faster, and
+// the stage kernels count it.  With |word| < 2^30 every first difference
+// of two words fits in int32, and the second and mixed differences do for
+// any field resolved finely enough to be differenced (the stencils form
+// them in wrap-around arithmetic, exact whenever the result fits).
 constexpr double MQ_WMAX = 1073741823.0;
 __device__ __forceinline__ int mq_word(double v, double inv) {
   const double t = fmin(fmax(v * inv, -MQ_WMAX), MQ_WMAX);
   return (int)rint(t);
 }
-constexpr int MQ_BITS = 28;        // words stay below 2^MQ_BITS after each exponent check
+constexpr int MQ_BITS = 29;        // words stay below 2^MQ_BITS after each exponent check
 constexpr int MQ_FLOOR = 6;        // exponents stay within 6 of the largest metric field's
 
 enum Lapse {

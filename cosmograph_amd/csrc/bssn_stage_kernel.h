@@ -792,13 +792,16 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       if (a.lag_mode != 0) {
         double m[6];
   #pragma unroll
-        for (int q = 0; q < 6; ++q) m[q] = ldo((const double*)RM(q, a.s_lag), idx);
+        for (int q = 0; q < 6; ++q)
+          m[q] = MQ ? (double)ldo((const int*)RS(q, a.s_lag), idx) * a.msc[q] : ldo((const double*)RM(q, a.s_lag), idx);
         if (a.lag_mode == 2) {
           const double tn = norm_factor<double>(m[0], m[1], m[2], m[3], m[4], m[5]);
   #pragma unroll
           for (int q = 0; q < 6; ++q) m[q] = (m[q] + (diag6(q) ? tn : 0.0)) / (1.0 - tn);
         }
-        const TC phl = (TC)(ldo((const double*)RM(PHI, a.s_lag), idx) + a.phi_F_lag);
+        const TC phl = (TC)((MQ ? (double)ldo((const int*)RS(PHI, a.s_lag), idx) * a.msc[PHI]
+                                : ldo((const double*)RM(PHI, a.s_lag), idx)) +
+                            a.phi_F_lag);
         const TC Dl = (TC)ldo((const TS*)RD(0, a.s_lag), idx);
         TC Sl[3];
   #pragma unroll

@@ -17,9 +17,13 @@ static int cons_by_order(int order, const StageArgs& a, hipStream_t s) {
 }
 
 int launch_constraints(int prec, int order, bool live_alpha, bool dust, const StageArgs& a, hipStream_t s) {
-  if (prec == 3)  // fixed-point metric, other fields FP32 (no dust fluid)
+  if (prec == 3) {  // fixed-point metric, other fields FP32
+    if (dust)
+      return live_alpha ? cons_by_order<float, true, MODE_MQ_CONS | MODE_DUST>(order, a, s)
+                        : cons_by_order<float, false, MODE_MQ_CONS | MODE_DUST>(order, a, s);
     return live_alpha ? cons_by_order<float, true, MODE_MQ_CONS>(order, a, s)
                       : cons_by_order<float, false, MODE_MQ_CONS>(order, a, s);
+  }
   if (prec == 2)  // all-FP32 storage (no dust fluid)
     return live_alpha ? cons_by_order<float, true, MODE_M32_CONS>(order, a, s)
                       : cons_by_order<float, false, MODE_M32_CONS>(order, a, s);

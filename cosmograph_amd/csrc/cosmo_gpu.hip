@@ -31,6 +31,10 @@ const char* kFieldNames[NF] = {"DIFFgamma11", "DIFFgamma12", "DIFFgamma13", "DIF
 
 // ------------------------------------------------- elementwise kernels --
 // Register table: slot pointers of every field (DIFFphi always FP64).
+struct MetScale {
+  double s[7];
+};
+
 struct RegTable {
   void* r[NF][4];
   double ms[7], mi[7];  // fixed-point metric: scale 2^e and 2^-e per field
@@ -213,12 +217,15 @@ __global__ void k_ext_reduce(const double* __restrict__ in, long rows, double* _
 // fluxes alpha D v^i, alpha S_j v^i (flux q*3 + i; q = 0 D, 1..3 S_j).  At
 // stage 1 of a fused step the metric is normalised on the fly as the stage
 // kernel does (bssn.cc:61-119).
-template <typename TS>
-__global__ void k_dust_flux(const double* __restrict__ met, const TS* __restrict__ alpha, const TS* __restrict__ dreg,
-                            long rstride, int slot, int normalize, double phiF, TS* __restrict__ flux, long n) {
+// (MT: the metric's storage, FP64 or 32-bit fixed-point words times ms[f];
+// either way field f, slot s at met + (f*4 + s)*rstride)
+template <typename TS, typename MT>
+__global__ void k_dust_flux(const MT* __restrict__ met, MetScale ms, const TS* __restrict__ alpha,
+                            const TS* __restrict__ dreg, long rstride, int slot, int normalize, double phiF,
+                            TS* __restrict__ flux, long n) {
   const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
-  auto M = [&](int f) { return met[((long)f * 4 + slot) * rstride + x]; };
+  auto M = [&](int f) { return (double)met[((long)f * 4 + slot) * rstride + x] * ms.s[f]; };
   double g11 = M(G11), g12 = M(G12), g13 = M(G13), g22 = M(G22), g23 = M(G23), g33 = M(G33);
   if (normalize) {
     const double omd = -1.0 * (g11 + g22 + g33 - g12 * g12 - g13 * g13 - g23 * g23 + g11 * g22 + g11 * g33 +
@@ -514,11 +521,13 @@ int cg_create(const cg_params* params, cg_ctx** out) {
   if (p.precision != CG_FP64 && p.precision != CG_FP32_SPLIT && p.precision != CG_FP32_ALL &&
       p.precision != CG_FP32_FIX)
     return CG_ERR_ARG;
-  // all-FP32 and fixed-point storage: BSSN with static dust or vacuum, the
-  // shift-free gauges the stage kernel forms by itself, on single grids
+  // all-FP32 and fixed-point storage: the shift-free gauges the stage kernel
+  // forms by itself, on single grids; BSSN with static dust or vacuum (and,
+  // with the fixed-point metric, the dust fluid)
   if ((p.precision == CG_FP32_ALL || p.precision == CG_FP32_FIX) &&
-      (p.matter == CG_MATTER_DUST || p.shift != CG_SHIFT_NONE || p.lapse == LAPSE_TEST_K_DRIVER))
+      (p.shift != CG_SHIFT_NONE || p.lapse == LAPSE_TEST_K_DRIVER))
     return CG_ERR_ARG;
+  if (p.precision == CG_FP32_ALL && p.matter == CG_MATTER_DUST) return CG_ERR_ARG;
   if (p.matter != CG_MATTER_NONE && p.matter != CG_MATTER_STATIC && p.matter != CG_MATTER_DUST) return CG_ERR_ARG;
   if (p.lapse < 0 || p.lapse >= LAPSE_N) return CG_ERR_ARG;
   if (p.shift < 0 || p.shift >= SHIFT_N) return CG_ERR_ARG;
@@ -1018,12 +1027,19 @@ static int dust_reg_init(cg_ctx* c, int rp, int ra, int rf) {
 static int dust_flux(cg_ctx* c, int slot, int normalize) {
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
+  MetScale ms, mi;
+  for (int f = 0; f < 7; ++f) ms.s[f] = 1.0;
+  if (c->mq) mq_scales(c, ms.s, mi.s);
   if (c->prec == CG_FP64)
-    hipLaunchKernelGGL(k_dust_flux<double>, dim3(nb), dim3(bs), 0, c->stream, (const double*)c->met,
+    hipLaunchKernelGGL((k_dust_flux<double, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)c->met, ms,
                        (const double*)reg_ptr(c, ALPHA, 0), (const double*)c->dreg, c->rstride, slot, normalize,
                        c->frw.g_phi, (double*)c->flux, c->npts);
+  else if (c->mq)
+    hipLaunchKernelGGL((k_dust_flux<float, int>), dim3(nb), dim3(bs), 0, c->stream, (const int*)reg_ptr(c, 0, 0), ms,
+                       (const float*)reg_ptr(c, ALPHA, 0), (const float*)c->dreg, c->rstride, slot, normalize,
+                       c->frw.g_phi, (float*)c->flux, c->npts);
   else
-    hipLaunchKernelGGL(k_dust_flux<float>, dim3(nb), dim3(bs), 0, c->stream, (const double*)c->met,
+    hipLaunchKernelGGL((k_dust_flux<float, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)c->met, ms,
                        (const float*)reg_ptr(c, ALPHA, 0), (const float*)c->dreg, c->rstride, slot, normalize,
                        c->frw.g_phi, (float*)c->flux, c->npts);
   HIPCHK(c, hipGetLastError());
@@ -1119,7 +1135,7 @@ int cg_k_finalize(cg_ctx* c, int n) {
   return CG_OK;
 }
 
-constexpr int kMqCheck = 4;
+constexpr int kMqCheck = 2;
 
 int cg_rk4_steps(cg_ctx* c, int nsteps) {
   if (!c || nsteps < 0) return CG_ERR_ARG;
@@ -1188,7 +1204,7 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
     int rc;
     c->sP = 0; c->sF = 1;
     // fixed-point metric: exponents checked every kMqCheck steps (a field's
-    // words stay below 2^MQ_BITS; four times that saturates)
+    // words stay below 2^MQ_BITS; twice that saturates)
     if (c->mq && s % kMqCheck == 0 && (rc = mq_check(c))) return rc;
     const bool pre = !ko_norm && !c->dreg && c->p.normalize_metric && s + 1 < nsteps && c->var &&
                      c->dma_allowed && c->dma_pick && dma_fits(c, *c->var);

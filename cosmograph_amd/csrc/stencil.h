@@ -67,57 +67,34 @@ __device__ __forceinline__ T fdm(const T* s, int c, int sa, int sb) {
 }
 
 // The same stencils on a tile of 32-bit fixed-point words (the conformal
-// metric of the fixed-point storage mode).  Words stay below 2^30 in
-// magnitude, so the sum or difference of two words is exact in int32; each
-// such pair is converted to FP64 once (exactly) and the weighted sums are
-// FP64.  The caller multiplies by the field's scale.
-template <int R>
-__device__ __forceinline__ double fd1q(const int* s, int c, int st) {
-  double acc = c1<R>(1) * (double)(s[c + st] - s[c - st]);
-#pragma unroll
-  for (int m = 2; m <= R; ++m) acc += c1<R>(m) * (double)(s[c + m * st] - s[c - m * st]);
-  return acc;
+// metric of the fixed-point storage mode), for the FP32 algebra: first
+// differences q(+m) - q(-m), second differences about the centre
+// q(+m) + q(-m) - 2 q(0) (the stencil weights sum to zero) and the mixed
+// cross differences are formed exactly in 32-bit wrap-around arithmetic
+// (exact whenever the difference itself fits in int32, which it does for a
+// field sampled finely enough to be differentiated), converted, and summed
+// with FP32 FMA; the caller applies the field's scale.  The FP32 sums keep
+// the relative precision of the derivative itself; FP32 storage of the
+// values would instead lose it to the cancellation of neighbouring values.
+__device__ __forceinline__ float qd(int a, int b) { return (float)(int)((unsigned)a - (unsigned)b); }
+__device__ __forceinline__ float qdd(int a, int b, int c0) {
+  return (float)(int)(((unsigned)a - (unsigned)c0) + ((unsigned)b - (unsigned)c0));
 }
 
-template <int R>
-__device__ __forceinline__ double fd2q(const int* s, int c, int st) {
-  double acc = c2<R>(0) * (double)s[c];
-#pragma unroll
-  for (int m = 1; m <= R; ++m) acc += c2<R>(m) * (double)(s[c + m * st] + s[c - m * st]);
-  return acc;
-}
-
-template <int R>
-__device__ __forceinline__ double fdmq(const int* s, int c, int sa, int sb) {
-  double acc = 0.0;
-#pragma unroll
-  for (int m = 1; m <= R; ++m) {
-    const int p = m * (sa + sb), q = m * (sa - sb);
-    acc += cm<R>(m) * ((double)(s[c + p] - s[c + q]) - (double)(s[c - q] - s[c - p]));
-  }
-  return acc;
-}
-
-// FP32 versions for the FP32 algebra: first differences q(+m) - q(-m),
-// second differences about the centre q(+m) + q(-m) - 2 q(0) (the stencil
-// weights sum to zero) and the mixed cross differences are formed exactly in
-// int32 and only then converted, so the FP32 arithmetic keeps the relative
-// precision of the derivative itself; FP32 storage of the values would
-// instead lose it to the cancellation of neighbouring values.
 template <int R>
 __device__ __forceinline__ float fd1qf(const int* s, int c, int st) {
-  float acc = float(c1<R>(1)) * (float)(s[c + st] - s[c - st]);
+  float acc = float(c1<R>(1)) * qd(s[c + st], s[c - st]);
 #pragma unroll
-  for (int m = 2; m <= R; ++m) acc += float(c1<R>(m)) * (float)(s[c + m * st] - s[c - m * st]);
+  for (int m = 2; m <= R; ++m) acc += float(c1<R>(m)) * qd(s[c + m * st], s[c - m * st]);
   return acc;
 }
 
 template <int R>
 __device__ __forceinline__ float fd2qf(const int* s, int c, int st) {
   const int q0 = s[c];
-  float acc = float(c2<R>(1)) * (float)((s[c + st] - q0) + (s[c - st] - q0));
+  float acc = float(c2<R>(1)) * qdd(s[c + st], s[c - st], q0);
 #pragma unroll
-  for (int m = 2; m <= R; ++m) acc += float(c2<R>(m)) * (float)((s[c + m * st] - q0) + (s[c - m * st] - q0));
+  for (int m = 2; m <= R; ++m) acc += float(c2<R>(m)) * qdd(s[c + m * st], s[c - m * st], q0);
   return acc;
 }
 
@@ -127,7 +104,8 @@ __device__ __forceinline__ float fdmqf(const int* s, int c, int sa, int sb) {
 #pragma unroll
   for (int m = 1; m <= R; ++m) {
     const int p = m * (sa + sb), q = m * (sa - sb);
-    acc += float(cm<R>(m)) * (float)((s[c + p] - s[c + q]) - (s[c - q] - s[c - p]));
+    const unsigned d = ((unsigned)s[c + p] - (unsigned)s[c + q]) - ((unsigned)s[c - q] - (unsigned)s[c - p]);
+    acc += float(cm<R>(m)) * (float)(int)d;
   }
   return acc;
 }
@@ -168,8 +146,8 @@ __device__ __forceinline__ double ko_sumq(const int* s, int c, int si, int sj, i
     double acc = 3.0 * w(0) * (double)s[c];
 #pragma unroll
     for (int m = 1; m <= NM; ++m)
-      acc += w(m) * (((double)(s[c + m * si] + s[c - m * si]) + (double)(s[c + m * sj] + s[c - m * sj])) +
-                     (double)(s[c + m * sk] + s[c - m * sk]));
+      acc += w(m) * ((((double)s[c + m * si] + (double)s[c - m * si]) + ((double)s[c + m * sj] + (double)s[c - m * sj])) +
+                     ((double)s[c + m * sk] + (double)s[c - m * sk]));
     return acc;
   } else {
     return 0.0;

@@ -58,7 +58,7 @@ def test_python_front_end_binds_every_symbol():
     # all-FP32 storage: BSSN + static dust on single grids, the stage kernel's own gauges
     {"precision": "fp32_all", "matter": "dust"}, {"precision": "fp32_all", "shift": "Static"},
     {"precision": "fp32_all", "lapse": "TestKDriverLapse"},
-    {"precision": "fp32_fix", "matter": "dust"}, {"precision": "fp32_fix", "shift": "GammaDriver"},
+    {"precision": "fp32_fix", "shift": "GammaDriver"}, {"precision": "fp32_fix", "lapse": "TestKDriverLapse"},
 ])
 def test_create_rejects_bad_parameters(bad):
     import cosmograph_amd as cg

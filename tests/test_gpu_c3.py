@@ -8,15 +8,20 @@ FP64 drop-in: the bars of the 128^3 FP64 test (1e-12 on every field's
 sample and norms; constraint statistics to 1e-12 of the terms they are the
 difference of).
 
-FP32 split (FP32 storage and stencils of A_ij, K, Gamma^i, alpha, D; FP64
-conformal metric, RK4 accumulation, FRW and reductions):
+The two 4-byte modes, FP32 split (FP32 storage and stencils of A_ij, K,
+Gamma^i, alpha, D; FP64 conformal metric) and the fixed-point metric (the
+same with the conformal metric in 32-bit words), both with FP64 RK4
+accumulation, FRW and reductions:
   * every field within 1e-5 relative L2 of the oracle on the 32^3 sample;
   * the Hamiltonian constraint no worse than the oracle's, to a stated
     tolerance: H L2 and H L-infinity at every output step at most
-    (1 + 1e-4) times the oracle's.  Rounding the stored fields to FP32 adds
-    a violation of its own, about 1e-5 of H here (measured: +1.0e-5 in L2,
-    +1.9e-5 in L-infinity after 203 steps); the tolerance allows 5x that,
-    and a drift of the evolution (growth of H) would exceed it."""
+    (1 + 1e-4) times the oracle's plus four FP32 ulps of the mean scale of
+    H (the size of the terms it is the difference of).  Rounding the stored
+    fields to 4 bytes adds a violation of its own, about 1e-5 of H here once
+    it has grown (split mode: +1.0e-5 in L2, +1.9e-5 in L-infinity after 203
+    steps); the relative tolerance allows 5x that, the floor covers the
+    first output steps, where H is the constraint-solved initial residual,
+    and a drift of the evolution (growth of H) would exceed both."""
 import sys
 
 import numpy as np
@@ -29,11 +34,13 @@ from test_gpu_c2 import check_constraints
 pytestmark = pytest.mark.gpu
 
 H_TOL = 1e-4
-# The fixed-point metric rounds the initial DIFFphi to words of 2^-28 of its
-# largest value, which adds about 1e-9 (quadrature) to the constraint-solved
-# initial residual H L2 ~ 9e-6 (measured: +1.4e-4 of it); at step 0 the bar is
-# 5e-4 of the oracle's value, at every later output step H_TOL.
-H_TOL0_FIX = 5e-4
+# Absolute floor of the H bar, in units of the mean scale of H per point (the
+# magnitude of the terms H is the difference of, ~1.06 here): four FP32 ulps.
+# It matters only while H is tiny: the initial data are constraint-solved (H
+# L-infinity 5e-5 at step 0), and rounding the stored state to 4 bytes
+# (the conformal metric's 32-bit words included) moves that residual's
+# maximum by ~1e-7 of the scale, i.e. by up to 2e-3 of its own value.
+H_FLOOR = 4 * 2.0 ** -23
 
 
 @pytest.fixture(scope="module")
@@ -90,23 +97,21 @@ def test_c3_fp32_split_gate(gpu, digest, prec):
     g = digests.gpu_context(gpu, d, prec, static_dust_ics)
     excess = {"L2": 0.0, "Linf": 0.0}
     bad = []
+    npts = float(cfg["n"]) ** 3
     for rec in d["constraints"]:
         if rec["step"]:
             g.step(cfg["every"])
         got = g.setConstraintCalcs()["H"]
         want = [digests.fhex(x) for x in rec["H"]]
-        tol = H_TOL0_FIX if (prec == "fp32_fix" and rec["step"] == 0) else H_TOL
+        floor = H_FLOOR * want[3] / npts  # want[3]: the sum of the H scale over the grid
         for key, q in (("L2", 7), ("Linf", 2)):
             r = got[q] / want[q] - 1.0
-            if rec["step"]:
-                excess[key] = max(excess[key], r)
-            else:
-                _say(f"c3 {prec} step 0 H {key} excess {r:.2e}")
-            if got[q] > want[q] * (1.0 + tol):
-                bad.append((rec["step"], key, r))
+            excess[key] = max(excess[key], r)
+            if got[q] > want[q] * (1.0 + H_TOL) + floor:
+                bad.append((rec["step"], key, r, got[q] - want[q], floor))
     assert not bad, bad
     errs = digests.sample_errors(g, d, sample)
-    _say(f"c3 {prec} H excess over the oracle after step 0: L2 {excess['L2']:.2e}, Linf {excess['Linf']:.2e}")
+    _say(f"c3 {prec} largest relative H excess over the oracle: L2 {excess['L2']:.2e}, Linf {excess['Linf']:.2e}")
     _say(f"c3 {prec} field errors (sample rel L2): " + " ".join(f"{f} {errs[f]['sample']:.1e}" for f in FIELDS))
     for f in FIELDS:
         assert errs[f]["sample"] <= 1e-5, (f, errs[f]["sample"])

@@ -114,12 +114,15 @@ def test_dust_constraints_and_averages(oracle_mod, gpu):
             assert abs(got[c][q] - w) <= 1e-10 * max(abs(w), abs(want[c][2])), (c, q, got[c][q], w)
 
 
+@pytest.mark.parametrize("prec", ["fp32_split", "fp32_fix"])
 @pytest.mark.parametrize("order,frw,lam", [(4, "empty", 0.0), (6, "empty", 0.02), (6, "full", 0.0)])
-def test_dust_fp32_split(oracle_mod, gpu, order, frw, lam):
+def test_dust_fp32_split(oracle_mod, gpu, order, frw, lam, prec):
+    """The 4-byte modes: FP64 conformal metric (fp32_split) and the metric
+    as 32-bit fixed point (fp32_fix), the dust registers and fluxes FP32."""
     shape = (18, 16, 20)
-    _say(f"case dust fp32 split order={order} frw={frw} lam={lam}")
+    _say(f"case dust {prec} order={order} frw={frw} lam={lam}")
     o, g = dust_pair(oracle_mod, gpu, shape, order, EMPTY_FRW if frw == "empty" else FULL_FRW, lam,
-                     precision="fp32_split")
+                     precision=prec)
     o.run_steps(3)
     g.step(3)
     check(g, o, tol=1e-5)
@@ -173,12 +176,14 @@ def test_dust_background_is_the_same_physics(gpu):
     assert np.all(pert.download("DIFFK_p") == 0.0)
     assert abs(pert.frw_get()[2] - (RHO_BAR + lam)) <= 1e-15
     f32 = _dust_ics(gpu, n, "fp32_split", lam, True, modes)
-    for g in (ref, pert, f32):
+    fix = _dust_ics(gpu, n, "fp32_fix", lam, True, modes)
+    for g in (ref, pert, f32, fix):
         g.step(3)
-    a, b, c = _totals(ref), _totals(pert), _totals(f32)
+    a, b, c, d = _totals(ref), _totals(pert), _totals(f32), _totals(fix)
     for f in FIELDS + DUST:
         if not np.any(a[f]):
-            assert not np.any(b[f]) and not np.any(c[f]), f
+            assert not np.any(b[f]) and not np.any(c[f]) and not np.any(d[f]), f
             continue
         assert rel_l2(b[f], a[f]) <= 1e-10, (f, rel_l2(b[f], a[f]))
         assert rel_l2(c[f], b[f]) <= 1e-5, (f, rel_l2(c[f], b[f]))
+        assert rel_l2(d[f], b[f]) <= 1e-5, (f, rel_l2(d[f], b[f]))
