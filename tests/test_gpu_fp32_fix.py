@@ -1,10 +1,11 @@
 """The fixed-point metric mode (CG_FP32_FIX: DIFFgamma_ij and DIFFphi as
-32-bit words with one power-of-two scale per field, differentiated exactly
-in FP64; every other BSSN field in FP32; RK4 accumulation, FRW background
-and reductions in FP64) against the FP64 oracle, through the C ABI.
+32-bit words with one power-of-two scale per field, differentiated by FP32
+sums of exact integer differences; every other BSSN field in FP32; RK4
+accumulation, FRW background and reductions in FP64) against the FP64
+oracle, through the C ABI.
 
 Bars: smooth random data of amplitude 1e-3 over a few steps, relative L2
-<= 2e-6 for the metric fields (words below 2^28 of the field's maximum) and,
+<= 2e-6 for the metric fields (the field's maximum in 29-bit words) and,
 for the FP32 fields, <= 2e-5 or no more than twice the error of the split
 mode (FP64 metric, the same FP32 fields) on the same data;
 constraint norms within 1e-4 of the oracle's; an upload reads back within
@@ -83,7 +84,7 @@ def test_fp32_fix_upload_round_trip(gpu):
     for f in METRIC:
         x = st[f]
         g.upload(f + "_p", x)
-        # the word is exact to half a word, 2^-29 of the largest |x|
+        # exact to half a word: the largest |x| is at least 2^28 words
         assert np.abs(g.download(f + "_p") - x).max() <= np.abs(x).max() * 2.0 ** -28, f
     # stepInit copies _p to _a (DIFFphi unchanged); a 1000x larger upload to
     # DIFFphi_p raises the field's scale and re-expresses _a in it
@@ -118,7 +119,7 @@ def test_fp32_fix_normalisation_sets_the_scale(oracle_mod, gpu):
     """A large off-diagonal component (0.5 here) makes the unit-determinant
     normalisation shift the diagonal components far beyond their own size;
     the exponent check covers the normalised values, so nothing saturates
-    and the steps follow the oracle (bars of the 2^-28 words of the larger
+    and the steps follow the oracle (bars of the words' rounding on the larger
     normalised range: 1e-5 on every field)."""
     _say("case fp32_fix normalisation range")
     st = smooth_random_state(SHAPE, seed=34, with_alpha=False)
