@@ -207,7 +207,7 @@ def c5_leg(cg, args, log):
     modes = gaussian_modes(n, args.spectral_index, args.seed)
     head = args.precision
     precs = [head] + [p for p in ("fp32_split",) if p != head] + ["fp64"]
-    runs, keep = {}, {}
+    runs, keep, tunes = {}, {}, {}
     for prec in precs:
         g = cg.BSSN(cg.GpuParams(n=n, order=order, precision=prec, matter="dust", lambda_=lam, ko_coeff=0.1))
         g.set_dust_background(True)   # perturbations of the FRW fluids (mean dust, Lambda as w = -1)
@@ -216,6 +216,7 @@ def c5_leg(cg, args, log):
         log(f"[c5] {prec}: device ICs {iters} iterations, {time.perf_counter() - t0:.1f} s")
         h0 = g.setConstraintCalcs()["H"][7]
         t, k_ms, nl, tune = time_steps(g, args.warmup, steps, not args.no_autotune)
+        tunes[f"{prec}_autotune"] = tune
         h1 = g.setConstraintCalcs()["H"][7]
         runs[prec] = {"value": n ** 3 * steps / t, "ms_per_step": 1e3 * t / steps, "stage_kernel_ms": k_ms,
                       "H_L2_start": h0, "H_L2_end": h1, "H_growth": h1 / h0 if h0 else None,
@@ -252,7 +253,7 @@ def c5_leg(cg, args, log):
     for p4 in keep:
         if p4 != head:
             line[p4] = runs[p4]
-    emit(line, {}, args.detail_file, log)
+    emit(line, tunes, args.detail_file, log)
 
 
 def load_traffic(workload):

@@ -87,11 +87,13 @@ struct StageArgs {
   // at dreg + (q*4 + slot)*rstride (storage type); the 12 fluxes
   // alpha D v^i, alpha S_j v^i of the stencil input at flux + q*rstride;
   // the matter sources use W, detg, g_ij of the previous stage's input
-  // (the reference's lag, dust.cc:226-264) read at slot s_lag
+  // (the reference's lag, dust.cc:226-264) read at slot s_lag, its
+  // DIFFgamma_ij at slot s_lag_met (a normalised copy, when the previous
+  // stage read one)
   void* dreg;
   const void* flux;
-  int s_lag;
-  int lag_mode;           // 0: lag = stencil input; 1: slot s_lag; 2: slot s_lag, metric normalised
+  int s_lag, s_lag_met;
+  int lag_mode;           // 0: lag = stencil input; 1: slots s_lag / s_lag_met; 2: the same, metric normalised
   double phi_F_lag;       // FRW phi the lagged state was seen with
   double* rho_out;        // dust stage: DIFFr per point (null: not kept)
   int nx, ny, nz;

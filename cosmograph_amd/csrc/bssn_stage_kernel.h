@@ -793,7 +793,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         double m[6];
   #pragma unroll
         for (int q = 0; q < 6; ++q)
-          m[q] = MQ ? (double)ldo((const int*)RS(q, a.s_lag), idx) * a.msc[q] : ldo((const double*)RM(q, a.s_lag), idx);
+          m[q] = MQ ? (double)ldo((const int*)RS(q, a.s_lag_met), idx) * a.msc[q]
+                    : ldo((const double*)RM(q, a.s_lag_met), idx);
         if (a.lag_mode == 2) {
           const double tn = norm_factor<double>(m[0], m[1], m[2], m[3], m[4], m[5]);
   #pragma unroll

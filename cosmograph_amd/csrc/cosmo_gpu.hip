@@ -221,11 +221,12 @@ __global__ void k_ext_reduce(const double* __restrict__ in, long rows, double* _
 // either way field f, slot s at met + (f*4 + s)*rstride)
 template <typename TS, typename MT>
 __global__ void k_dust_flux(const MT* __restrict__ met, MetScale ms, const TS* __restrict__ alpha,
-                            const TS* __restrict__ dreg, long rstride, int slot, int normalize, double phiF,
-                            TS* __restrict__ flux, long n) {
+                            const TS* __restrict__ dreg, long rstride, int slot, int mslot, int normalize,
+                            double phiF, TS* __restrict__ flux, long n) {
   const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= n) return;
-  auto M = [&](int f) { return (double)met[((long)f * 4 + slot) * rstride + x] * ms.s[f]; };
+  // DIFFgamma_ij at slot mslot (a normalised copy of slot's state, or slot)
+  auto M = [&](int f) { return (double)met[((long)f * 4 + (f < PHI ? mslot : slot)) * rstride + x] * ms.s[f]; };
   double g11 = M(G11), g12 = M(G12), g13 = M(G13), g22 = M(G22), g23 = M(G23), g33 = M(G33);
   if (normalize) {
     const double omd = -1.0 * (g11 + g22 + g33 - g12 * g12 - g13 * g13 - g23 * g23 + g11 * g22 + g11 * g33 +
@@ -883,7 +884,7 @@ StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int
   a.dust_bg = c->dust_bg ? 1 : 0;
   a.dreg = c->dreg;
   a.flux = c->flux;
-  a.s_lag = rin;
+  a.s_lag = a.s_lag_met = rin;
   a.lag_mode = 0;
   a.phi_F_lag = c->frw.g_phi;
   a.rho_out = c->rho_src;
@@ -1024,7 +1025,8 @@ static int dust_reg_init(cg_ctx* c, int rp, int ra, int rf) {
 }
 
 // fluxes of the dust fluid at every point of register slot `slot`
-static int dust_flux(cg_ctx* c, int slot, int normalize) {
+static int dust_flux(cg_ctx* c, int slot, int normalize, int mslot = -1) {
+  if (mslot < 0) mslot = slot;
   const int bs = 256;
   const long nb = (c->npts + bs - 1) / bs;
   MetScale ms, mi;
@@ -1032,15 +1034,15 @@ static int dust_flux(cg_ctx* c, int slot, int normalize) {
   if (c->mq) mq_scales(c, ms.s, mi.s);
   if (c->prec == CG_FP64)
     hipLaunchKernelGGL((k_dust_flux<double, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)c->met, ms,
-                       (const double*)reg_ptr(c, ALPHA, 0), (const double*)c->dreg, c->rstride, slot, normalize,
+                       (const double*)reg_ptr(c, ALPHA, 0), (const double*)c->dreg, c->rstride, slot, mslot, normalize,
                        c->frw.g_phi, (double*)c->flux, c->npts);
   else if (c->mq)
     hipLaunchKernelGGL((k_dust_flux<float, int>), dim3(nb), dim3(bs), 0, c->stream, (const int*)reg_ptr(c, 0, 0), ms,
-                       (const float*)reg_ptr(c, ALPHA, 0), (const float*)c->dreg, c->rstride, slot, normalize,
+                       (const float*)reg_ptr(c, ALPHA, 0), (const float*)c->dreg, c->rstride, slot, mslot, normalize,
                        c->frw.g_phi, (float*)c->flux, c->npts);
   else
     hipLaunchKernelGGL((k_dust_flux<float, double>), dim3(nb), dim3(bs), 0, c->stream, (const double*)c->met, ms,
-                       (const float*)reg_ptr(c, ALPHA, 0), (const float*)c->dreg, c->rstride, slot, normalize,
+                       (const float*)reg_ptr(c, ALPHA, 0), (const float*)c->dreg, c->rstride, slot, mslot, normalize,
                        c->frw.g_phi, (float*)c->flux, c->npts);
   HIPCHK(c, hipGetLastError());
   return CG_OK;
@@ -1074,7 +1076,7 @@ int cg_rk_evolve(cg_ctx* c) {
     if (int rc = dust_flux(c, c->sA, 0)) return rc;
     if (c->dust_rk > 0) {
       a.lag_mode = 1;
-      a.s_lag = c->sC;
+      a.s_lag = a.s_lag_met = c->sC;
       a.phi_F_lag = c->dust_lag_phiF;
     }
     c->dust_lag_phiF = c->frw.g_phi;
@@ -1165,7 +1167,8 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
   const bool flrw = lapse_needs_kavg(c->p.lapse);  // averages needed inside the step
   SrcArgs src4{};
   // dust: the sources of stage n use W, detg, g_ij of stage n-1's input
-  int prev_in = 0, prev_norm = 0;
+  // (its DIFFgamma_ij from the normalised copy when that stage read one)
+  int prev_in = 0, prev_met = 0, prev_norm = 0;
   double prev_phiF = 0.0;
   // Accumulator-free form (no _f sweeps; the KO path with normalisation
   // keeps the reference's accumulator)
@@ -1178,13 +1181,16 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
     a.incr = incr ? 1 : 0;
     a.s_a2 = sa2;
     if (c->dreg) {
-      if (int rc = dust_flux(c, rin, normalize)) return rc;
+      const int met_in = in_norm >= 0 ? in_norm : rin;
+      if (int rc = dust_flux(c, rin, normalize, met_in)) return rc;
       if (n > 1) {
         a.lag_mode = prev_norm ? 2 : 1;
         a.s_lag = prev_in;
+        a.s_lag_met = prev_met;
         a.phi_F_lag = prev_phiF;
       }
       prev_in = rin;
+      prev_met = met_in;
       prev_norm = normalize;
       prev_phiF = c->frw.g_phi;
     }
@@ -1206,7 +1212,7 @@ int cg_rk4_steps(cg_ctx* c, int nsteps) {
     // fixed-point metric: exponents checked every kMqCheck steps (a field's
     // words stay below 2^MQ_BITS; twice that saturates)
     if (c->mq && s % kMqCheck == 0 && (rc = mq_check(c))) return rc;
-    const bool pre = !ko_norm && !c->dreg && c->p.normalize_metric && s + 1 < nsteps && c->var &&
+    const bool pre = !ko_norm && c->p.normalize_metric && s + 1 < nsteps && c->var &&
                      c->dma_allowed && c->dma_pick && dma_fits(c, *c->var);
     if (incr) {
       // a1, a2, a3 rotate over slots 1..3 so that the normalised copy of the
@@ -1697,7 +1703,7 @@ int cg_autotune(cg_ctx* c, int reps, char* report, size_t report_len) {
             if (st == 1 && c->p.normalize_metric) {
               // steady state of the fused step: the normalised copy with
               // DMA staging (slot 2 stands in), on the fly without
-              if (d == 1 && !c->dreg && !ko_norm_path(c)) a.s_in_norm = 2;
+              if (d == 1 && !ko_norm_path(c)) a.s_in_norm = 2;
               else a.normalize = 1;
             }
             a.rho_out = nullptr;  // dust: DIFFr_a keeps the last real stage's source

@@ -1,7 +1,8 @@
 // Instantiations of the stage kernel with the dust fluid and the
 // fixed-point metric (every other field FP32), orders 4 and 6: the default
 // tile at one wave per SIMD and the 3-waves-per-SIMD tiles with their
-// LDS-DMA twins.
+// LDS-DMA twins; at order 6 also a 2-waves-per-SIMD budget, which holds the
+// kernel's live values without spilling (the autotuner keeps the faster).
 #include "bssn_stage_kernel.h"
 
 namespace cg {
@@ -20,4 +21,6 @@ CG_DQ(CG_REGISTER_DMQ, 3, 4, 8, 8, 1)
 CG_DQ(CG_REGISTER_DMQ_D, 2, 4, 8, 8, 3)
 CG_DQ(CG_REGISTER_DMQ_D, 3, 4, 8, 8, 3)
 CG_DQ(CG_REGISTER_DMQ_D, 3, 4, 4, 16, 3)
+CG_DQ(CG_REGISTER_DMQ_D, 3, 4, 8, 8, 2)
+CG_DQ(CG_REGISTER_DMQ_D, 3, 4, 4, 16, 2)
 }  // namespace cg

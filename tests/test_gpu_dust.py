@@ -55,8 +55,12 @@ def check(g, o, regs="p", tol=1e-12, names=None):
             assert err <= (tol if np.any(ref != 0) else 1e-30), (f, r, err)
 
 
-@pytest.mark.parametrize("order,frw,lam", [(4, "empty", 0.0), (6, "empty", 0.02), (4, "full", 0.0)])
+@pytest.mark.parametrize("order,frw,lam", [(4, "empty", 0.0), (6, "empty", 0.02), (4, "full", 0.0), (6, "full", 0.0)])
 def test_dust_fused_steps(oracle_mod, gpu, order, frw, lam):
+    """Three fused steps in one call.  At order 6 the LDS-DMA kernels run:
+    stages 4 of the first two steps write the normalised copy of the new
+    state, from which the next step's stage 1, its fluxes and stage 2's
+    lagged sources read the metric."""
     shape = (18, 16, 20)
     _say(f"case dust fused order={order} frw={frw} lam={lam}")
     o, g = dust_pair(oracle_mod, gpu, shape, order, EMPTY_FRW if frw == "empty" else FULL_FRW, lam)
