@@ -841,6 +841,48 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     if constexpr (DUST) {
       if (a.rho_out) a.rho_out[idx] = (double)DIFFr;
     }
+    if constexpr (DUST && !CONS) {
+      // dust right-hand sides (dust.cc:203-224): flux divergence plus the
+      // momentum source of populateDerivedFields (dust.cc:139-150), and the
+      // fluid's RK4 update, done here, before the BSSN algebra, so that none
+      // of their inputs (fluxes' divergence, v^i, W, D) stays live through it
+      TC drhs[NDUST];
+      drhs[0] = ddust[0];
+      const TC pre = TC(0.5) * alpha * Wc * Dc * exp(TC(4) * phi);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const TC dp4 = TC(4) * dphi[k];
+        const TC acc = vc[0] * vc[0] * (dp4 * GM(0, 0) + dg[k][0]) + vc[1] * vc[1] * (dp4 * GM(1, 1) + dg[k][3]) +
+                       vc[2] * vc[2] * (dp4 * GM(2, 2) + dg[k][5]) +
+                       TC(2) * (vc[0] * vc[1] * (dp4 * GM(0, 1) + dg[k][1]) + vc[0] * vc[2] * (dp4 * GM(0, 2) + dg[k][2]) +
+                                vc[1] * vc[2] * (dp4 * GM(1, 2) + dg[k][4]));
+        drhs[1 + k] = ddust[1 + k] + (pre * acc - Wc * Dc * dal[k]);
+      }
+#pragma unroll
+      for (int q = 0; q < NDUST; ++q) {
+        const double cc = (double)drhs[q];
+        TS* __restrict__ o = RD(q, a.s_out);
+        if (a.stage == 0) {
+          sto(o, idx, (TS)cc);
+          continue;
+        }
+        const double p = (double)ldo((const TS*)RD(q, a.s_p), idx);
+        TS* __restrict__ fr = RD(q, a.s_f);
+        double fo = 0.0;
+        if (!a.incr) {
+          if (a.stage >= 2) fo = (double)ldo((const TS*)fr, idx);
+        } else if (a.stage == 4) {
+          // accumulator-free form: K3Finalize's accumulator from the three stage outputs
+          const double a1 = (double)ldo((const TS*)fr, idx);
+          const double a2 = (double)ldo((const TS*)RD(q, a.s_a2), idx);
+          const double a3 = (double)ldo((const TS*)RD(q, a.s_in), idx);
+          fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
+        }
+        const double inc = a.cf * cc;
+        sto(o, idx, a.stage == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc));
+        if (a.stage != 4 && !a.incr) sto(fr, idx, (TS)(fo + inc));
+      }
+    }
 
     // A^ij and A_ij A^ij (bssn.cc:463-472)
     TC Ac[6];
@@ -1285,46 +1327,6 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
           else sto(RM(G11 + s, a.s_norm_out), idx, g[s]);
           sto(RS(A11 + s, a.s_norm_out), idx,
               (TS)((An[s] - (1.0 / 3.0) * ((diag6(s) ? 1.0 : 0.0) + g[s]) * trA) / (1.0 - tn)));
-        }
-      }
-      if constexpr (DUST) {
-        // dust right-hand sides (dust.cc:203-224): flux divergence plus the
-        // momentum source of populateDerivedFields (dust.cc:139-150)
-        TC drhs[NDUST];
-        drhs[0] = ddust[0];
-        const TC pre = TC(0.5) * alpha * Wc * Dc * exp(TC(4) * phi);
-  #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const TC dp4 = TC(4) * dphi[k];
-          const TC acc = vc[0] * vc[0] * (dp4 * GM(0, 0) + dg[k][0]) + vc[1] * vc[1] * (dp4 * GM(1, 1) + dg[k][3]) +
-                         vc[2] * vc[2] * (dp4 * GM(2, 2) + dg[k][5]) +
-                         TC(2) * (vc[0] * vc[1] * (dp4 * GM(0, 1) + dg[k][1]) + vc[0] * vc[2] * (dp4 * GM(0, 2) + dg[k][2]) +
-                                  vc[1] * vc[2] * (dp4 * GM(1, 2) + dg[k][4]));
-          drhs[1 + k] = ddust[1 + k] + (pre * acc - Wc * Dc * dal[k]);
-        }
-  #pragma unroll
-        for (int q = 0; q < NDUST; ++q) {
-          const double cc = (double)drhs[q];
-          TS* __restrict__ o = RD(q, a.s_out);
-          if (st == 0) {
-            sto(o, idx, (TS)cc);
-            continue;
-          }
-          const double p = (double)ldo((const TS*)RD(q, a.s_p), idx);
-          TS* __restrict__ fr = RD(q, a.s_f);
-          double fo = 0.0;
-          if (!a.incr) {
-            if (st >= 2) fo = (double)ldo((const TS*)fr, idx);
-          } else if (st == 4) {
-            // accumulator-free form: K3Finalize's accumulator from the three stage outputs
-            const double a1 = (double)ldo((const TS*)fr, idx);
-            const double a2 = (double)ldo((const TS*)RD(q, a.s_a2), idx);
-            const double a3 = (double)ldo((const TS*)RD(q, a.s_in), idx);
-            fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
-          }
-          const double inc = a.cf * cc;
-          sto(o, idx, st == 4 ? (TS)(fo + (inc + p)) : (TS)(p + a.ca * cc));
-          if (st != 4 && !a.incr) sto(fr, idx, (TS)(fo + inc));
         }
       }
     }  // constraint mode / stage

@@ -1,8 +1,9 @@
 // Instantiations of the stage kernel with the dust fluid and the
-// fixed-point metric (every other field FP32), orders 4 and 6: the default
-// tile at one wave per SIMD and the 3-waves-per-SIMD tiles with their
-// LDS-DMA twins; at order 6 also a 2-waves-per-SIMD budget, which holds the
-// kernel's live values without spilling (the autotuner keeps the faster).
+// fixed-point metric (every other field FP32), orders 4 and 6: tiles with
+// the register budget of 3 waves per SIMD and, at order 6, tiles without a
+// budget (the frozen-lapse kernels fit 3 waves in 166 VGPRs without
+// spilling, where the budgeted ones spill 24-36 B), each with its LDS-DMA
+// twin; the autotuner keeps the fastest.
 #include "bssn_stage_kernel.h"
 
 namespace cg {
@@ -17,10 +18,9 @@ constexpr int MODE_DUST_MQ = MODE_DUST | MODE_MQ_STAGE;
   REG(float, float, 3, R, R, TI, TJ, TK, WPE, false) \
   REG(float, float, 3, R, R, TI, TJ, TK, WPE, true)
 CG_DQ(CG_REGISTER_DMQ, 2, 4, 8, 8, 1)
-CG_DQ(CG_REGISTER_DMQ, 3, 4, 8, 8, 1)
+CG_DQ(CG_REGISTER_DMQ_D, 3, 4, 8, 8, 1)
 CG_DQ(CG_REGISTER_DMQ_D, 2, 4, 8, 8, 3)
 CG_DQ(CG_REGISTER_DMQ_D, 3, 4, 8, 8, 3)
 CG_DQ(CG_REGISTER_DMQ_D, 3, 4, 4, 16, 3)
-CG_DQ(CG_REGISTER_DMQ_D, 3, 4, 8, 8, 2)
-CG_DQ(CG_REGISTER_DMQ_D, 3, 4, 4, 16, 2)
+CG_DQ(CG_REGISTER_DMQ_D, 3, 4, 4, 16, 1)
 }  // namespace cg
