@@ -88,8 +88,13 @@ __device__ __host__ constexpr bool metric_field(int f) { return f <= G33 || f ==
 template <bool M32>
 __device__ __host__ constexpr bool mfield(int f) { return !M32 && metric_field(f); }
 
-// codes of the dust fluxes in the staging sequence (beyond the field codes)
+// codes of the dust fluxes in the staging sequence (beyond the field codes):
+// flux q*3 + i alone, or the three fluxes of quantity q in one tile
 constexpr int FLUX0 = 32;
+constexpr int FLUXG0 = 48;
+// codes of the two-field tiles of first-derivative fields: (K, Gamma^1),
+// (Gamma^2, Gamma^3)
+constexpr int KG0 = 60, KG1 = 61;
 
 // order in which fields are staged through LDS:
 // PHI, G11..G33, [ALPHA], KK, GAM1..GAM3, [12 dust fluxes], A11..A33 (the
@@ -115,20 +120,28 @@ __device__ __host__ constexpr int stencil_seq(int s) {
 // LDS buffer s % 3, and buffer 1 is FP32-sized, so every FP64 (metric) field
 // sits at a position s % 3 != 1:
 // PHI, KK, G11, G12, GAM1, G13, G22, GAM2, G23, G33, GAM3, [ALPHA],
-// [12 dust fluxes], A11..A33 (everything after the head is FP32).
-template <bool LA, bool DU = false>
+// [12 dust fluxes, or 4 tiles of three], A11..A33 (everything after the head
+// is FP32).  KG (every field 4 bytes): K and Gamma^i in two two-field tiles
+// at positions 2 and 5 (buffer 2, the larger one), the head
+// PHI, G11, (K, GAM1), G12, G13, (GAM2, GAM3), G22, G23, G33.
+template <bool LA, bool DU = false, bool GRP = false, bool KG = false>
 __device__ __host__ constexpr int stencil_seq_split(int s) {
   constexpr int head[11] = {PHI, KK, G11, G12, GAM1, G13, G22, GAM2, G23, G33, GAM3};
-  if (s < 11) return head[s];
+  constexpr int headk[9] = {PHI, G11, KG0, G12, G13, KG1, G22, G23, G33};
+  constexpr int NH = KG ? 9 : 11;
+  if (s < NH) return KG ? headk[s] : head[s];
   if (LA) {
-    if (s == 11) return ALPHA;
+    if (s == NH) return ALPHA;
     s -= 1;
   }
-  if (DU) {
-    if (s < 11 + NFLUX) return FLUX0 + (s - 11);
+  if (DU && GRP) {
+    if (s < NH + NDUST) return FLUXG0 + (s - NH);
+    s -= NDUST;
+  } else if (DU) {
+    if (s < NH + NFLUX) return FLUX0 + (s - NH);
     s -= NFLUX;
   }
-  return A11 + (s - 11 < 5 ? s - 11 : 5);
+  return A11 + (s - NH < 5 ? s - NH : 5);
 }
 
 // LDS DMA of 16 bytes per lane: lane l copies 16 bytes at base + voff into
@@ -523,7 +536,17 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   // stencil fields in order; A_ij only when KO is active or for the
   // constraints; the dust fluxes only in the dust stage
   constexpr bool FLUXES = DUST && !CONS;
-  constexpr int NSEQ = (LIVE_ALPHA ? 18 : 17) + (FLUXES ? NFLUX : 0);
+  // 4-byte DMA staging: the three fluxes of a quantity share one LDS tile,
+  // each with the halo of its own derivative axis only (x part: HI x TJ x
+  // TK, y part: TI x HJ x TK, z part: TI x TJ x HK4)
+  constexpr bool FGRP = DMA && FLUXES && SPLIT;
+  constexpr int FGX = HI * TJ * TK, FGY = TI * HJ * TK, FGZ = TI * TJ * HK4;
+  // every field 4 bytes, no KO: K and the Gamma^i, which only have first
+  // derivatives, go two to a tile, each field in the same three parts, where
+  // such a tile fits the larger staging buffers (the halo of 3 of order 6)
+  constexpr bool KGRP = DMA && M32 && KO_ORDER == 0 && !CONS &&
+                        (2 * (FGX + FGY + FGZ) / 4 + 63) / 64 <= (HV / 2 + 63) / 64;
+  constexpr int NSEQ = (LIVE_ALPHA ? 18 : 17) - (KGRP ? 2 : 0) + (FLUXES ? (FGRP ? NDUST : NFLUX) : 0);
   const int nseq = (use_ko || CONS) ? NSEQ : NSEQ - 6;
   if constexpr (!LIVE_ALPHA) {
 #pragma unroll
@@ -603,6 +626,27 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       } else if (CONS && f >= A11 && f <= A33) {                                                                         \
         const int q = f - A11;                                                                                           \
         dA[0][q] = fd1<R>(L, c4, SI4) * idx1; dA[1][q] = fd1<R>(L, c4, SJ4) * idx1; dA[2][q] = fd1<R>(L, c4, SK) * idx1; \
+      } else if (KGRP && f >= KG0) {                                                                                     \
+        /* two first-derivative fields in three parts each; the same points as the full tile */                         \
+        const int cx = ((li + H) * TJ + lj) * TK + lk, cy = FGX + (li * HJ + lj + H) * TK + lk;                          \
+        const int cz = FGX + FGY + (li * TJ + lj) * HK4 + lk + KH4;                                                      \
+        TC dd[2][3];                                                                                                     \
+        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                                                  \
+          dd[h][0] = fd1<R>(L, h * (FGX + FGY + FGZ) + cx, TJ * TK) * idx1;                                              \
+          dd[h][1] = fd1<R>(L, h * (FGX + FGY + FGZ) + cy, TK) * idx1;                                                   \
+          dd[h][2] = fd1<R>(L, h * (FGX + FGY + FGZ) + cz, 1) * idx1;                                                    \
+        }                                                                                                                \
+        if (f == KG0) {                                                                                                  \
+          _Pragma("unroll") for (int d = 0; d < 3; ++d) { dK[d] = dd[0][d]; dG[d][0] = dd[1][d]; }                      \
+        } else {                                                                                                         \
+          _Pragma("unroll") for (int d = 0; d < 3; ++d) { dG[d][1] = dd[0][d]; dG[d][2] = dd[1][d]; }                   \
+        }                                                                                                                \
+      } else if (FGRP && f >= FLUXG0) {                                                                                  \
+        /* the three fluxes of quantity q, each along its own axis, summed in the order x, y, z */                       \
+        const int q = f - FLUXG0;                                                                                        \
+        ddust[q] += fd1<R>(L, ((li + H) * TJ + lj) * TK + lk, TJ * TK) * idx1;                                           \
+        ddust[q] += fd1<R>(L, FGX + (li * HJ + lj + H) * TK + lk, TK) * idx1;                                            \
+        ddust[q] += fd1<R>(L, FGX + FGY + (li * TJ + lj) * HK4 + lk + KH4, 1) * idx1;                                    \
       } else if (FLUXES && f >= FLUX0) {                                                                                 \
         /* dust.cc:203-224: flux q*3 + i is differentiated along i only */                                               \
         const int q = (f - FLUX0) / 3, dir = (f - FLUX0) % 3;                                                            \
@@ -653,11 +697,78 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         qoff[m] = goff16(4 * (g < NQD ? g : 0), HK4, KH4);
       }
     }
+    // flux tiles: quad g of the x, y and z parts in turn; its element offset
+    // counts from flux 3q (x) of the quantity, the y and z fluxes being the
+    // next two arrays (the launcher keeps 3 * rstride * 4 bytes below 2^32)
+    constexpr int NQG = (FGX + FGY + FGZ) / 4, CFG = (NQG + 63) / 64, MFG = (CFG + NW - 1) / NW;
+    static_assert(!FGRP || (FGX % 4 == 0 && FGY % 4 == 0 && CFG <= CQD), "flux tiles: whole quads, one FP32 buffer");
+    auto fchunk = [&](int m) { const int ch = wv + m * NW; return ch < CFG ? ch : CFG - 1; };
+    unsigned foff[FGRP ? MFG : 1];
+    if constexpr (FGRP) {
+#pragma unroll
+      for (int m = 0; m < MFG; ++m) {
+        const int g = fchunk(m) * 64 + ln;
+        const int e = 4 * (g < NQG ? g : 0);
+        int x, y, z, d;
+        if (e < FGX) {
+          d = 0; x = i0 + e / (TJ * TK) - H; y = j0 + (e / TK) % TJ; z = k0 + e % TK;
+        } else if (e < FGX + FGY) {
+          const int r = e - FGX;
+          d = 1; x = i0 + r / (HJ * TK); y = j0 + (r / TK) % HJ - H; z = k0 + r % TK;
+        } else {
+          const int r = e - FGX - FGY;
+          d = 2; x = i0 + r / (TJ * HK4); y = j0 + (r / HK4) % TJ; z = k0 + r % HK4 - KH4;
+        }
+        x += x < 0 ? a.nx : 0; x -= x >= a.nx ? a.nx : 0;
+        y += y < 0 ? a.ny : 0; y -= y >= a.ny ? a.ny : 0;
+        z += z < 0 ? a.nz : 0; z -= z >= a.nz ? a.nz : 0;
+        foff[m] = (unsigned)d * (unsigned)a.rstride + (unsigned)((x * a.ny + y) * a.nz + z);
+      }
+    }
+    // two-field tiles of K and Gamma^i: quad g of field h = g / (NCR / 4),
+    // then its x, y and z parts as a flux tile; the second field's register
+    // is 4 * rstride elements after the first's (the launcher keeps
+    // 5 * rstride * 4 bytes below 2^32)
+    constexpr int NCR = FGX + FGY + FGZ, NQK = 2 * NCR / 4, CKG = (NQK + 63) / 64, MKG = (CKG + NW - 1) / NW;
+    static_assert(!KGRP || (FGX % 4 == 0 && FGY % 4 == 0 && CKG <= CPR), "two-field tiles: whole quads, buffer 2");
+    auto kchunk = [&](int m) { const int ch = wv + m * NW; return ch < CKG ? ch : CKG - 1; };
+    unsigned koff[KGRP ? MKG : 1];
+    if constexpr (KGRP) {
+#pragma unroll
+      for (int m = 0; m < MKG; ++m) {
+        const int g = kchunk(m) * 64 + ln;
+        int e = 4 * (g < NQK ? g : 0);
+        const int h = e >= NCR ? 1 : 0;
+        e -= h * NCR;
+        int x, y, z;
+        if (e < FGX) {
+          x = i0 + e / (TJ * TK) - H; y = j0 + (e / TK) % TJ; z = k0 + e % TK;
+        } else if (e < FGX + FGY) {
+          const int r = e - FGX;
+          x = i0 + r / (HJ * TK); y = j0 + (r / TK) % HJ - H; z = k0 + r % TK;
+        } else {
+          const int r = e - FGX - FGY;
+          x = i0 + r / (TJ * HK4); y = j0 + (r / HK4) % TJ; z = k0 + r % HK4 - KH4;
+        }
+        x += x < 0 ? a.nx : 0; x -= x >= a.nx ? a.nx : 0;
+        y += y < 0 ? a.ny : 0; y -= y >= a.ny ? a.ny : 0;
+        z += z < 0 ? a.nz : 0; z -= z >= a.nz ? a.nz : 0;
+        koff[m] = (unsigned)h * 4u * (unsigned)a.rstride + (unsigned)((x * a.ny + y) * a.nz + z);
+      }
+    }
     typedef __attribute__((address_space(3))) char lds_char;
     const unsigned lds0 = (unsigned)(size_t)(lds_char*)lb0, lds1 = lds0 + P64, lds2 = lds0 + P64 + P32;
     auto dma = [&](const int f, const int b) {
       const unsigned dst = b == 0 ? lds0 : b == 1 ? lds1 : lds2;
-      if (!SPLIT || mfield<M32>(f)) {
+      if (KGRP && f >= KG0) {
+        const void* src = (const void*)INS(f == KG0 ? KK : GAM2);
+#pragma unroll
+        for (int m = 0; m < MKG; ++m) dma16(src, koff[m] * 4u, dst + kchunk(m) * 1024);
+      } else if (FGRP && f >= FLUXG0) {
+        const void* src = (const void*)((const TS*)a.flux + (long)(f - FLUXG0) * 3 * a.rstride);
+#pragma unroll
+        for (int m = 0; m < MFG; ++m) dma16(src, foff[m] * 4u, dst + fchunk(m) * 1024);
+      } else if (!SPLIT || mfield<M32>(f)) {
         const void* src = mfield<M32>(f) ? (const void*)INM(f) : (const void*)INS(f);
 #pragma unroll
         for (int m = 0; m < MPR; ++m) dma16(src, poff[m] * 8u, dst + pchunk(m) * 1024);
@@ -668,7 +779,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       }
     };
     auto seq = [&](int s) {
-      return SPLIT ? stencil_seq_split<LIVE_ALPHA, FLUXES>(s) : stencil_seq<LIVE_ALPHA, FLUXES>(s);
+      return SPLIT ? stencil_seq_split<LIVE_ALPHA, FLUXES, FGRP, KGRP>(s) : stencil_seq<LIVE_ALPHA, FLUXES>(s);
     };
     dma(seq(0), 0);
     if (nseq > 1) dma(seq(1), 1);
@@ -679,7 +790,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         // position s + 1 are outstanding; for every wave after the barrier
         const int f1 = seq(s + 1 < NSEQ ? s + 1 : 0);
         if (s + 1 < nseq) {
-          if (!SPLIT || mfield<M32>(f1)) dma_wait_barrier<MPR>(); else dma_wait_barrier<MQD>();
+          if (!SPLIT || mfield<M32>(f1)) dma_wait_barrier<MPR>();
+          else if (KGRP && f1 >= KG0) dma_wait_barrier<MKG>();
+          else if (FGRP && f1 >= FLUXG0) dma_wait_barrier<MFG>();
+          else dma_wait_barrier<MQD>();
         } else {
           dma_wait_barrier<0>();
         }

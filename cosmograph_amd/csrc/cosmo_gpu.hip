@@ -456,9 +456,14 @@ bool variant_fits(const cg_ctx* c, const StageVariant& v) {
 
 // the LDS-DMA twin of v applies to this grid: compiled, nz % 4 == 0 (16-byte
 // groups never straddle the periodic wrap) and one wrap of the k halo
-// (rounded up to 4) suffices
+// (rounded up to 4) suffices; the 4-byte dust kernels address the three
+// fluxes of a quantity from the first one, and the kernels with every field
+// in 4 bytes Gamma^1 and Gamma^3 from K and Gamma^2, with a 32-bit byte offset
 bool dma_fits(const cg_ctx* c, const StageVariant& v) {
   if (!v.launch_dma) return false;
+  const unsigned long long rs4 = 4ull * (unsigned long long)c->rstride;
+  if (v.dust && v.prec != CG_FP64 && 3ull * rs4 > 0xFFFFFFFFull) return false;
+  if ((v.prec == CG_FP32_ALL || v.prec == CG_FP32_FIX) && 5ull * rs4 > 0xFFFFFFFFull) return false;
   const int kh = (v.halo + 3) / 4 * 4;
   return c->p.nz % 4 == 0 && c->p.nz >= kh && c->p.nz > v.tk + kh - 2;
 }
