@@ -131,8 +131,10 @@ struct StageArgs {
   int cons_pass;
   int tile_base, tile_count;
   // fixed-point metric (MODE_MQ): scale 2^e and its inverse of field f < 7,
+  // the scale times 1/dx and 1/dx^2 rounded to FP32 (the stencils' factors),
   // and the count of saturated words
   double msc[7], minv[7];
+  float msd1[7], msd2[7];
   unsigned int* mq_sat;
 };
 

@@ -174,6 +174,40 @@ __device__ __forceinline__ void dma16v(const void* addr, unsigned lds) {
 }
 #pragma clang diagnostic pop
 
+// e / D for 0 <= e < 2^13 as one 24-bit multiply and a shift, full-rate
+// instructions where a division by a constant becomes the quarter-rate
+// 32-bit high multiply; the reciprocal is checked for every e at compile time
+constexpr int ceil_log2(unsigned d) { return d <= 1 ? 0 : 1 + ceil_log2((d + 1) / 2); }
+template <unsigned D>
+struct SmallDiv {
+  static constexpr unsigned S = 13 + ceil_log2(D);
+  static constexpr unsigned M = ((1u << S) + D - 1) / D;
+  static constexpr bool exact() {
+    for (unsigned e = 0; e < (1u << 13); ++e)
+      if ((e * M) >> S != e / D) return false;
+    return M < (1u << 24);
+  }
+};
+template <unsigned D>
+__device__ __forceinline__ unsigned div13(unsigned e) {
+  static_assert(SmallDiv<D>::exact(), "24-bit reciprocal exact below 2^13");
+  return (unsigned)__umul24(e, SmallDiv<D>::M) >> SmallDiv<D>::S;
+}
+
+// Wrapped global element offset of element e (< 2^13) of a box of rows of
+// RK elements along k, RJ rows along j, starting at (x0, y0, z0) (at most
+// one period outside the grid).  24-bit multiplies: the launcher of the DMA
+// kernels keeps nx * ny below 2^24.
+template <int RK, int RJ>
+__device__ __forceinline__ unsigned box_off(int e, int x0, int y0, int z0, int nx, int ny, int nz) {
+  const unsigned r = div13<RK>((unsigned)e), hi = div13<RJ>(r);
+  int x = x0 + (int)hi, y = y0 + (int)(r - (unsigned)__umul24(hi, RJ)), z = z0 + (int)((unsigned)e - (unsigned)__umul24(r, RK));
+  x += x < 0 ? nx : 0; x -= x >= nx ? nx : 0;
+  y += y < 0 ? ny : 0; y -= y >= ny ? ny : 0;
+  z += z < 0 ? nz : 0; z -= z >= nz ? nz : 0;
+  return (unsigned)__umul24((unsigned)__umul24(x, ny) + y, nz) + (unsigned)z;
+}
+
 // End of one step of the DMA staging: this wave's vector memory operations
 // older than the newest K have completed (they retire in order), then the
 // block barrier.  No fence: the DMA loads of the next fields stay in flight
@@ -586,7 +620,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       }                                                                                                                  \
       if (MQW && (f <= G33 || f == PHI)) {                                                                               \
         /* exact integer differences of the words, FP32 sums, the field's scale */                                      \
-        const TC s1 = (TC)(a.msc[f < 7 ? f : 0] * i1), s2 = (TC)(a.msc[f < 7 ? f : 0] * i2);                           \
+        const TC s1 = a.msd1[f < 7 ? f : 0], s2 = a.msd2[f < 7 ? f : 0];                                                \
         const TC d0 = fd1qf<R>(LQ, c4, SI4) * s1, d1 = fd1qf<R>(LQ, c4, SJ4) * s1, d2 = fd1qf<R>(LQ, c4, SK) * s1;       \
         const TC e00 = fd2qf<R>(LQ, c4, SI4) * s2, e11 = fd2qf<R>(LQ, c4, SJ4) * s2, e22 = fd2qf<R>(LQ, c4, SK) * s2;    \
         const TC e01 = fdmqf<R>(LQ, c4, SI4, SJ4) * s2, e02 = fdmqf<R>(LQ, c4, SI4, SK) * s2,                           \
@@ -671,14 +705,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     // wrapped global element offset of halo element e of a tile with rows of
     // hk_n elements starting kh before the tile (e % 4 == 0 for quads, e % 2
     // == 0 for pairs: the group never leaves its row or wraps inside)
-    auto goff16 = [&](int e, int hk_n, int kh) -> unsigned {
-      const int hk = e % hk_n, hj = (e / hk_n) % HJ, hi = e / (hk_n * HJ);
-      int x = i0 + hi - H, y = j0 + hj - H, z = k0 + hk - kh;
-      x += x < 0 ? a.nx : 0; x -= x >= a.nx ? a.nx : 0;
-      y += y < 0 ? a.ny : 0; y -= y >= a.ny ? a.ny : 0;
-      z += z < 0 ? a.nz : 0; z -= z >= a.nz ? a.nz : 0;
-      return (unsigned)((x * a.ny + y) * a.nz + z);
-    };
+    static_assert(HV < (1 << 13) && 2 * (FGX + FGY + FGZ) < (1 << 13), "tile elements below 2^13");
     // chunk m of this wave is chunk min(wv + m*NW, last) of the field: lane l
     // takes group 64*chunk + l, which lands at LDS byte 16*(64*chunk + l)
     auto pchunk = [&](int m) { const int ch = wv + m * NW; return ch < CPR ? ch : CPR - 1; };
@@ -687,14 +714,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
 #pragma unroll
     for (int m = 0; m < MPR; ++m) {
       const int g = pchunk(m) * 64 + ln;
-      poff[m] = goff16(2 * (g < NPR ? g : 0), HK, KH);
+      poff[m] = box_off<HK, HJ>(2 * (g < NPR ? g : 0), i0 - H, j0 - H, k0 - KH, a.nx, a.ny, a.nz);
     }
     unsigned qoff[SPLIT ? MQD : 1];
     if constexpr (SPLIT) {
 #pragma unroll
       for (int m = 0; m < MQD; ++m) {
         const int g = qchunk(m) * 64 + ln;
-        qoff[m] = goff16(4 * (g < NQD ? g : 0), HK4, KH4);
+        qoff[m] = box_off<HK4, HJ>(4 * (g < NQD ? g : 0), i0 - H, j0 - H, k0 - KH4, a.nx, a.ny, a.nz);
       }
     }
     // flux tiles: quad g of the x, y and z parts in turn; its element offset
@@ -709,20 +736,16 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
       for (int m = 0; m < MFG; ++m) {
         const int g = fchunk(m) * 64 + ln;
         const int e = 4 * (g < NQG ? g : 0);
-        int x, y, z, d;
+        unsigned o;
+        int d;
         if (e < FGX) {
-          d = 0; x = i0 + e / (TJ * TK) - H; y = j0 + (e / TK) % TJ; z = k0 + e % TK;
+          d = 0; o = box_off<TK, TJ>(e, i0 - H, j0, k0, a.nx, a.ny, a.nz);
         } else if (e < FGX + FGY) {
-          const int r = e - FGX;
-          d = 1; x = i0 + r / (HJ * TK); y = j0 + (r / TK) % HJ - H; z = k0 + r % TK;
+          d = 1; o = box_off<TK, HJ>(e - FGX, i0, j0 - H, k0, a.nx, a.ny, a.nz);
         } else {
-          const int r = e - FGX - FGY;
-          d = 2; x = i0 + r / (TJ * HK4); y = j0 + (r / HK4) % TJ; z = k0 + r % HK4 - KH4;
+          d = 2; o = box_off<HK4, TJ>(e - FGX - FGY, i0, j0, k0 - KH4, a.nx, a.ny, a.nz);
         }
-        x += x < 0 ? a.nx : 0; x -= x >= a.nx ? a.nx : 0;
-        y += y < 0 ? a.ny : 0; y -= y >= a.ny ? a.ny : 0;
-        z += z < 0 ? a.nz : 0; z -= z >= a.nz ? a.nz : 0;
-        foff[m] = (unsigned)d * (unsigned)a.rstride + (unsigned)((x * a.ny + y) * a.nz + z);
+        foff[m] = (unsigned)d * (unsigned)a.rstride + o;
       }
     }
     // two-field tiles of K and Gamma^i: quad g of field h = g / (NCR / 4),
@@ -740,20 +763,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         int e = 4 * (g < NQK ? g : 0);
         const int h = e >= NCR ? 1 : 0;
         e -= h * NCR;
-        int x, y, z;
-        if (e < FGX) {
-          x = i0 + e / (TJ * TK) - H; y = j0 + (e / TK) % TJ; z = k0 + e % TK;
-        } else if (e < FGX + FGY) {
-          const int r = e - FGX;
-          x = i0 + r / (HJ * TK); y = j0 + (r / TK) % HJ - H; z = k0 + r % TK;
-        } else {
-          const int r = e - FGX - FGY;
-          x = i0 + r / (TJ * HK4); y = j0 + (r / HK4) % TJ; z = k0 + r % HK4 - KH4;
-        }
-        x += x < 0 ? a.nx : 0; x -= x >= a.nx ? a.nx : 0;
-        y += y < 0 ? a.ny : 0; y -= y >= a.ny ? a.ny : 0;
-        z += z < 0 ? a.nz : 0; z -= z >= a.nz ? a.nz : 0;
-        koff[m] = (unsigned)h * 4u * (unsigned)a.rstride + (unsigned)((x * a.ny + y) * a.nz + z);
+        unsigned o;
+        if (e < FGX) o = box_off<TK, TJ>(e, i0 - H, j0, k0, a.nx, a.ny, a.nz);
+        else if (e < FGX + FGY) o = box_off<TK, HJ>(e - FGX, i0, j0 - H, k0, a.nx, a.ny, a.nz);
+        else o = box_off<HK4, TJ>(e - FGX - FGY, i0, j0, k0 - KH4, a.nx, a.ny, a.nz);
+        koff[m] = (unsigned)h * 4u * (unsigned)a.rstride + o;
       }
     }
     typedef __attribute__((address_space(3))) char lds_char;
@@ -823,23 +837,29 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         auto poff = [&](int q) -> unsigned {  // element offset of block point q, 0 outside the grid
           const int qk = q % TK, qj = (q / TK) % TJ, qi = q / (TK * TJ);
           const int x = i0 + qi, y = j0 + qj, z = k0 + qk;
-          return (x < a.nx && y < a.ny && z < a.nz) ? (unsigned)((x * a.ny + y) * a.nz + z) : 0u;
+          return (x < a.nx && y < a.ny && z < a.nz) ? (unsigned)__umul24((unsigned)__umul24(x, a.ny) + y, a.nz) + z
+                                                    : 0u;
         };
         const unsigned o2 = poff(64 * wv + 2 * (ln & 31)), o4 = poff(64 * wv + 4 * (ln & 15));
         const unsigned reg = lds0 + wv * PFR;
         const bool need_f = a.incr ? a.stage == 4 : a.stage >= 2;
+        // each lane's address: a wave-uniform field base plus the lane's
+        // element offset (its slot, and for FP32 chunks its field of the
+        // pair), formed once
         const bool f64 = (ln >> 5) != 0;  // FP64 chunk: lanes 32-63 load _f
+        const long l64 = (long)(f64 ? a.s_f : a.s_p) * a.rstride + o2;
   #pragma unroll
         for (int m = 0; m < N64; ++m) {
-          const char* b = m < 7 ? (const char*)RM(m, f64 ? a.s_f : a.s_p) : (const char*)RS(m, f64 ? a.s_f : a.s_p);
-          if (!f64 || need_f) dma16v(b + (size_t)o2 * 8u, reg + m * 1024);
+          const char* b = m < 7 ? (const char*)RM(m, 0) : (const char*)RS(m, 0);
+          if (!f64 || need_f) dma16v(b + l64 * 8, reg + m * 1024);
         }
         const bool f32 = ((ln >> 4) & 1) != 0;  // FP32 chunk: lanes 16-31 and 48-63 load _f
+        const long l32 = (long)((ln >> 5) * 4 + (f32 ? a.s_f : a.s_p)) * a.rstride + o4;
   #pragma unroll
         for (int i = 0; i < (N32 + 1) / 2; ++i) {
-          const int n = 2 * i + (ln >> 5);
-          const char* b = (const char*)RS(F32 + (n < N32 ? n : 0), f32 ? a.s_f : a.s_p);
-          if (n < N32 && (!f32 || need_f)) dma16v(b + (size_t)o4 * 4u, reg + (N64 + i) * 1024);
+          const int n = 2 * i + (ln >> 5);  // field F32 + n, 4 * rstride elements after field F32 + 2i for odd lanes
+          const char* b = (const char*)RS(F32 + 2 * i, 0);
+          if (n < N32 && (!f32 || need_f)) dma16v(b + l32 * 4, reg + (N64 + i) * 1024);
         }
       }
     }

@@ -458,9 +458,11 @@ bool variant_fits(const cg_ctx* c, const StageVariant& v) {
 // groups never straddle the periodic wrap) and one wrap of the k halo
 // (rounded up to 4) suffices; the 4-byte dust kernels address the three
 // fluxes of a quantity from the first one, and the kernels with every field
-// in 4 bytes Gamma^1 and Gamma^3 from K and Gamma^2, with a 32-bit byte offset
+// in 4 bytes Gamma^1 and Gamma^3 from K and Gamma^2, with a 32-bit byte
+// offset; their halo offsets are formed with 24-bit multiplies (nx * ny < 2^24)
 bool dma_fits(const cg_ctx* c, const StageVariant& v) {
   if (!v.launch_dma) return false;
+  if ((long long)c->p.nx * c->p.ny >= (1ll << 24)) return false;
   const unsigned long long rs4 = 4ull * (unsigned long long)c->rstride;
   if (v.dust && v.prec != CG_FP64 && 3ull * rs4 > 0xFFFFFFFFull) return false;
   if ((v.prec == CG_FP32_ALL || v.prec == CG_FP32_FIX) && 5ull * rs4 > 0xFFFFFFFFull) return false;
@@ -875,6 +877,10 @@ StageArgs make_args(cg_ctx* c, int stage, int rin, int rp, int rf, int rout, int
   a.use_H = (c->p.a_adj_amp != 0.0 || c->p.k_damping_amp != 0.0);
   a.inv_dx = 1.0 / c->p.dx;
   a.inv_dx2 = 1.0 / (c->p.dx * c->p.dx);
+  for (int f = 0; f < 7; ++f) {
+    a.msd1[f] = (float)(a.msc[f] * a.inv_dx);
+    a.msd2[f] = (float)(a.msc[f] * a.inv_dx2);
+  }
   a.dt = c->p.dt;
   // RK4Register weights (RK4Register.h:129-176)
   const double dt = c->p.dt;
