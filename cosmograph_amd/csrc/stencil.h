@@ -77,8 +77,10 @@ __device__ __forceinline__ T fdm(const T* s, int c, int sa, int sb) {
 // the relative precision of the derivative itself; FP32 storage of the
 // values would instead lose it to the cancellation of neighbouring values.
 __device__ __forceinline__ float qd(int a, int b) { return (float)(int)((unsigned)a - (unsigned)b); }
-__device__ __forceinline__ float qdd(int a, int b, int c0) {
-  return (float)(int)(((unsigned)a - (unsigned)c0) + ((unsigned)b - (unsigned)c0));
+// (a - c0) + (b - c0) as (a + b) - 2 c0: the same value in wrap-around
+// arithmetic, one operation fewer (c2 = 2 c0 once per stencil)
+__device__ __forceinline__ float qdd(int a, int b, unsigned c2) {
+  return (float)(int)(((unsigned)a + (unsigned)b) - c2);
 }
 
 template <int R>
@@ -91,10 +93,10 @@ __device__ __forceinline__ float fd1qf(const int* s, int c, int st) {
 
 template <int R>
 __device__ __forceinline__ float fd2qf(const int* s, int c, int st) {
-  const int q0 = s[c];
-  float acc = float(c2<R>(1)) * qdd(s[c + st], s[c - st], q0);
+  const unsigned q2 = 2u * (unsigned)s[c];
+  float acc = float(c2<R>(1)) * qdd(s[c + st], s[c - st], q2);
 #pragma unroll
-  for (int m = 2; m <= R; ++m) acc += float(c2<R>(m)) * qdd(s[c + m * st], s[c - m * st], q0);
+  for (int m = 2; m <= R; ++m) acc += float(c2<R>(m)) * qdd(s[c + m * st], s[c - m * st], q2);
   return acc;
 }
 
