@@ -152,6 +152,19 @@ def test_dust_autotune_leaves_the_state(oracle_mod, gpu):
         assert np.array_equal(a.download(f + "_p"), b.download(f + "_p")), f
 
 
+@pytest.mark.parametrize("prec", ["fp32_fix", "fp32_split"])
+def test_dust_autotuned_vs_oracle(oracle_mod, gpu, prec):
+    """The autotuner's pick (any tile, register budget and staging path that
+    fits) steps the 4-byte dust modes to the oracle's bars."""
+    shape = (16, 16, 40)
+    _say(f"case dust autotuned {prec}")
+    o, g = dust_pair(oracle_mod, gpu, shape, 6, FULL_FRW, 0.0, precision=prec)
+    g.autotune(1)
+    o.run_steps(3)
+    g.step(3)
+    check(g, o, tol=1e-5)
+
+
 def _dust_ics(gpu, n, prec, lam, bg, modes):
     g = gpu.BSSN(gpu.GpuParams(n=n, order=6, precision=prec, matter="dust", lambda_=lam))
     g.set_dust_background(bg)

@@ -140,8 +140,12 @@ def test_autotuned_variant_parity(oracle_mod, gpu):
 
 
 @pytest.mark.parametrize("tile,precision,tol", [((2, 8, 16), "fp64", 1e-12), ((4, 4, 16), "fp64", 1e-12),
-                                                ((8, 8, 8), "fp32_split", 1e-5)])
+                                                ((8, 8, 8), "fp32_split", 1e-5), ((4, 4, 16), "fp32_fix", 2e-5),
+                                                ((4, 8, 8), "fp32_fix", 2e-5), ((4, 4, 16), "fp32_all", 1e-4)])
 def test_tile_variants_parity(oracle_mod, gpu, tile, precision, tol):
+    """Each tile against the oracle on a grid ragged along k (nz = 40), with
+    LDS-DMA staging: in the 4-byte modes at order 6 that includes the two-field
+    tiles of K and the Gamma^i."""
     shape = (24, 24, 40)
     _say(f"case tile {tile} {precision}")
     st = smooth_random_state(shape, seed=12)
