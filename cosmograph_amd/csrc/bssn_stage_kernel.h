@@ -40,6 +40,14 @@ __device__ __forceinline__ T norm_factor(T g11, T g12, T g13, T g22, T g23, T g3
   // bssn.cc:75-88:  1 - det(1 + DIFFgamma), then -(det^(1/3) - 1)
   T omd = -(g11 + g22 + g33 - g12 * g12 - g13 * g13 - g23 * g23 + g11 * g22 + g11 * g33 + g22 * g33 -
             g23 * g23 * g11 - g13 * g13 * g22 - g12 * g12 * g33 + T(2) * g12 * g13 * g23 + g11 * g22 * g33);
+  // 1 - (1 - x)^(1/3) for |x| < 2^-10 from its binomial series,
+  // x/3 + x^2/9 + 5x^3/81 + 10x^4/243 + 22x^5/729 + 154x^6/6561 + ..., whose
+  // first omitted term is below 2^-52 of the sum there (the unit-determinant
+  // drift of one step is far smaller); log1p/expm1 otherwise
+  if (fabs(omd) < T(0x1p-10))
+    return omd * (T(1.0 / 3.0) +
+                  omd * (T(1.0 / 9.0) +
+                         omd * (T(5.0 / 81.0) + omd * (T(10.0 / 243.0) + omd * (T(22.0 / 729.0) + omd * T(154.0 / 6561.0))))));
   return -expm1(log1p(-omd) / T(3));
 }
 
