@@ -299,8 +299,19 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   constexpr int F32 = M32 ? 0 : 7;                              // first field of the FP32 prefetch chunks
   constexpr int N64 = M32 ? 0 : SPLIT ? 7 : (LIVE_ALPHA ? NF : NF - 1);  // FP64-stored fields updated (ALPHA last)
   constexpr int N32 = M32 ? (LIVE_ALPHA ? NF : NF - 1) : SPLIT ? (LIVE_ALPHA ? NF - 7 : NF - 8) : 0;  // FP32 ones
-  constexpr int PFR = (N64 + (N32 + 1) / 2) * 1024;             // prefetch bytes per wave
-  constexpr int LSTAGE = 2 * P64 + P32, LPF = PF ? NW * PFR : 0;
+  // every field 4 bytes, frozen lapse: stage 4 of the accumulator-free form
+  // also prefetches a2 of the first NA2 fields, in the groups (16 lanes, 64
+  // points, one field) of the staging LDS left after the _p/_f groups: more
+  // LDS per block would leave 2 blocks per CU instead of 3 (a 52 KB layout
+  // with every a2 measured stage 4 0.21 ms faster and stages 1-3 0.12 ms
+  // slower at 256^3)
+  constexpr int LSTAGE = 2 * P64 + P32;
+  constexpr bool PF2 = PF && M32 && !LIVE_ALPHA;
+  constexpr int CPF = (2 * N32 + 3) / 4;                        // _p/_f chunks of the FP32 fields
+  constexpr int NA2_ = PF2 ? (LSTAGE / (NW * 1024) - N64 - CPF) * 4 : 0;
+  constexpr int NA2 = NA2_ < 0 ? 0 : NA2_ > N32 ? N32 : NA2_;   // a2 fields, from chunk N64 + CPF on
+  constexpr int PFR = (N64 + CPF + (NA2 + 3) / 4) * 1024;       // prefetch bytes per wave
+  constexpr int LPF = PF ? NW * PFR : 0;
   __shared__ __align__(16) char ldma[DMA ? (LSTAGE > LPF ? LSTAGE : LPF) : 16];
 
   // Register path: the halo tile is loaded in pairs of k-neighbours (HK is even, so a pair
@@ -869,6 +880,15 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
           const char* b = (const char*)RS(F32 + 2 * i, 0);
           if (n < N32 && (!f32 || need_f)) dma16v(b + l32 * 4, reg + (N64 + i) * 1024);
         }
+        if (PF2 && a.incr && a.stage == 4) {
+          // a2 of field F32 + 4i + lane/16 in chunk N64 + CPF + i
+          const long la = (long)((ln >> 4) * 4 + a.s_a2) * a.rstride + o4;
+  #pragma unroll
+          for (int i = 0; i < (NA2 + 3) / 4; ++i) {
+            const char* b = (const char*)RS(F32 + 4 * i, 0);
+            if (4 * i + (ln >> 4) < NA2) dma16v(b + la * 4, reg + (N64 + CPF + i) * 1024);
+          }
+        }
       }
     }
   } else {
@@ -1310,6 +1330,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         if (MQ && f < 7) return (double)*reinterpret_cast<const int*>(w) * a.msc[f < 7 ? f : 0];
         return (double)*reinterpret_cast<const TS*>(w);
       };
+      // stage 4: a2 of field f (f - F32 < NA2) from the prefetch
+      auto la2 = [&](int f) -> double {
+        const int n = f - F32;
+        const char* w = pfr + (N64 + CPF + (n >> 2)) * 1024 + (n & 3) * 256 + pq * 4;
+        if (MQ && f < 7) return (double)*reinterpret_cast<const int*>(w) * a.msc[f < 7 ? f : 0];
+        return (double)*reinterpret_cast<const TS*>(w);
+      };
       if constexpr (PF) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       // one loop per RK4 form, each with only its own loads and stores
       auto epilogue = [&](auto incr_tag) {
@@ -1340,7 +1367,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
                 a2v[g] = ldo((const double*)RM(g, a.s_a2), idx);
                 a3m[g] = ldo(INM(g), idx);
               } else {
-                a2v[g] = lw(a.s_a2);
+                a2v[g] = PF2 && g - F32 < NA2 ? la2(g) : lw(a.s_a2);
                 // the fixed-point metric's stage-4 input, re-read here rather
                 // than kept in registers through the stencil phase
                 if (MQ && g < 7) a3m[g < 7 ? g : 0] = lw(a.s_in);
