@@ -295,7 +295,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   // compute (constraint mode) they hold the scaled FP64 values instead
   constexpr bool MQW = MQ && sizeof(TC) == 4;
   static_assert(!MQ || sizeof(TS) == 4, "the fixed-point metric shares the 4-byte storage");
-  constexpr bool PF = DMA && (MODE & ~(MODE_M32 | MODE_MQ)) == 0;  // epilogue prefetch
+  // epilogue prefetch: the stage kernels, and with every field in 4 bytes
+  // the dust ones too (which then load the fluid's point values first)
+  constexpr bool PF = DMA && ((MODE & ~(MODE_M32 | MODE_MQ)) == 0 ||
+                              (M32 && (MODE & ~(MODE_M32 | MODE_MQ | MODE_DUST)) == 0));
+  constexpr bool PFD = PF && (MODE & MODE_DUST) != 0;
   constexpr int F32 = M32 ? 0 : 7;                              // first field of the FP32 prefetch chunks
   constexpr int N64 = M32 ? 0 : SPLIT ? 7 : (LIVE_ALPHA ? NF : NF - 1);  // FP64-stored fields updated (ALPHA last)
   constexpr int N32 = M32 ? (LIVE_ALPHA ? NF : NF - 1) : SPLIT ? (LIVE_ALPHA ? NF - 7 : NF - 8) : 0;  // FP32 ones
@@ -554,6 +558,35 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   if constexpr (DMA) {
     if (a.matter == MATTER_STATIC) Dpt = (TC)ldo((const TS*)a.DIFFD, idx);
   }
+  // dust fluid with the epilogue prefetch: its stage input, RK4 inputs and
+  // lagged state at the point, loaded after the staging and waited for
+  // before the prefetch is issued (vector memory operations retire in order:
+  // waiting for them in the algebra would also wait for the prefetch)
+  static_assert(!PFD || MQ, "the prefetching dust kernels hold the metric as fixed-point words");
+  constexpr int ND = PFD ? NDUST : 1;
+  TS pdi[ND], pdp[ND], pd1[ND], pd2[ND], pdl[ND];
+  int plw[PFD ? 7 : 1];
+  auto dust_preload = [&]() {
+    if constexpr (PFD) {
+      const bool rk = a.stage >= 1, f1 = a.incr ? a.stage == 4 : a.stage >= 2, f2 = a.incr && a.stage == 4;
+      const bool lag = a.lag_mode != 0;
+#pragma unroll
+      for (int q = 0; q < NDUST; ++q) {
+        pdi[q] = ldo((const TS*)RD(q, a.s_in), idx);
+        pdp[q] = rk ? ldo((const TS*)RD(q, a.s_p), idx) : TS(0);
+        pd1[q] = f1 ? ldo((const TS*)RD(q, a.s_f), idx) : TS(0);
+        pd2[q] = f2 ? ldo((const TS*)RD(q, a.s_a2), idx) : TS(0);
+        pdl[q] = lag ? ldo((const TS*)RD(q, a.s_lag), idx) : TS(0);
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) plw[q] = lag ? ldo((const int*)RS(q, a.s_lag_met), idx) : 0;
+      plw[6] = lag ? ldo((const int*)RS(PHI, a.s_lag), idx) : 0;
+#pragma unroll
+      for (int q = 0; q < NDUST; ++q) asm volatile("" ::"v"(pdi[q]), "v"(pdp[q]), "v"(pd1[q]), "v"(pd2[q]), "v"(pdl[q]));
+#pragma unroll
+      for (int q = 0; q < 7; ++q) asm volatile("" ::"v"(plw[q]));
+    }
+  };
 
   // inverse conformal metric from DIFF variables (bssn.cc:447-455)
   TC gi[6];
@@ -843,6 +876,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     // cost nothing, instead of at their first use in the algebra, where the
     // epilogue prefetch is in flight (vector memory operations retire in
     // order, so those waits would also wait for the prefetch).
+    dust_preload();
   #pragma unroll
     for (int f = 0; f < NF; ++f) asm volatile("" ::"v"(v[f]));
     asm volatile("" ::"v"(Dpt));
@@ -934,9 +968,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
     }
     if constexpr (DUST) {
       // Dust::getDustData at the point (dust.cc:177-200)
-      Dc = (TC)ldo((const TS*)RD(0, a.s_in), idx);
+      Dc = PFD ? (TC)pdi[0] : (TC)ldo((const TS*)RD(0, a.s_in), idx);
   #pragma unroll
-      for (int j = 0; j < 3; ++j) Sc[j] = (TC)ldo((const TS*)RD(1 + j, a.s_in), idx);
+      for (int j = 0; j < 3; ++j) Sc[j] = PFD ? (TC)pdi[1 + j] : (TC)ldo((const TS*)RD(1 + j, a.s_in), idx);
       {
         const TC em4 = exp(TC(-4) * phi), e6 = exp(TC(6) * phi);
         const TC giSS = em4 * (gi[0] * Sc[0] * Sc[0] + gi[3] * Sc[1] * Sc[1] + gi[5] * Sc[2] * Sc[2] +
@@ -955,20 +989,22 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         double m[6];
   #pragma unroll
         for (int q = 0; q < 6; ++q)
-          m[q] = MQ ? (double)ldo((const int*)RS(q, a.s_lag_met), idx) * a.msc[q]
-                    : ldo((const double*)RM(q, a.s_lag_met), idx);
+          m[q] = PFD  ? (double)plw[PFD ? q : 0] * a.msc[q]
+                 : MQ ? (double)ldo((const int*)RS(q, a.s_lag_met), idx) * a.msc[q]
+                      : ldo((const double*)RM(q, a.s_lag_met), idx);
         if (a.lag_mode == 2) {
           const double tn = norm_factor<double>(m[0], m[1], m[2], m[3], m[4], m[5]);
   #pragma unroll
           for (int q = 0; q < 6; ++q) m[q] = (m[q] + (diag6(q) ? tn : 0.0)) / (1.0 - tn);
         }
-        const TC phl = (TC)((MQ ? (double)ldo((const int*)RS(PHI, a.s_lag), idx) * a.msc[PHI]
-                                : ldo((const double*)RM(PHI, a.s_lag), idx)) +
+        const TC phl = (TC)((PFD  ? (double)plw[PFD ? 6 : 0] * a.msc[PHI]
+                             : MQ ? (double)ldo((const int*)RS(PHI, a.s_lag), idx) * a.msc[PHI]
+                                  : ldo((const double*)RM(PHI, a.s_lag), idx)) +
                             a.phi_F_lag);
-        const TC Dl = (TC)ldo((const TS*)RD(0, a.s_lag), idx);
+        const TC Dl = PFD ? (TC)pdl[0] : (TC)ldo((const TS*)RD(0, a.s_lag), idx);
         TC Sl[3];
   #pragma unroll
-        for (int j = 0; j < 3; ++j) Sl[j] = (TC)ldo((const TS*)RD(1 + j, a.s_lag), idx);
+        for (int j = 0; j < 3; ++j) Sl[j] = PFD ? (TC)pdl[PFD ? 1 + j : 0] : (TC)ldo((const TS*)RD(1 + j, a.s_lag), idx);
         const TC g11 = (TC)m[0], g12 = (TC)m[1], g13 = (TC)m[2], g22 = (TC)m[3], g23 = (TC)m[4], g33 = (TC)m[5];
         const TC i11 = TC(1) + g22 + g33 - g23 * g23 + g22 * g33, i22 = TC(1) + g11 + g33 - g13 * g13 + g11 * g33;
         const TC i33 = TC(1) + g11 + g22 - g12 * g12 + g11 * g22, i12 = g13 * g23 - g12 * (TC(1) + g33);
@@ -1028,16 +1064,17 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
           sto(o, idx, (TS)cc);
           continue;
         }
-        const double p = (double)ldo((const TS*)RD(q, a.s_p), idx);
+        const int qq = PFD ? q : 0;
+        const double p = PFD ? (double)pdp[qq] : (double)ldo((const TS*)RD(q, a.s_p), idx);
         TS* __restrict__ fr = RD(q, a.s_f);
         double fo = 0.0;
         if (!a.incr) {
-          if (a.stage >= 2) fo = (double)ldo((const TS*)fr, idx);
+          if (a.stage >= 2) fo = PFD ? (double)pd1[qq] : (double)ldo((const TS*)fr, idx);
         } else if (a.stage == 4) {
           // accumulator-free form: K3Finalize's accumulator from the three stage outputs
-          const double a1 = (double)ldo((const TS*)fr, idx);
-          const double a2 = (double)ldo((const TS*)RD(q, a.s_a2), idx);
-          const double a3 = (double)ldo((const TS*)RD(q, a.s_in), idx);
+          const double a1 = PFD ? (double)pd1[qq] : (double)ldo((const TS*)fr, idx);
+          const double a2 = PFD ? (double)pd2[qq] : (double)ldo((const TS*)RD(q, a.s_a2), idx);
+          const double a3 = PFD ? (double)pdi[qq] : (double)ldo((const TS*)RD(q, a.s_in), idx);
           fo = ((a1 - p) + 2.0 * (a2 - p) + (a3 - p)) * (1.0 / 3.0);
         }
         const double inc = a.cf * cc;
