@@ -152,6 +152,19 @@ def test_dust_autotune_leaves_the_state(oracle_mod, gpu):
         assert np.array_equal(a.download(f + "_p"), b.download(f + "_p")), f
 
 
+@pytest.mark.parametrize("prec,lapse", [("fp32_fix", "OnePlusLog"), ("fp64", "Harmonic")])
+def test_dust_live_lapse_vs_oracle(oracle_mod, gpu, prec, lapse):
+    """A live lapse with the dust fluid (the kernels that stage and evolve
+    DIFFalpha), order 6 on a grid with LDS-DMA staging: the fixed-point
+    kernels with their epilogue prefetch, and FP64."""
+    shape = (16, 16, 40)
+    _say(f"case dust live lapse {prec} {lapse}")
+    o, g = dust_pair(oracle_mod, gpu, shape, 6, FULL_FRW, 0.0, lapse=lapse, precision=prec)
+    o.run_steps(3)
+    g.step(3)
+    check(g, o, tol=1e-5 if prec != "fp64" else 1e-12, names=FIELDS + DUST)
+
+
 @pytest.mark.parametrize("prec", ["fp32_fix", "fp32_split"])
 def test_dust_autotuned_vs_oracle(oracle_mod, gpu, prec):
     """The autotuner's pick (any tile, register budget and staging path that
