@@ -152,6 +152,20 @@ def test_dust_autotune_leaves_the_state(oracle_mod, gpu):
         assert np.array_equal(a.download(f + "_p"), b.download(f + "_p")), f
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32_fix", 1e-5)])
+def test_dust_reference_rk4_form_vs_oracle(oracle_mod, gpu, prec, tol):
+    """Fused dust steps with the reference's _f accumulator (the normalised
+    copy then feeds stage 1 through slot Y, and stage 2's lagged metric
+    comes from there) against the oracle."""
+    shape = (16, 16, 40)
+    _say(f"case dust reference RK4 form {prec}")
+    o, g = dust_pair(oracle_mod, gpu, shape, 6, FULL_FRW, 0.0, precision=prec)
+    g.set_fused_accumulator(True)
+    o.run_steps(3)
+    g.step(3)
+    check(g, o, tol=tol)
+
+
 @pytest.mark.parametrize("prec,lapse", [("fp32_fix", "OnePlusLog"), ("fp64", "Harmonic")])
 def test_dust_live_lapse_vs_oracle(oracle_mod, gpu, prec, lapse):
     """A live lapse with the dust fluid (the kernels that stage and evolve
