@@ -52,6 +52,9 @@ CASES = [
     ("fp64", 6, (16, 16, 18), "Harmonic", True),     # nz % 4 != 0: register staging, no normalised copy
     ("fp32_split", 6, (32, 32, 32), "Static", True),
     ("fp32_split", 4, (20, 24, 16), "Harmonic", False),
+    ("fp32_fix", 6, (32, 32, 32), "Static", True),       # stage 4 prefetches a2 (accumulator-free form)
+    ("fp32_fix", 6, (20, 24, 16), "OnePlusLog", True),
+    ("fp32_all", 6, (32, 32, 32), "Static", True),
 ]
 
 
