@@ -159,6 +159,42 @@ def test_batch_nan_guard_names_realisation(gpu):
     assert b.members[0].numNaNs("DIFFphi_p") == 0
 
 
+def test_batch_ensemble_sizes_vs_oracle(oracle_mod, gpu):
+    """One realisation of each grid size of the ensemble configuration (48^3,
+    96^3, 128^3; order 4, P(k) ~ k^-2 constraint-solved ICs, rms 1e-2) in one
+    batch, 2 steps, against the oracle: FP64 to 1e-12, FP32 split to 1e-5."""
+    from cosmograph_amd.ics import static_dust_ics
+    sizes = (48, 96, 128)
+    ics = [static_dust_ics(n, -2.0, 1e-2, seed=n) for n in sizes]
+    want = []
+    for n, (fields, frw) in zip(sizes, ics):
+        _say(f"case ensemble sizes: oracle {n}^3")
+        o = oracle_mod.Sim(oracle_mod.make_params((n, n, n), order=4))
+        for name, arr in fields.items():
+            o.field(name)[:] = arr
+        o.frw_set(frw["phi"], frw["K"])
+        for rho, w in frw["fluids"]:
+            o.frw_add_fluid(rho, w)
+        o.run_steps(2)
+        want.append({f: o.field(f + "_p").copy() for f in FIELDS})
+        del o
+    for prec, tol in (("fp64", 1e-12), ("fp32_split", 1e-5)):
+        _say(f"case ensemble sizes: batch {prec}")
+        b = gpu.Batch([gpu.GpuParams(n=n, order=4, precision=prec, matter="static") for n in sizes])
+        for m, (fields, frw) in zip(b.members, ics):
+            for name, arr in fields.items():
+                m.upload(name, arr)
+            m.frw_set(frw["phi"], frw["K"])
+            for rho, w in frw["fluids"]:
+                m.frw_add_fluid(rho, w)
+        b.step(2)
+        for n, m, w in zip(sizes, b.members, want):
+            for f in FIELDS:
+                err = rel_l2(m.download(f + "_p"), w[f])
+                assert err <= (tol if np.any(w[f]) else 1e-30), (prec, n, f, err)
+        del b
+
+
 def test_batch_rejects_mixed_precision(gpu):
     with pytest.raises(gpu.CosmoGpuError):
         gpu.Batch([gpu.GpuParams(n=16), gpu.GpuParams(n=16, precision="fp32_split")])
