@@ -305,10 +305,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   constexpr int N32 = M32 ? (LIVE_ALPHA ? NF : NF - 1) : SPLIT ? (LIVE_ALPHA ? NF - 7 : NF - 8) : 0;  // FP32 ones
   // every field 4 bytes, frozen lapse: stage 4 of the accumulator-free form
   // also prefetches a2 of the first NA2 fields, in the groups (16 lanes, 64
-  // points, one field) of the staging LDS left after the _p/_f groups: more
-  // LDS per block would leave 2 blocks per CU instead of 3 (a 52 KB layout
-  // with every a2 measured stage 4 0.21 ms faster and stages 1-3 0.12 ms
-  // slower at 256^3)
+  // points, one field) of the staging LDS left after the _p/_f groups (a
+  // 52 KB layout with every a2, issued from one loop with _p/_f, measured
+  // stage 4 0.21 ms faster but stages 1-3 0.12 ms slower at 256^3)
   constexpr int LSTAGE = 2 * P64 + P32;
   constexpr bool PF2 = PF && M32 && !LIVE_ALPHA;
   constexpr int CPF = (2 * N32 + 3) / 4;                        // _p/_f chunks of the FP32 fields
