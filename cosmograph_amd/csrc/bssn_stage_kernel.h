@@ -313,6 +313,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
   constexpr int CPF = (2 * N32 + 3) / 4;                        // _p/_f chunks of the FP32 fields
   constexpr int NA2_ = PF2 ? (LSTAGE / (NW * 1024) - N64 - CPF) * 4 : 0;
   constexpr int NA2 = NA2_ < 0 ? 0 : NA2_ > N32 ? N32 : NA2_;   // a2 fields, from chunk N64 + CPF on
+  constexpr int A2F = N32 - NA2;  // the last NA2 fields (the metric's first ones read a3 from memory anyway)
   constexpr int PFR = (N64 + CPF + (NA2 + 3) / 4) * 1024;       // prefetch bytes per wave
   constexpr int LPF = PF ? NW * PFR : 0;
   __shared__ __align__(16) char ldma[DMA ? (LSTAGE > LPF ? LSTAGE : LPF) : 16];
@@ -914,11 +915,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
           if (n < N32 && (!f32 || need_f)) dma16v(b + l32 * 4, reg + (N64 + i) * 1024);
         }
         if (PF2 && a.incr && a.stage == 4) {
-          // a2 of field F32 + 4i + lane/16 in chunk N64 + CPF + i
+          // a2 of field F32 + A2F + 4i + lane/16 in chunk N64 + CPF + i
           const long la = (long)((ln >> 4) * 4 + a.s_a2) * a.rstride + o4;
   #pragma unroll
           for (int i = 0; i < (NA2 + 3) / 4; ++i) {
-            const char* b = (const char*)RS(F32 + 4 * i, 0);
+            const char* b = (const char*)RS(F32 + A2F + 4 * i, 0);
             if (4 * i + (ln >> 4) < NA2) dma16v(b + la * 4, reg + (N64 + CPF + i) * 1024);
           }
         }
@@ -1366,9 +1367,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
         if (MQ && f < 7) return (double)*reinterpret_cast<const int*>(w) * a.msc[f < 7 ? f : 0];
         return (double)*reinterpret_cast<const TS*>(w);
       };
-      // stage 4: a2 of field f (f - F32 < NA2) from the prefetch
+      // stage 4: a2 of field f (f - F32 >= A2F) from the prefetch
       auto la2 = [&](int f) -> double {
-        const int n = f - F32;
+        const int n = f - F32 - A2F;
         const char* w = pfr + (N64 + CPF + (n >> 2)) * 1024 + (n & 3) * 256 + pq * 4;
         if (MQ && f < 7) return (double)*reinterpret_cast<const int*>(w) * a.msc[f < 7 ? f : 0];
         return (double)*reinterpret_cast<const TS*>(w);
@@ -1403,7 +1404,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int t, cons
                 a2v[g] = ldo((const double*)RM(g, a.s_a2), idx);
                 a3m[g] = ldo(INM(g), idx);
               } else {
-                a2v[g] = PF2 && g - F32 < NA2 ? la2(g) : lw(a.s_a2);
+                a2v[g] = PF2 && g - F32 >= A2F ? la2(g) : lw(a.s_a2);
                 // the fixed-point metric's stage-4 input, re-read here rather
                 // than kept in registers through the stencil phase
                 if (MQ && g < 7) a3m[g < 7 ? g : 0] = lw(a.s_in);
